@@ -1,0 +1,189 @@
+/*
+ * tsg -- MI355X (gfx950) tiled FP64 SpGEMM, C = A * B with A, B, C in CSR.
+ *
+ * C ABI of libtsg.so.  No HIP or torch types appear in any signature: device
+ * buffers are plain pointers, streams are passed as void *.
+ *
+ * Arithmetic: every output entry c_ij is accumulated from +0.0 over its
+ * products a_ik * b_kj in ascending position of k within row i of A (A-row
+ * order, as the reference's thmkl_dcsrmultcsr.c:61-72), one fused multiply-add
+ * per product by default (TSG_FP_FUSED) or a rounded product followed by a
+ * rounded add (TSG_FP_SEPARATE).  Given the same mode, results are bitwise
+ * identical to the CPU reference on column-sorted, duplicate-free inputs.
+ * Structural zeros (products that cancel to 0.0) are kept.
+ *
+ * Index types: inputs use int32 row pointers and columns (the reference's
+ * MAT_PTR_TYPE, common.h:15-17); the device API returns int64 row pointers
+ * for C so outputs past 2^31 entries are expressible.  The reference-
+ * compatible host entry thmkl_dcsrmultcsr keeps int32 and reports overflow
+ * through *info.
+ */
+#ifndef TSG_H
+#define TSG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+enum {
+    TSG_OK = 0,
+    TSG_ERR_INVALID = -1,   /* bad argument (negative size, null pointer) */
+    TSG_ERR_HIP = -2,       /* a HIP runtime call failed                 */
+    TSG_ERR_ALLOC = -3,     /* device or host allocation failed          */
+    TSG_ERR_OVERFLOW = -4,  /* nnz(C) does not fit the requested index   */
+    TSG_ERR_STATE = -5,     /* numeric phase without a matching symbolic */
+    TSG_ERR_NODEVICE = -6   /* no usable HIP device                      */
+};
+
+/* ------------------------------------------------------------- methods */
+enum {
+    TSG_METHOD_AUTO = 0,    /* tile path for tile-dense A, row path else */
+    TSG_METHOD_TILE = 1,    /* 16x16 bitmask tiles (TileSpGEMM)          */
+    TSG_METHOD_ROW = 2      /* row-binned hash / bitmap accumulation     */
+};
+
+enum { TSG_FP_FUSED = 0, TSG_FP_SEPARATE = 1 };
+
+typedef struct tsg_handle_s *tsg_handle_t;
+
+/* ====================================================================
+ * Reference-compatible host entry.
+ *
+ * Replaces thmkl_dcsrmultcsr (reference thmkl_dcsrmultcsr.c:7-10), same
+ * argument list and meaning:
+ *   *request == 1  fill ic[0..m] (0-based, ic[m] = nnz(C)); *info = 1
+ *   *request == 0  with ic from the previous call, fill jc[ic[i]..ic[i+1])
+ *                  ascending and c; *info = 0
+ * m = rows of A and C, n = columns of A = rows of B, k = columns of B and C.
+ * trans, sort and nzmax are accepted and ignored, as in the reference.
+ * Host pointers in, host pointers out; the device work runs on device 0 (or
+ * $TSG_DEVICE).  The symbolic plan of the request=1 call is cached and reused
+ * by the request=0 call when (m, n, k, ia, ja, ib, jb) match; otherwise the
+ * symbolic phase is recomputed.
+ * Errors (the reference has none): *info < 0 with a TSG_ERR_* code;
+ * TSG_ERR_OVERFLOW when nnz(C) > INT32_MAX (use tsg_dcsrmultcsr_i64).
+ * ==================================================================== */
+void thmkl_dcsrmultcsr(const char *trans, const int *request, const int *sort,
+                       const int *m, const int *n, const int *k,
+                       double *a, int *ja, int *ia,
+                       double *b, int *jb, int *ib,
+                       double *c, int *jc, int *ic,
+                       const int *nzmax, int *info);
+
+/* Same two-phase contract with an int64 ic (no overflow at nnz(C) >= 2^31).
+ * Returns TSG_OK or a TSG_ERR_* code. */
+int tsg_dcsrmultcsr_i64(int request, int m, int n, int k,
+                        const double *a, const int *ja, const int *ia,
+                        const double *b, const int *jb, const int *ib,
+                        double *c, int *jc, int64_t *ic);
+
+/* ====================================================================
+ * Device API.  All matrix pointers are device pointers on the handle's
+ * device; calls are asynchronous on the handle's stream except where noted.
+ * ==================================================================== */
+int tsg_create(tsg_handle_t *handle, int device);
+int tsg_destroy(tsg_handle_t handle);
+/* stream: a hipStream_t, or NULL for the handle's own stream */
+int tsg_set_stream(tsg_handle_t handle, void *stream);
+int tsg_set_method(tsg_handle_t handle, int method);
+int tsg_set_fp_mode(tsg_handle_t handle, int fp_mode);
+/* Method the last symbolic phase used (TSG_METHOD_TILE or TSG_METHOD_ROW). */
+int tsg_get_method_used(tsg_handle_t handle);
+/* Device workspace currently held by the handle, bytes. */
+size_t tsg_workspace_bytes(tsg_handle_t handle);
+
+/* Symbolic phase (thmkl request=1, device side).  Builds the plan (tile
+ * structures or row bins) kept in the handle, writes C's row pointers d_ic
+ * (m+1 int64) and returns nnz(C) in *nnz_c.  Synchronizes the stream once
+ * (nnz(C) is needed on the host to size jc and c). */
+int tsg_spgemm_symbolic(tsg_handle_t handle, int m, int n, int k,
+                        const int *d_ia, const int *d_ja, int64_t nnz_a,
+                        const int *d_ib, const int *d_jb, int64_t nnz_b,
+                        int64_t *d_ic, int64_t *nnz_c);
+
+/* Numeric phase (thmkl request=0, device side).  Must follow a symbolic
+ * phase on the same handle with the same structure (values may differ, so
+ * one symbolic plan serves any number of numeric calls).  Fills d_jc/d_c
+ * (nnz(C) entries, columns ascending per row).  Asynchronous. */
+int tsg_spgemm_numeric(tsg_handle_t handle, int m, int n, int k,
+                       const int *d_ia, const int *d_ja, const double *d_a,
+                       int64_t nnz_a,
+                       const int *d_ib, const int *d_jb, const double *d_b,
+                       int64_t nnz_b,
+                       const int64_t *d_ic, int *d_jc, double *d_c);
+
+/* Device CSR transpose (reference utils.h:158-195): row j of the output
+ * holds column j of the input, entries by ascending source row.  d_tp has
+ * n+1 entries.  Used for C = A * A^T, whose B the reference builds on the
+ * host (main_spgemm-col.cu:4914-4937). */
+int tsg_csr_transpose(tsg_handle_t handle, int m, int n,
+                      const int *d_ia, const int *d_ja, const double *d_a,
+                      int64_t nnz, int *d_tp, int *d_ti, double *d_tv);
+
+/* Sum over A's entries a_ik of |row k of B| -- the intermediate-product
+ * count (reference main_spgemm-col.cu:4950-4955); flops = 2 * this.
+ * Synchronizes the stream. */
+int tsg_intermediate_products(tsg_handle_t handle, int m, const int *d_ia,
+                              const int *d_ja, int64_t nnz_a, const int *d_ib,
+                              uint64_t *products);
+
+/* ====================================================================
+ * Plan inspection (tile path).  After a symbolic phase with
+ * TSG_METHOD_TILE, copies an internal tile array to host memory so the tile
+ * format and the tile-level steps can be checked against the reference's
+ * CPU restatement.  tsg_plan_array_bytes returns the size (0 if absent).
+ * Synchronizes the stream.
+ * ==================================================================== */
+enum {
+    /* A in tile-CSR (main_spgemm-col.cu:4976-5240) */
+    TSG_A_TILE_PTR = 0,   /* int32[blkm+1]                                 */
+    TSG_A_TILE_COL,       /* int32[numblkA]                                */
+    TSG_A_TILE_OFF,       /* int32[numblkA+1]  payload offsets (nnzb_A)     */
+    TSG_A_MASK,           /* uint16[16*numblkA] row masks, bit 15-c        */
+    TSG_A_ROWPTR,         /* uint8[16*numblkA]  in-tile row starts          */
+    TSG_A_IDX,            /* uint8[nnzA]        (r<<4)|c                    */
+    TSG_A_VAL,            /* double[nnzA]       (after a numeric phase)     */
+    /* B: tile-CSR payload + tile-CSC index (main_spgemm-col.cu:5003-5360) */
+    TSG_B_TILE_PTR,
+    TSG_B_TILE_COL,
+    TSG_B_TILE_OFF,
+    TSG_B_MASK,
+    TSG_B_ROWPTR,
+    TSG_B_IDX,
+    TSG_B_VAL,
+    TSG_B_CSC_PTR,        /* int32[blknB+1]                                */
+    TSG_B_CSC_ROW,        /* int32[numblkB]  tile row K per tile column     */
+    TSG_B_CSC_TILE,       /* int32[numblkB]  tile-CSR position of (K, J)   */
+    /* C tile structure (steps 1-2, main_spgemm-col.cu:997-1325) */
+    TSG_C_TILE_PTR,       /* int32[blkm+1]                                 */
+    TSG_C_TILE_COL,       /* int32[numblkC]                                */
+    TSG_C_MASK,           /* uint16[16*numblkC]                            */
+    TSG_C_ROWPTR,         /* uint8[16*numblkC]                             */
+    TSG_C_TILE_OFF,       /* int64[numblkC+1]  payload offsets (nnzb_C)    */
+    TSG_PLAN_ARRAY_COUNT
+};
+size_t tsg_plan_array_bytes(tsg_handle_t handle, int which);
+int tsg_plan_array_copy(tsg_handle_t handle, int which, void *host_dst,
+                        size_t bytes);
+
+/* Per-stage device time of the last symbolic + numeric phases, measured with
+ * HIP events on the handle's stream when timing is enabled (off by default:
+ * events add a few microseconds).  Stage names are returned through
+ * tsg_stage_name; times in milliseconds. */
+int tsg_enable_timing(tsg_handle_t handle, int enable);
+int tsg_stage_count(tsg_handle_t handle);
+const char *tsg_stage_name(tsg_handle_t handle, int stage);
+double tsg_stage_ms(tsg_handle_t handle, int stage);
+
+/* Human-readable text of the last error on this handle (or globally for
+ * NULL). */
+const char *tsg_last_error(tsg_handle_t handle);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TSG_H */

@@ -1,0 +1,247 @@
+"""GPU parity: the HIP paths (through the C ABI) against the reference's
+fixtures and the CPU oracle.  Structure bit-exact; values bitwise in both FP
+modes (every output is accumulated in A-row order, as the reference's CPU
+SpGEMM does).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import tilespgemm_amd as tsg
+from oracle import py_oracle as O
+from tilespgemm_amd import gen
+from tests.util import assert_csr_equal, golden_names, load_golden
+
+pytestmark = pytest.mark.gpu
+
+METHODS = [("tile", tsg.METHOD_TILE), ("row", tsg.METHOD_ROW)]
+
+
+@pytest.fixture(scope="module")
+def handle():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    h = tsg.Handle(0)
+    yield h
+    h.close()
+
+
+def to_dev(m, n, X):
+    return tsg.DeviceCsr.from_host(m, n, X[0], X[1], X[2])
+
+
+def run(handle, m, n, k, A, B, method, fp=tsg.FP_FUSED, same=False):
+    dA = to_dev(m, n, A)
+    dB = dA if same else to_dev(n, k, B)
+    handle.set_method(method)
+    handle.set_fp_mode(fp)
+    ic, jc, c = handle.spgemm(dA, dB)
+    return ic.cpu().numpy(), jc.cpu().numpy(), c.cpu().numpy()
+
+
+def is_square_aa(g):
+    return (g["m"] == g["n"] == g["k"] and np.array_equal(g["A"][0], g["B"][0])
+            and np.array_equal(g["A"][1], g["B"][1]) and np.array_equal(g["A"][2], g["B"][2]))
+
+
+@pytest.mark.parametrize("name", golden_names())
+@pytest.mark.parametrize("mname,method", METHODS)
+def test_fixture_parity(handle, name, mname, method):
+    g = load_golden(name)
+    same = is_square_aa(g)
+    got = run(handle, g["m"], g["n"], g["k"], g["A"], g["B"], method, tsg.FP_FUSED, same)
+    print(f"[{name} {mname}] nnzC={got[0][-1]} used={handle.method_used}", flush=True)
+    assert handle.method_used == method
+    assert_csr_equal(got, (g["ic"], g["jc"], g["c_fma"]))
+    got = run(handle, g["m"], g["n"], g["k"], g["A"], g["B"], method, tsg.FP_SEPARATE, same)
+    assert_csr_equal(got, (g["ic"], g["jc"], g["c_nofma"]))
+
+
+def _cases():
+    L64 = gen.laplacian2d(64)
+    S12 = gen.stencil27(12, seed=3)
+    E = gen.erdos_renyi(4000, 12.0, seed=21)
+    R12 = gen.rmat(12, 40_000, seed=5)
+    R12T = gen.transpose(4096, 4096, *R12)
+    R14 = gen.rmat(14, 250_000, seed=6)
+    R14T = gen.transpose(16384, 16384, *R14)
+    return [
+        ("lap64_AA", 4096, 4096, 4096, L64, L64, True),
+        ("st27_12_AA", 1728, 1728, 1728, S12, S12, True),
+        ("er4000_AA", 4000, 4000, 4000, E, E, True),
+        ("rmat12_AAT", 4096, 4096, 4096, R12, R12T, False),
+        ("rmat14_AAT", 16384, 16384, 16384, R14, R14T, False),
+    ]
+
+
+@pytest.mark.parametrize("case", _cases(), ids=lambda c: c[0])
+@pytest.mark.parametrize("mname,method", METHODS)
+def test_generated_parity(handle, case, mname, method):
+    name, m, n, k, A, B, same = case
+    want = O.csr_spgemm(m, k, A, B, fused=True)
+    print(f"[{name} {mname}] nnzC={want[0][-1]}", flush=True)
+    got = run(handle, m, n, k, A, B, method, tsg.FP_FUSED, same)
+    assert_csr_equal(got, want)
+
+
+def _wide_case():
+    """Rows with > 1024 products and > 1024 outputs spread over k > 65536
+    columns: big-row symbolic, item splitting by count and by width."""
+    rng = np.random.default_rng(17)
+    m, n, k = 40, 3000, 300_000
+    A = gen.erdos_renyi(m, 400.0, seed=1, ncols=n)
+    rows = np.repeat(np.arange(n), 30)
+    cols = rng.integers(0, k, rows.size)
+    cols[::7] = rng.integers(0, 2000, cols[::7].size)      # a dense column band
+    B = gen.from_coo(n, rows, cols, rng.uniform(-1, 1, rows.size))
+    return m, n, k, A, B
+
+
+def test_row_path_big_rows_and_items(handle):
+    m, n, k, A, B = _wide_case()
+    want = O.csr_spgemm(m, k, A, B, fused=True)
+    print(f"[wide] nnzC={want[0][-1]} max row={np.diff(want[0]).max()}", flush=True)
+    got = run(handle, m, n, k, A, B, tsg.METHOD_ROW)
+    assert_csr_equal(got, want)
+
+
+def test_row_path_more_than_2p20_columns(handle):
+    """k > 2^20: the big-row symbolic bitmap runs in several column passes."""
+    rng = np.random.default_rng(4)
+    m, n, k = 24, 500, 1_500_000
+    A = gen.erdos_renyi(m, 300.0, seed=2, ncols=n)
+    rows = np.repeat(np.arange(n), 20)
+    B = gen.from_coo(n, rows, rng.integers(0, k, rows.size), rng.uniform(-1, 1, rows.size))
+    want = O.csr_spgemm(m, k, A, B, fused=True)
+    got = run(handle, m, n, k, A, B, tsg.METHOD_ROW)
+    assert_csr_equal(got, want)
+
+
+def test_tile_plan_arrays_match_tile_oracle(handle):
+    """The tile path's intermediate arrays equal the restated reference tile
+    format and tile steps (main_spgemm-col.cu:4968-5360, :997-1325)."""
+    A = gen.stencil27(9, seed=2)
+    R = gen.rmat(10, 6000, seed=8)
+    RT = gen.transpose(1024, 1024, *R)
+    for (m, n, k, XA, XB, same) in ((729, 729, 729, A, A, True),
+                                    (1024, 1024, 1024, R, RT, False)):
+        run(handle, m, n, k, XA, XB, tsg.METHOD_TILE, same=same)
+        tA = O.tile_convert(m, n, XA)
+        tB = O.tile_convert(n, k, XB)
+        csc = O.tile_csc(tB)
+        tptr, tcol = O.tile_step1(tA, tB)
+        maskC, rowptrC, offC = O.tile_step2(tA, tB, csc, tptr, tcol)
+        P = handle.plan_array
+        np.testing.assert_array_equal(P("A_TILE_PTR"), tA["tile_ptr"])
+        np.testing.assert_array_equal(P("A_TILE_COL"), tA["tile_col"])
+        np.testing.assert_array_equal(P("A_TILE_OFF"), tA["tile_off"])
+        np.testing.assert_array_equal(P("A_MASK"), tA["mask"])
+        np.testing.assert_array_equal(P("A_ROWPTR"), tA["rowptr"])
+        np.testing.assert_array_equal(P("A_IDX"), tA["idx"])
+        np.testing.assert_array_equal(P("A_VAL").view(np.int64), tA["val"].view(np.int64))
+        np.testing.assert_array_equal(P("B_TILE_PTR"), tB["tile_ptr"])
+        np.testing.assert_array_equal(P("B_MASK"), tB["mask"])
+        np.testing.assert_array_equal(P("B_CSC_PTR"), csc["col_ptr"])
+        np.testing.assert_array_equal(P("B_CSC_ROW"), csc["col_row"])
+        np.testing.assert_array_equal(P("B_CSC_TILE"), csc["col_tile"])
+        np.testing.assert_array_equal(P("C_TILE_PTR"), tptr)
+        np.testing.assert_array_equal(P("C_TILE_COL"), tcol)
+        np.testing.assert_array_equal(P("C_MASK"), maskC)
+        np.testing.assert_array_equal(P("C_ROWPTR"), rowptrC)
+        np.testing.assert_array_equal(P("C_TILE_OFF"), offC)
+
+
+@pytest.mark.parametrize("mname,method", METHODS)
+def test_symbolic_reuse_with_new_values(handle, mname, method):
+    """One symbolic plan serves numeric calls with different values."""
+    import torch
+    S = gen.stencil27(10, seed=1)
+    n = 1000
+    dA = to_dev(n, n, S)
+    handle.set_method(method)
+    handle.set_fp_mode(tsg.FP_FUSED)
+    ic, nnz = handle.symbolic(dA, dA)
+    for seed in (1, 2):
+        vals = np.random.default_rng(seed).standard_normal(S[1].size)
+        dA.data.copy_(torch.from_numpy(vals))
+        jc = torch.empty(nnz, dtype=torch.int32, device="cuda:0")
+        c = torch.empty(nnz, dtype=torch.float64, device="cuda:0")
+        handle.numeric(dA, dA, ic, jc, c)
+        want = O.csr_spgemm(n, n, (S[0], S[1], vals), (S[0], S[1], vals))
+        assert_csr_equal((ic.cpu().numpy(), jc.cpu().numpy(), c.cpu().numpy()), want)
+
+
+@pytest.mark.parametrize("mname,method", METHODS)
+def test_empty_and_degenerate(handle, mname, method):
+    # all-zero matrix
+    Z = (np.zeros(33, np.int32), np.zeros(0, np.int32), np.zeros(0))
+    got = run(handle, 32, 32, 32, Z, Z, method)
+    assert got[0].tolist() == [0] * 33 and got[1].size == 0
+    # 1 x n times n x 1
+    g = load_golden("row_times_col")
+    got = run(handle, 1, 40, 1, g["A"], g["B"], method)
+    assert_csr_equal(got, (g["ic"], g["jc"], g["c_fma"]))
+
+
+def test_unsorted_input_goes_to_row_path(handle):
+    rng = np.random.default_rng(3)
+    A = gen.erdos_renyi(300, 7.0, seed=9)
+    ip, ix, d = (x.copy() for x in A)
+    for i in range(300):                      # reverse every row
+        ix[ip[i]:ip[i + 1]] = ix[ip[i]:ip[i + 1]][::-1].copy()
+        d[ip[i]:ip[i + 1]] = d[ip[i]:ip[i + 1]][::-1].copy()
+    U = (ip, ix, d)
+    want = O.ref_thmkl(300, 300, 300, U, A, "fma") if O.ref_available() else \
+        O.csr_spgemm(300, 300, U, A)
+    got = run(handle, 300, 300, 300, U, A, tsg.METHOD_AUTO)
+    assert handle.method_used == tsg.METHOD_ROW
+    assert_csr_equal(got, want)
+    with pytest.raises(tsg.TsgError):
+        run(handle, 300, 300, 300, U, A, tsg.METHOD_TILE)
+    _ = rng
+
+
+def test_invalid_input_is_rejected(handle):
+    A = gen.erdos_renyi(50, 4.0, seed=1)
+    bad = (A[0], A[1].copy(), A[2])
+    bad[1][3] = 50                            # column out of range
+    with pytest.raises(tsg.TsgError):
+        run(handle, 50, 50, 50, bad, A, tsg.METHOD_AUTO)
+    badp = (A[0].copy(), A[1], A[2])
+    badp[0][10] = badp[0][11] + 5             # non-monotone row pointers
+    with pytest.raises(tsg.TsgError):
+        run(handle, 50, 50, 50, badp, A, tsg.METHOD_AUTO)
+
+
+def test_transpose_matches_host(handle):
+    R = gen.rmat(12, 30_000, seed=2)
+    dR = to_dev(4096, 4096, R)
+    T = handle.transpose(dR)
+    want = gen.transpose(4096, 4096, *R)
+    np.testing.assert_array_equal(T.indptr.cpu().numpy(), want[0])
+    np.testing.assert_array_equal(T.indices.cpu().numpy(), want[1])
+    np.testing.assert_array_equal(T.data.cpu().numpy(), want[2])
+
+
+def test_intermediate_products(handle):
+    S = gen.stencil27(11, seed=0)
+    dS = to_dev(1331, 1331, S)
+    assert handle.intermediate_products(dS, dS) == (9 * 11 - 10) ** 3
+
+
+@pytest.mark.parametrize("name", ["lap16_AA_refvals", "rmat9_AAT", "rect_203x157x91"])
+def test_reference_host_entry(name):
+    """thmkl_dcsrmultcsr: same signature and two-phase contract as the
+    reference (thmkl_dcsrmultcsr.c:7-95), host pointers in and out."""
+    g = load_golden(name)
+    m, n, k = g["m"], g["n"], g["k"]
+    A = g["A"]
+    B = A if is_square_aa(g) else g["B"]
+    ic = tsg.thmkl_dcsrmultcsr(1, m, n, k, A, B)
+    np.testing.assert_array_equal(ic, g["ic"])
+    jc, c = tsg.thmkl_dcsrmultcsr(0, m, n, k, A, B, ic=ic)
+    assert_csr_equal((ic, jc, c), (g["ic"], g["jc"], g["c_fma"]))
+    ic64 = tsg.dcsrmultcsr_i64(1, m, n, k, A, B)
+    jc64, c64 = tsg.dcsrmultcsr_i64(0, m, n, k, A, B, ic=ic64)
+    assert_csr_equal((ic64, jc64, c64), (g["ic"], g["jc"], g["c_fma"]))

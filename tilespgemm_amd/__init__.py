@@ -1,0 +1,328 @@
+"""tilespgemm_amd -- MI355X (gfx950) tiled FP64 SpGEMM behind a C ABI.
+
+Python host side over libtsg.so (ctypes).  torch is used only for device
+memory and streams; every computation runs in the library's HIP kernels, and
+the functions here fail loudly when libtsg.so or a GPU is missing -- there is
+no CPU fallback.
+
+    >>> import tilespgemm_amd as tsg
+    >>> h = tsg.Handle()                       # device 0
+    >>> ic, jc, c = h.spgemm(A, B)             # A, B: tsg.DeviceCsr on cuda:0
+
+Host-pointer, reference-compatible entry (thmkl_dcsrmultcsr.c:7-10):
+
+    >>> ic = tsg.thmkl_dcsrmultcsr(1, m, n, k, A, B)                # request=1
+    >>> jc, c = tsg.thmkl_dcsrmultcsr(0, m, n, k, A, B, ic=ic)      # request=0
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtsg.so")
+
+METHOD_AUTO, METHOD_TILE, METHOD_ROW = 0, 1, 2
+FP_FUSED, FP_SEPARATE = 0, 1
+
+ERRORS = {
+    -1: "TSG_ERR_INVALID", -2: "TSG_ERR_HIP", -3: "TSG_ERR_ALLOC",
+    -4: "TSG_ERR_OVERFLOW", -5: "TSG_ERR_STATE", -6: "TSG_ERR_NODEVICE",
+}
+
+# Plan arrays (include/tsg.h), name -> (enum, numpy dtype)
+PLAN_ARRAYS = {
+    "A_TILE_PTR": (0, np.int32), "A_TILE_COL": (1, np.int32), "A_TILE_OFF": (2, np.int32),
+    "A_MASK": (3, np.uint16), "A_ROWPTR": (4, np.uint8), "A_IDX": (5, np.uint8),
+    "A_VAL": (6, np.float64),
+    "B_TILE_PTR": (7, np.int32), "B_TILE_COL": (8, np.int32), "B_TILE_OFF": (9, np.int32),
+    "B_MASK": (10, np.uint16), "B_ROWPTR": (11, np.uint8), "B_IDX": (12, np.uint8),
+    "B_VAL": (13, np.float64),
+    "B_CSC_PTR": (14, np.int32), "B_CSC_ROW": (15, np.int32), "B_CSC_TILE": (16, np.int32),
+    "C_TILE_PTR": (17, np.int32), "C_TILE_COL": (18, np.int32), "C_MASK": (19, np.uint16),
+    "C_ROWPTR": (20, np.uint8), "C_TILE_OFF": (21, np.int64),
+}
+
+# Every symbol include/tsg.h declares, with its ctypes signature.
+_vp, _i, _i64, _sz = C.c_void_p, C.c_int, C.c_int64, C.c_size_t
+SIGNATURES = {
+    "thmkl_dcsrmultcsr": (None, [_vp] * 17),
+    "tsg_dcsrmultcsr_i64": (_i, [_i, _i, _i, _i] + [_vp] * 9),
+    "tsg_create": (_i, [_vp, _i]),
+    "tsg_destroy": (_i, [_vp]),
+    "tsg_set_stream": (_i, [_vp, _vp]),
+    "tsg_set_method": (_i, [_vp, _i]),
+    "tsg_set_fp_mode": (_i, [_vp, _i]),
+    "tsg_get_method_used": (_i, [_vp]),
+    "tsg_workspace_bytes": (_sz, [_vp]),
+    "tsg_spgemm_symbolic": (_i, [_vp, _i, _i, _i, _vp, _vp, _i64, _vp, _vp, _i64, _vp, _vp]),
+    "tsg_spgemm_numeric": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _i64,
+                                _vp, _vp, _vp]),
+    "tsg_csr_transpose": (_i, [_vp, _i, _i, _vp, _vp, _vp, _i64, _vp, _vp, _vp]),
+    "tsg_intermediate_products": (_i, [_vp, _i, _vp, _vp, _i64, _vp, _vp]),
+    "tsg_plan_array_bytes": (_sz, [_vp, _i]),
+    "tsg_plan_array_copy": (_i, [_vp, _i, _vp, _sz]),
+    "tsg_enable_timing": (_i, [_vp, _i]),
+    "tsg_stage_count": (_i, [_vp]),
+    "tsg_stage_name": (C.c_char_p, [_vp, _i]),
+    "tsg_stage_ms": (C.c_double, [_vp, _i]),
+    "tsg_last_error": (C.c_char_p, [_vp]),
+}
+
+_lib = None
+
+
+class TsgError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def lib_path() -> str:
+    return os.environ.get("TSG_LIB", LIB_PATH)
+
+
+def load():
+    """Load libtsg.so (built by tilespgemm_amd/build.py); raise if absent."""
+    global _lib
+    if _lib is None:
+        path = lib_path()
+        if not os.path.exists(path):
+            raise FileNotFoundError(
+                f"{path} not found: build it with `python -m tilespgemm_amd.build`")
+        lib = C.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def compile_library() -> str:
+    """Build libtsg.so in-tree (tilespgemm_amd/build.py)."""
+    import importlib
+    return importlib.import_module("tilespgemm_amd.build").build()
+
+
+def _check(rc: int, handle=None):
+    if rc != 0:
+        msg = load().tsg_last_error(handle)
+        raise TsgError(rc, msg.decode() if msg else "")
+
+
+# ------------------------------------------------------------ device CSR
+@dataclass
+class DeviceCsr:
+    """CSR matrix resident on the GPU (torch tensors: int32 indptr/indices,
+    float64 data)."""
+    rows: int
+    cols: int
+    indptr: "object"
+    indices: "object"
+    data: "object"
+
+    @property
+    def nnz(self) -> int:
+        return int(self.indices.numel())
+
+    @classmethod
+    def from_host(cls, rows, cols, indptr, indices, data, device="cuda:0"):
+        import torch
+        indptr = np.asarray(indptr)
+        if indptr[-1] >= 2**31:
+            raise TsgError(-4, "input nnz must fit int32")
+        return cls(rows, cols,
+                   torch.from_numpy(np.ascontiguousarray(indptr, dtype=np.int32)).to(device),
+                   torch.from_numpy(np.ascontiguousarray(indices, dtype=np.int32)).to(device),
+                   torch.from_numpy(np.ascontiguousarray(data, dtype=np.float64)).to(device))
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else C.c_void_p(0)
+
+
+class Handle:
+    """A tsg handle on one device.  Work is issued on torch's current stream
+    of that device (or, for torch's default stream, on a blocking stream that
+    orders with it), so results order naturally with torch operations."""
+
+    def __init__(self, device: int = 0, method: int = METHOD_AUTO,
+                 fp_mode: int = FP_FUSED, timing: bool = False):
+        import torch
+        self._torch = torch
+        self.device = device
+        h = C.c_void_p()
+        _check(load().tsg_create(C.byref(h), device))
+        self._h = h
+        with torch.cuda.device(device):
+            self.stream = torch.cuda.current_stream()
+        # torch's default stream is the null stream; the handle's own stream is
+        # a blocking stream that orders with it.  Any other torch stream is
+        # used directly.
+        if self.stream.cuda_stream:
+            _check(load().tsg_set_stream(self._h, C.c_void_p(self.stream.cuda_stream)), self._h)
+        self.set_method(method)
+        _check(load().tsg_set_fp_mode(self._h, fp_mode), self._h)
+        if timing:
+            _check(load().tsg_enable_timing(self._h, 1), self._h)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().tsg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_method(self, method: int):
+        _check(load().tsg_set_method(self._h, method), self._h)
+
+    def set_fp_mode(self, fp_mode: int):
+        _check(load().tsg_set_fp_mode(self._h, fp_mode), self._h)
+
+    @property
+    def method_used(self) -> int:
+        return load().tsg_get_method_used(self._h)
+
+    @property
+    def workspace_bytes(self) -> int:
+        return int(load().tsg_workspace_bytes(self._h))
+
+    def symbolic(self, A: DeviceCsr, B: DeviceCsr, ic=None):
+        """Row pointers of C = A*B (int64, m+1) and nnz(C)."""
+        torch = self._torch
+        if A.cols != B.rows:
+            raise TsgError(-1, "A.cols != B.rows")
+        if ic is None:
+            ic = torch.empty(A.rows + 1, dtype=torch.int64, device=A.indptr.device)
+        nnz = C.c_int64(0)
+        _check(load().tsg_spgemm_symbolic(
+            self._h, A.rows, A.cols, B.cols, _ptr(A.indptr), _ptr(A.indices), A.nnz,
+            _ptr(B.indptr), _ptr(B.indices), B.nnz, _ptr(ic), C.byref(nnz)), self._h)
+        return ic, int(nnz.value)
+
+    def numeric(self, A: DeviceCsr, B: DeviceCsr, ic, jc, c):
+        _check(load().tsg_spgemm_numeric(
+            self._h, A.rows, A.cols, B.cols, _ptr(A.indptr), _ptr(A.indices), _ptr(A.data),
+            A.nnz, _ptr(B.indptr), _ptr(B.indices), _ptr(B.data), B.nnz, _ptr(ic), _ptr(jc),
+            _ptr(c)), self._h)
+
+    def spgemm(self, A: DeviceCsr, B: DeviceCsr):
+        """C = A*B on the device; returns (ic int64, jc int32, c float64)."""
+        torch = self._torch
+        ic, nnz = self.symbolic(A, B)
+        dev = A.indptr.device
+        jc = torch.empty(max(nnz, 1), dtype=torch.int32, device=dev)
+        c = torch.empty(max(nnz, 1), dtype=torch.float64, device=dev)
+        self.numeric(A, B, ic, jc, c)
+        return ic, jc[:nnz], c[:nnz]
+
+    def transpose(self, A: DeviceCsr) -> DeviceCsr:
+        """A^T as a DeviceCsr (rows by ascending source row)."""
+        torch = self._torch
+        dev = A.indptr.device
+        tp = torch.empty(A.cols + 1, dtype=torch.int32, device=dev)
+        ti = torch.empty(max(A.nnz, 1), dtype=torch.int32, device=dev)
+        tv = torch.empty(max(A.nnz, 1), dtype=torch.float64, device=dev)
+        _check(load().tsg_csr_transpose(self._h, A.rows, A.cols, _ptr(A.indptr),
+                                        _ptr(A.indices), _ptr(A.data), A.nnz, _ptr(tp),
+                                        _ptr(ti), _ptr(tv)), self._h)
+        return DeviceCsr(A.cols, A.rows, tp, ti[:A.nnz], tv[:A.nnz])
+
+    def intermediate_products(self, A: DeviceCsr, B: DeviceCsr) -> int:
+        out = C.c_uint64(0)
+        _check(load().tsg_intermediate_products(self._h, A.rows, _ptr(A.indptr),
+                                                _ptr(A.indices), A.nnz, _ptr(B.indptr),
+                                                C.byref(out)), self._h)
+        return int(out.value)
+
+    def plan_array(self, name: str) -> np.ndarray:
+        which, dt = PLAN_ARRAYS[name]
+        nbytes = int(load().tsg_plan_array_bytes(self._h, which))
+        out = np.empty(nbytes // np.dtype(dt).itemsize, dtype=dt)
+        if nbytes:
+            _check(load().tsg_plan_array_copy(self._h, which, out.ctypes.data_as(C.c_void_p),
+                                              nbytes), self._h)
+        return out
+
+    def stage_times(self) -> dict:
+        L = load()
+        n = L.tsg_stage_count(self._h)
+        out = {}
+        for i in range(n):
+            name = L.tsg_stage_name(self._h, i).decode()
+            out[name] = out.get(name, 0.0) + L.tsg_stage_ms(self._h, i)
+        return out
+
+
+# ----------------------------------------------------- host-pointer entries
+def thmkl_dcsrmultcsr(request: int, m: int, n: int, k: int, A, B, ic=None):
+    """Reference-compatible two-phase call on host arrays (int32 ic).
+
+    request=1 -> returns ic (int32, m+1); request=0 -> returns (jc, c).
+    Raises TsgError when *info < 0."""
+    L = load()
+    ia = np.ascontiguousarray(A[0], dtype=np.int32)
+    ja = np.ascontiguousarray(A[1], dtype=np.int32)
+    a = np.ascontiguousarray(A[2], dtype=np.float64)
+    same = B is A
+    ib = ia if same else np.ascontiguousarray(B[0], dtype=np.int32)
+    jb = ja if same else np.ascontiguousarray(B[1], dtype=np.int32)
+    b = a if same else np.ascontiguousarray(B[2], dtype=np.float64)
+    cm, cn, ck = C.c_int(m), C.c_int(n), C.c_int(k)
+    req, srt, nzmax, info = C.c_int(request), C.c_int(1), C.c_int(0), C.c_int(0)
+    P = lambda x: x.ctypes.data_as(C.c_void_p)  # noqa: E731
+    if request == 1:
+        ic = np.zeros(m + 1, dtype=np.int32)
+        jc = np.zeros(1, dtype=np.int32)
+        c = np.zeros(1, dtype=np.float64)
+    else:
+        ic = np.ascontiguousarray(ic, dtype=np.int32)
+        nnz = int(ic[-1])
+        jc = np.zeros(max(nnz, 1), dtype=np.int32)
+        c = np.zeros(max(nnz, 1), dtype=np.float64)
+    L.thmkl_dcsrmultcsr(b"n", C.byref(req), C.byref(srt), C.byref(cm), C.byref(cn),
+                        C.byref(ck), P(a), P(ja), P(ia), P(b), P(jb), P(ib), P(c), P(jc),
+                        P(ic), C.byref(nzmax), C.byref(info))
+    if info.value < 0:
+        raise TsgError(info.value, (L.tsg_last_error(None) or b"").decode())
+    if request == 1:
+        return ic
+    nnz = int(ic[-1])
+    return jc[:nnz], c[:nnz]
+
+
+def dcsrmultcsr_i64(request: int, m: int, n: int, k: int, A, B, ic=None):
+    """As thmkl_dcsrmultcsr with an int64 ic (nnz(C) may exceed 2^31-1)."""
+    L = load()
+    ia = np.ascontiguousarray(A[0], dtype=np.int32)
+    ja = np.ascontiguousarray(A[1], dtype=np.int32)
+    a = np.ascontiguousarray(A[2], dtype=np.float64)
+    same = B is A
+    ib = ia if same else np.ascontiguousarray(B[0], dtype=np.int32)
+    jb = ja if same else np.ascontiguousarray(B[1], dtype=np.int32)
+    b = a if same else np.ascontiguousarray(B[2], dtype=np.float64)
+    P = lambda x: x.ctypes.data_as(C.c_void_p)  # noqa: E731
+    if request == 1:
+        ic = np.zeros(m + 1, dtype=np.int64)
+        jc = np.zeros(1, dtype=np.int32)
+        c = np.zeros(1, dtype=np.float64)
+    else:
+        ic = np.ascontiguousarray(ic, dtype=np.int64)
+        nnz = int(ic[-1])
+        jc = np.zeros(max(nnz, 1), dtype=np.int32)
+        c = np.zeros(max(nnz, 1), dtype=np.float64)
+    rc = L.tsg_dcsrmultcsr_i64(request, m, n, k, P(a), P(ja), P(ia), P(b), P(jb), P(ib),
+                               P(c), P(jc), P(ic))
+    _check(rc)
+    if request == 1:
+        return ic
+    nnz = int(ic[-1])
+    return jc[:nnz], c[:nnz]

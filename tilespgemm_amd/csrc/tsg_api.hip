@@ -1,0 +1,496 @@
+// C ABI of libtsg.so: handles, the device two-phase SpGEMM, the
+// reference-compatible host entry thmkl_dcsrmultcsr, plan inspection.
+#include "tsg_internal.h"
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <mutex>
+
+int tile_count_only(const CsrIn &X, Arena &ws, hipStream_t s);   // tsg_tile.hip
+
+struct tsg_handle_s {
+    int device = 0;
+    int cus = 256;
+    hipStream_t stream = nullptr;
+    hipStream_t own_stream = nullptr;
+    int method = TSG_METHOD_AUTO;
+    int fp_mode = TSG_FP_FUSED;
+    int method_used = TSG_METHOD_AUTO;
+    Arena ws;
+    RowPlan row;
+    TilePlan tile;
+    StageTimer tm;
+    std::string err;
+};
+
+namespace {
+
+std::mutex g_err_mu;
+std::string g_err;
+
+void set_global_error(const std::string &e) {
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    g_err = e;
+}
+
+// Makes the handle's device current for the duration of a call.
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) TSG_HIP_TRY(hipSetDevice(dev));
+    }
+    ~DeviceGuard() {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+template <class F>
+int guarded(tsg_handle_t h, F &&f) {
+    try {
+        f();
+        if (h) h->err.clear();
+        return TSG_OK;
+    } catch (const TsgError &e) {
+        if (h) h->err = e.msg;
+        set_global_error(e.msg);
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        if (h) h->err = "host allocation failed";
+        set_global_error("host allocation failed");
+        return TSG_ERR_ALLOC;
+    } catch (...) {
+        if (h) h->err = "unknown error";
+        set_global_error("unknown error");
+        return TSG_ERR_INVALID;
+    }
+}
+
+void check_csr_args(int m, int n, const int *ia, const int *ja, int64_t nnz, const char *what) {
+    TSG_CHECK(m >= 0 && n >= 0 && nnz >= 0, TSG_ERR_INVALID,
+              std::string(what) + ": negative size");
+    TSG_CHECK(ia != nullptr, TSG_ERR_INVALID, std::string(what) + ": null row pointer");
+    TSG_CHECK(nnz == 0 || ja != nullptr, TSG_ERR_INVALID, std::string(what) + ": null columns");
+    TSG_CHECK(nnz < (int64_t(1) << 31), TSG_ERR_OVERFLOW,
+              std::string(what) + ": inputs are limited to 2^31-1 entries (int32 indices)");
+}
+
+int choose_method(tsg_handle_t h, const CsrIn &A, bool sorted) {
+    if (h->method == TSG_METHOD_TILE)
+        TSG_CHECK(sorted, TSG_ERR_INVALID,
+                  "the tile path needs column-sorted, duplicate-free rows");
+    if (h->method != TSG_METHOD_AUTO) return h->method;
+    if (!sorted) return TSG_METHOD_ROW;
+    if (A.nnz == 0 || A.rows == 0) return TSG_METHOD_ROW;
+    // Tile path when A's 16x16 tiles hold enough entries to amortize the
+    // per-tile metadata (~60 bytes per tile), row path otherwise.
+    const int nt = tile_count_only(A, h->ws, h->stream);
+    const double per_tile = nt > 0 ? (double)A.nnz / nt : 0.0;
+    return per_tile >= 6.0 ? TSG_METHOD_TILE : TSG_METHOD_ROW;
+}
+
+}  // namespace
+
+int device_cu_count(int device) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return 256;
+    return prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+}
+
+extern "C" {
+
+int tsg_create(tsg_handle_t *handle, int device) {
+    if (!handle) return TSG_ERR_INVALID;
+    *handle = nullptr;
+    return guarded(nullptr, [&] {
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+            (void)hipGetLastError();
+            throw TsgError{TSG_ERR_NODEVICE, "no HIP device available"};
+        }
+        TSG_CHECK(device >= 0 && device < ndev, TSG_ERR_INVALID, "device index out of range");
+        std::unique_ptr<tsg_handle_s> h(new tsg_handle_s());
+        h->device = device;
+        DeviceGuard dg(device);
+        h->cus = device_cu_count(device);
+        // A blocking stream: it orders with the null stream, which is where
+        // callers that never create streams (and torch's default stream) work.
+        TSG_HIP_TRY(hipStreamCreate(&h->own_stream));
+        h->stream = h->own_stream;
+        h->tm.stream = h->stream;
+        *handle = h.release();
+    });
+}
+
+int tsg_destroy(tsg_handle_t h) {
+    if (!h) return TSG_OK;
+    {
+        DeviceGuard dg(h->device);
+        if (h->stream) (void)hipStreamSynchronize(h->stream);
+        h->tm.clear();
+        h->ws.release();
+        if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
+    }
+    delete h;
+    return TSG_OK;
+}
+
+int tsg_set_stream(tsg_handle_t h, void *stream) {
+    if (!h) return TSG_ERR_INVALID;
+    h->stream = stream ? static_cast<hipStream_t>(stream) : h->own_stream;
+    h->tm.stream = h->stream;
+    return TSG_OK;
+}
+
+int tsg_set_method(tsg_handle_t h, int method) {
+    if (!h || method < TSG_METHOD_AUTO || method > TSG_METHOD_ROW) return TSG_ERR_INVALID;
+    h->method = method;
+    return TSG_OK;
+}
+
+int tsg_set_fp_mode(tsg_handle_t h, int fp_mode) {
+    if (!h || (fp_mode != TSG_FP_FUSED && fp_mode != TSG_FP_SEPARATE)) return TSG_ERR_INVALID;
+    h->fp_mode = fp_mode;
+    return TSG_OK;
+}
+
+int tsg_get_method_used(tsg_handle_t h) { return h ? h->method_used : TSG_ERR_INVALID; }
+
+size_t tsg_workspace_bytes(tsg_handle_t h) { return h ? h->ws.held() : 0; }
+
+int tsg_enable_timing(tsg_handle_t h, int enable) {
+    if (!h) return TSG_ERR_INVALID;
+    h->tm.enabled = enable != 0;
+    return TSG_OK;
+}
+
+int tsg_stage_count(tsg_handle_t h) {
+    if (!h) return 0;
+    if (h->tm.ms.size() != h->tm.names.size()) {
+        if (guarded(h, [&] { DeviceGuard dg(h->device); h->tm.finalize(); }) != TSG_OK) return 0;
+    }
+    return (int)h->tm.names.size();
+}
+
+const char *tsg_stage_name(tsg_handle_t h, int i) {
+    if (!h || i < 0 || i >= (int)h->tm.names.size()) return "";
+    return h->tm.names[i].c_str();
+}
+
+double tsg_stage_ms(tsg_handle_t h, int i) {
+    if (!h || i < 0) return -1.0;
+    if (h->tm.ms.size() != h->tm.names.size()) tsg_stage_count(h);
+    if (i >= (int)h->tm.ms.size()) return -1.0;
+    return h->tm.ms[i];
+}
+
+const char *tsg_last_error(tsg_handle_t h) {
+    if (h) return h->err.c_str();
+    static thread_local std::string copy;
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    copy = g_err;
+    return copy.c_str();
+}
+
+int tsg_spgemm_symbolic(tsg_handle_t h, int m, int n, int k, const int *d_ia, const int *d_ja,
+                        int64_t nnz_a, const int *d_ib, const int *d_jb, int64_t nnz_b,
+                        int64_t *d_ic, int64_t *nnz_c) {
+    if (!h) return TSG_ERR_INVALID;
+    return guarded(h, [&] {
+        check_csr_args(m, n, d_ia, d_ja, nnz_a, "A");
+        check_csr_args(n, k, d_ib, d_jb, nnz_b, "B");
+        TSG_CHECK(d_ic && nnz_c, TSG_ERR_INVALID, "null output");
+        DeviceGuard dg(h->device);
+        h->ws.reset();
+        h->tm.clear();
+        h->tm.stream = h->stream;
+        h->row.valid = false;
+        h->tile.valid = false;
+        CsrIn A{m, n, d_ia, d_ja, nullptr, nnz_a};
+        CsrIn B{n, k, d_ib, d_jb, nullptr, nnz_b};
+        const int vf = validate_csr_pair(A, B, h->ws, h->stream);
+        TSG_CHECK((vf & 0x303) == 0, TSG_ERR_INVALID,
+                  std::string("invalid CSR input: ") + ((vf & 0x3) ? "A" : "B") +
+                      ((vf & 0x101) ? " has bad row pointers" : " has a column out of range"));
+        const int method = choose_method(h, A, (vf & 0x404) == 0);
+        h->method_used = method;
+        if (method == TSG_METHOD_TILE) {
+            const bool b_is_a = d_ia == d_ib && d_ja == d_jb && m == n && n == k;
+            tile_symbolic(h->tile, A, B, b_is_a, d_ic, nnz_c, h->ws, h->stream, h->tm, h->cus);
+        } else {
+            row_symbolic(h->row, A, B, d_ic, nnz_c, h->ws, h->stream, h->tm, h->cus);
+        }
+    });
+}
+
+int tsg_spgemm_numeric(tsg_handle_t h, int m, int n, int k, const int *d_ia, const int *d_ja,
+                       const double *d_a, int64_t nnz_a, const int *d_ib, const int *d_jb,
+                       const double *d_b, int64_t nnz_b, const int64_t *d_ic, int *d_jc,
+                       double *d_c) {
+    if (!h) return TSG_ERR_INVALID;
+    return guarded(h, [&] {
+        check_csr_args(m, n, d_ia, d_ja, nnz_a, "A");
+        check_csr_args(n, k, d_ib, d_jb, nnz_b, "B");
+        TSG_CHECK(nnz_a == 0 || d_a, TSG_ERR_INVALID, "null A values");
+        TSG_CHECK(nnz_b == 0 || d_b, TSG_ERR_INVALID, "null B values");
+        DeviceGuard dg(h->device);
+        CsrIn A{m, n, d_ia, d_ja, d_a, nnz_a};
+        CsrIn B{n, k, d_ib, d_jb, d_b, nnz_b};
+        const int fused = h->fp_mode == TSG_FP_FUSED;
+        if (h->method_used == TSG_METHOD_TILE) {
+            TSG_CHECK(h->tile.valid, TSG_ERR_STATE, "numeric phase without a symbolic phase");
+            tile_numeric(h->tile, A, B, d_ic, d_jc, d_c, fused, h->ws, h->stream, h->tm, h->cus);
+        } else {
+            TSG_CHECK(h->row.valid, TSG_ERR_STATE, "numeric phase without a symbolic phase");
+            row_numeric(h->row, A, B, d_ic, d_jc, d_c, fused, h->ws, h->stream, h->tm, h->cus);
+        }
+    });
+}
+
+int tsg_csr_transpose(tsg_handle_t h, int m, int n, const int *d_ia, const int *d_ja,
+                      const double *d_a, int64_t nnz, int *d_tp, int *d_ti, double *d_tv) {
+    if (!h) return TSG_ERR_INVALID;
+    return guarded(h, [&] {
+        check_csr_args(m, n, d_ia, d_ja, nnz, "A");
+        TSG_CHECK(d_tp && (nnz == 0 || d_ti), TSG_ERR_INVALID, "null output");
+        DeviceGuard dg(h->device);
+        Arena tmp;
+        csr_transpose(m, n, d_ia, d_ja, d_a, nnz, d_tp, d_ti, d_a ? d_tv : nullptr, tmp,
+                      h->stream);
+        TSG_HIP_TRY(hipStreamSynchronize(h->stream));
+    });
+}
+
+int tsg_intermediate_products(tsg_handle_t h, int m, const int *d_ia, const int *d_ja,
+                              int64_t nnz_a, const int *d_ib, uint64_t *products) {
+    if (!h || !products) return TSG_ERR_INVALID;
+    return guarded(h, [&] {
+        check_csr_args(m, 1, d_ia, d_ja, nnz_a, "A");
+        DeviceGuard dg(h->device);
+        Arena tmp;
+        int64_t *rp = tmp.alloc_n<int64_t>((size_t)m + 1);
+        int64_t *sc = tmp.alloc_n<int64_t>(scan_tmp_elems((size_t)m + 1));
+        csr_row_products(m, d_ia, d_ja, d_ib, rp, h->stream);
+        exclusive_scan_i64(rp, rp, (size_t)m, sc, h->stream);
+        int64_t tot = 0;
+        TSG_HIP_TRY(hipMemcpyAsync(&tot, rp + m, sizeof(int64_t), hipMemcpyDeviceToHost,
+                                   h->stream));
+        TSG_HIP_TRY(hipStreamSynchronize(h->stream));
+        *products = (uint64_t)tot;
+    });
+}
+
+// ------------------------------------------------------- plan inspection
+static bool plan_array(tsg_handle_t h, int which, const void **ptr, size_t *bytes) {
+    const TilePlan &p = h->tile;
+    if (!p.valid) return false;
+    const TileMat &A = p.A, &B = p.B;
+    const size_t nC = (size_t)p.numblkC;
+    switch (which) {
+    case TSG_A_TILE_PTR: *ptr = A.tile_ptr; *bytes = 4 * ((size_t)A.blkm + 1); return true;
+    case TSG_A_TILE_COL: *ptr = A.tile_col; *bytes = 4 * (size_t)A.numblk; return true;
+    case TSG_A_TILE_OFF: *ptr = A.tile_off; *bytes = 4 * ((size_t)A.numblk + 1); return true;
+    case TSG_A_MASK: *ptr = A.mask; *bytes = 2 * 16 * (size_t)A.numblk; return true;
+    case TSG_A_ROWPTR: *ptr = A.rowptr; *bytes = 16 * (size_t)A.numblk; return true;
+    case TSG_A_IDX: *ptr = A.idx; *bytes = (size_t)A.nnz; return true;
+    case TSG_A_VAL: *ptr = A.val; *bytes = 8 * (size_t)A.nnz; return true;
+    case TSG_B_TILE_PTR: *ptr = B.tile_ptr; *bytes = 4 * ((size_t)B.blkm + 1); return true;
+    case TSG_B_TILE_COL: *ptr = B.tile_col; *bytes = 4 * (size_t)B.numblk; return true;
+    case TSG_B_TILE_OFF: *ptr = B.tile_off; *bytes = 4 * ((size_t)B.numblk + 1); return true;
+    case TSG_B_MASK: *ptr = B.mask; *bytes = 2 * 16 * (size_t)B.numblk; return true;
+    case TSG_B_ROWPTR: *ptr = B.rowptr; *bytes = 16 * (size_t)B.numblk; return true;
+    case TSG_B_IDX: *ptr = B.idx; *bytes = (size_t)B.nnz; return true;
+    case TSG_B_VAL: *ptr = B.val; *bytes = 8 * (size_t)B.nnz; return true;
+    case TSG_B_CSC_PTR: *ptr = B.csc_ptr; *bytes = 4 * ((size_t)B.blkn + 1); return true;
+    case TSG_B_CSC_ROW: *ptr = B.csc_row; *bytes = 4 * (size_t)B.numblk; return true;
+    case TSG_B_CSC_TILE: *ptr = B.csc_tile; *bytes = 4 * (size_t)B.numblk; return true;
+    case TSG_C_TILE_PTR: *ptr = p.tptrC; *bytes = 4 * ((size_t)A.blkm + 1); return true;
+    case TSG_C_TILE_COL: *ptr = p.tcolC; *bytes = 4 * nC; return true;
+    case TSG_C_MASK: *ptr = p.maskC; *bytes = 2 * 16 * nC; return true;
+    case TSG_C_ROWPTR: *ptr = p.rowptrC; *bytes = 16 * nC; return true;
+    case TSG_C_TILE_OFF: *ptr = p.offC; *bytes = 8 * (nC + 1); return true;
+    default: return false;
+    }
+}
+
+size_t tsg_plan_array_bytes(tsg_handle_t h, int which) {
+    if (!h) return 0;
+    const void *p;
+    size_t b;
+    return plan_array(h, which, &p, &b) ? b : 0;
+}
+
+int tsg_plan_array_copy(tsg_handle_t h, int which, void *dst, size_t bytes) {
+    if (!h || !dst) return TSG_ERR_INVALID;
+    return guarded(h, [&] {
+        const void *p;
+        size_t b;
+        TSG_CHECK(plan_array(h, which, &p, &b), TSG_ERR_STATE, "no such plan array");
+        TSG_CHECK(bytes >= b, TSG_ERR_INVALID, "destination too small");
+        DeviceGuard dg(h->device);
+        TSG_HIP_TRY(hipStreamSynchronize(h->stream));
+        if (b) TSG_HIP_TRY(hipMemcpy(dst, p, b, hipMemcpyDeviceToHost));
+    });
+}
+
+}  // extern "C"
+
+// =================================================================
+// Host-pointer entries.  One process-wide context per device holds the
+// device copies and the plan between the request=1 and request=0 calls.
+namespace {
+
+struct HostCtx {
+    std::mutex mu;
+    tsg_handle_t h = nullptr;
+    Arena buf;
+    // identity of the cached symbolic phase
+    bool have = false;
+    int m = -1, n = -1, k = -1;
+    const void *ia = nullptr, *ja = nullptr, *ib = nullptr, *jb = nullptr;
+    int64_t nnz_a = -1, nnz_b = -1, nnz_c = -1;
+    int *d_ia = nullptr, *d_ja = nullptr, *d_ib = nullptr, *d_jb = nullptr;
+    double *d_a = nullptr, *d_b = nullptr;
+    int64_t *d_ic = nullptr;
+};
+
+HostCtx &host_ctx() {
+    static HostCtx ctx;
+    return ctx;
+}
+
+int env_device() {
+    const char *e = std::getenv("TSG_DEVICE");
+    return e ? std::atoi(e) : 0;
+}
+
+// Uploads the inputs, runs the symbolic phase and remembers its identity.
+void host_symbolic(HostCtx &x, int m, int n, int k, const int *ia, const int *ja,
+                   const int *ib, const int *jb) {
+    if (!x.h) {
+        int rc = tsg_create(&x.h, env_device());
+        if (rc != TSG_OK) throw TsgError{rc, tsg_last_error(nullptr)};
+    }
+    TSG_CHECK(m >= 0 && n >= 0 && k >= 0 && ia && ib, TSG_ERR_INVALID, "bad arguments");
+    DeviceGuard dg(x.h->device);
+    const int64_t nnz_a = ia[m] - ia[0], nnz_b = ib[n] - ib[0];
+    TSG_CHECK(ia[0] == 0 && ib[0] == 0, TSG_ERR_INVALID, "row pointers must start at 0");
+    x.have = false;
+    x.buf.reset();
+    x.d_ia = x.buf.alloc_n<int>((size_t)m + 1);
+    x.d_ja = x.buf.alloc_n<int>((size_t)nnz_a + 1);
+    x.d_a = x.buf.alloc_n<double>((size_t)nnz_a + 1);
+    const bool same = ia == ib && ja == jb && m == n;
+    if (same) {
+        x.d_ib = x.d_ia;
+        x.d_jb = x.d_ja;
+        x.d_b = x.d_a;
+    } else {
+        x.d_ib = x.buf.alloc_n<int>((size_t)n + 1);
+        x.d_jb = x.buf.alloc_n<int>((size_t)nnz_b + 1);
+        x.d_b = x.buf.alloc_n<double>((size_t)nnz_b + 1);
+    }
+    x.d_ic = x.buf.alloc_n<int64_t>((size_t)m + 1);
+    hipStream_t s = x.h->stream;
+    TSG_HIP_TRY(hipMemcpyAsync(x.d_ia, ia, sizeof(int) * ((size_t)m + 1), hipMemcpyHostToDevice, s));
+    if (nnz_a)
+        TSG_HIP_TRY(hipMemcpyAsync(x.d_ja, ja, sizeof(int) * nnz_a, hipMemcpyHostToDevice, s));
+    if (!same) {
+        TSG_HIP_TRY(hipMemcpyAsync(x.d_ib, ib, sizeof(int) * ((size_t)n + 1), hipMemcpyHostToDevice, s));
+        if (nnz_b)
+            TSG_HIP_TRY(hipMemcpyAsync(x.d_jb, jb, sizeof(int) * nnz_b, hipMemcpyHostToDevice, s));
+    }
+    int64_t nnz_c = 0;
+    int rc = tsg_spgemm_symbolic(x.h, m, n, k, x.d_ia, x.d_ja, nnz_a, x.d_ib, x.d_jb, nnz_b,
+                                 x.d_ic, &nnz_c);
+    if (rc != TSG_OK) throw TsgError{rc, tsg_last_error(x.h)};
+    x.have = true;
+    x.m = m; x.n = n; x.k = k;
+    x.ia = ia; x.ja = ja; x.ib = ib; x.jb = jb;
+    x.nnz_a = nnz_a; x.nnz_b = nnz_b; x.nnz_c = nnz_c;
+}
+
+bool host_matches(const HostCtx &x, int m, int n, int k, const int *ia, const int *ja,
+                  const int *ib, const int *jb) {
+    return x.have && x.m == m && x.n == n && x.k == k && x.ia == ia && x.ja == ja &&
+           x.ib == ib && x.jb == jb && x.nnz_a == (int64_t)(ia[m] - ia[0]) &&
+           x.nnz_b == (int64_t)(ib[n] - ib[0]);
+}
+
+// Numeric phase from host buffers into host buffers.
+void host_numeric(HostCtx &x, const double *a, const double *b, int *jc, double *c) {
+    DeviceGuard dg(x.h->device);
+    hipStream_t s = x.h->stream;
+    if (x.nnz_a)
+        TSG_HIP_TRY(hipMemcpyAsync(x.d_a, a, sizeof(double) * x.nnz_a, hipMemcpyHostToDevice, s));
+    if (x.d_b != x.d_a && x.nnz_b)
+        TSG_HIP_TRY(hipMemcpyAsync(x.d_b, b, sizeof(double) * x.nnz_b, hipMemcpyHostToDevice, s));
+    int *d_jc = x.buf.alloc_n<int>((size_t)x.nnz_c + 1);
+    double *d_c = x.buf.alloc_n<double>((size_t)x.nnz_c + 1);
+    int rc = tsg_spgemm_numeric(x.h, x.m, x.n, x.k, x.d_ia, x.d_ja, x.d_a, x.nnz_a, x.d_ib,
+                                x.d_jb, x.d_b, x.nnz_b, x.d_ic, d_jc, d_c);
+    if (rc != TSG_OK) throw TsgError{rc, tsg_last_error(x.h)};
+    if (x.nnz_c) {
+        TSG_HIP_TRY(hipMemcpyAsync(jc, d_jc, sizeof(int) * x.nnz_c, hipMemcpyDeviceToHost, s));
+        TSG_HIP_TRY(hipMemcpyAsync(c, d_c, sizeof(double) * x.nnz_c, hipMemcpyDeviceToHost, s));
+    }
+    TSG_HIP_TRY(hipStreamSynchronize(s));
+}
+
+template <class IC>
+int host_entry(int request, int m, int n, int k, const double *a, const int *ja, const int *ia,
+               const double *b, const int *jb, const int *ib, double *c, int *jc, IC *ic) {
+    HostCtx &x = host_ctx();
+    std::lock_guard<std::mutex> lk(x.mu);
+    return guarded(nullptr, [&] {
+        TSG_CHECK(request == 0 || request == 1, TSG_ERR_INVALID, "request must be 0 or 1");
+        TSG_CHECK(ic != nullptr, TSG_ERR_INVALID, "null ic");
+        if (request == 1) {
+            host_symbolic(x, m, n, k, ia, ja, ib, jb);
+            if (sizeof(IC) == 4)
+                TSG_CHECK(x.nnz_c <= 0x7fffffffLL, TSG_ERR_OVERFLOW,
+                          "nnz(C) exceeds int32; use tsg_dcsrmultcsr_i64");
+            std::vector<int64_t> tmp((size_t)m + 1);
+            DeviceGuard dg(x.h->device);
+            TSG_HIP_TRY(hipMemcpy(tmp.data(), x.d_ic, sizeof(int64_t) * ((size_t)m + 1),
+                                  hipMemcpyDeviceToHost));
+            for (int i = 0; i <= m; i++) ic[i] = (IC)tmp[i];
+            return;
+        }
+        if (!host_matches(x, m, n, k, ia, ja, ib, jb)) host_symbolic(x, m, n, k, ia, ja, ib, jb);
+        TSG_CHECK((int64_t)ic[m] == x.nnz_c && ic[0] == 0, TSG_ERR_STATE,
+                  "ic does not match the symbolic phase of these inputs");
+        TSG_CHECK(x.nnz_c == 0 || (jc && c), TSG_ERR_INVALID, "null jc or c");
+        host_numeric(x, a, b, jc, c);
+    });
+}
+
+}  // namespace
+
+extern "C" {
+
+void thmkl_dcsrmultcsr(const char *trans, const int *request, const int *sort, const int *m,
+                       const int *n, const int *k, double *a, int *ja, int *ia, double *b,
+                       int *jb, int *ib, double *c, int *jc, int *ic, const int *nzmax,
+                       int *info) {
+    (void)trans;
+    (void)sort;
+    (void)nzmax;
+    if (!request || !m || !n || !k || !info) {
+        if (info) *info = TSG_ERR_INVALID;
+        return;
+    }
+    int rc = host_entry<int>(*request, *m, *n, *k, a, ja, ia, b, jb, ib, c, jc, ic);
+    *info = rc != TSG_OK ? rc : (*request == 1 ? 1 : 0);
+}
+
+int tsg_dcsrmultcsr_i64(int request, int m, int n, int k, const double *a, const int *ja,
+                        const int *ia, const double *b, const int *jb, const int *ib, double *c,
+                        int *jc, int64_t *ic) {
+    return host_entry<int64_t>(request, m, n, k, a, ja, ia, b, jb, ib, c, jc, ic);
+}
+
+}  // extern "C"

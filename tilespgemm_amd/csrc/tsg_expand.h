@@ -1,0 +1,176 @@
+// Flat product expansion, LDS hash tables, ordered accumulation and
+// group-wide sorts shared by the row path and the tile path (gfx950, wave64).
+#pragma once
+
+#include "tsg_device.h"
+
+// ------------------------------------------------- flat product expansion
+// Range of B row t restricted to columns >= c, through the block pointer
+// table (entries per row: nblk+1; entry b = first position with col >= b<<bs).
+__device__ __forceinline__ int brange_lo(const int *__restrict__ bptr, int stride, int bs,
+                                         const int *__restrict__ jb, int t, int c) {
+    const int b = c >> bs;
+    int lo = bptr[(size_t)t * stride + b];
+    if (c & ((1 << bs) - 1)) {
+        int hi = bptr[(size_t)t * stride + b + 1];
+        lo = lower_bound_i32(jb, lo, hi, c);
+    }
+    return lo;
+}
+
+// Visits the products of the A entries [pa, pe) of one row in A-row order,
+// G lanes at a time, optionally restricted to B columns [c0, c1).  The
+// A entries are taken in groups of G starting at pa + gofs, stride gstride.
+// f(active, col, a, b) is called once per batch by every lane of the group;
+// batches are issued in product order and lane order inside a batch is
+// product order.
+template <int G, bool RANGED, bool VALUES, class F>
+__device__ __forceinline__ void for_products(int pa, int pe, int gofs, int gstride,
+                                             const int *__restrict__ ja,
+                                             const double *__restrict__ aval,
+                                             const int *__restrict__ ib,
+                                             const int *__restrict__ jb,
+                                             const double *__restrict__ bval,
+                                             const int *__restrict__ bptr, int bstride, int bs,
+                                             int c0, int c1, F &&f) {
+    const int l = __lane_id() & (G - 1);
+    for (int g = pa + gofs; g < pe; g += gstride) {
+        const int p = g + l;
+        const bool v = p < pe;
+        int lo = 0, hi = 0;
+        double a = 0.0;
+        if (v) {
+            const int t = ja[p];
+            if (VALUES) a = aval[p];
+            if (RANGED) {
+                lo = brange_lo(bptr, bstride, bs, jb, t, c0);
+                hi = brange_lo(bptr, bstride, bs, jb, t, c1);
+            } else {
+                lo = ib[t];
+                hi = ib[t + 1];
+            }
+        }
+        const int cnt = hi - lo;
+        const int incl = group_incl_scan<G>(cnt);
+        const int total = __shfl(incl, G - 1, G);
+        for (int base = 0; base < total; base += G) {
+            const int q = base + l;
+            const bool act = q < total;
+            int s = 0;
+#pragma unroll
+            for (int step = G / 2; step > 0; step >>= 1) {
+                int probe = __shfl(incl, s + step - 1, G);
+                if (probe <= q) s += step;
+            }
+            const int incl_s = __shfl(incl, s, G);
+            const int cnt_s = __shfl(cnt, s, G);
+            const int lo_s = __shfl(lo, s, G);
+            const double a_s = VALUES ? __shfl(a, s, G) : 0.0;
+            const int e = lo_s + (q - (incl_s - cnt_s));
+            int col = 0;
+            double b = 0.0;
+            if (act) {
+                col = jb[e];
+                if (VALUES) b = bval[e];
+            }
+            f(act, col, a_s, b);
+        }
+    }
+}
+
+// Insert col into an LDS linear-probing table; returns the slot and whether
+// this call created the key.  At most SLOTS probes: a full table (which the
+// symbolic phase's sizing rules out) yields -1 instead of spinning.
+template <int SLOTS>
+__device__ __forceinline__ int hash_insert(int *keys, int col, bool &created) {
+    unsigned h = hash_col(col, SLOTS - 1);
+    created = false;
+    for (int probe = 0; probe < SLOTS; probe++) {
+        int k = keys[h];
+        if (k == col) return (int)h;
+        if (k == -1) {
+            int old = atomicCAS(&keys[h], -1, col);
+            if (old == -1) { created = true; return (int)h; }
+            if (old == col) return (int)h;
+        }
+        h = (h + 1) & (SLOTS - 1);
+    }
+    return -1;
+}
+
+template <int SLOTS>
+__device__ __forceinline__ int hash_find(const int *keys, int col) {
+    unsigned h = hash_col(col, SLOTS - 1);
+    for (int probe = 0; probe < SLOTS; probe++) {
+        if (keys[h] == col) return (int)h;
+        h = (h + 1) & (SLOTS - 1);
+    }
+    return -1;
+}
+
+// Ordered read-modify-write of vals[slot] by the active lanes of a batch:
+// lanes sharing a tag bucket go one per round, lowest lane (earliest product)
+// first, so every slot sees its products in product order.
+template <bool FUSED, int NTAGS>
+__device__ __forceinline__ void ordered_accumulate(bool act, int slot, double a, double b,
+                                                   double *vals, unsigned *tags,
+                                                   unsigned me) {
+    bool pending = act && slot >= 0;
+    const unsigned bucket = (unsigned)slot & (NTAGS - 1);
+    while (__ballot(pending)) {
+        if (pending) atomicMin(&tags[bucket], me);
+        wave_lds_fence();
+        bool win = pending && tags[bucket] == me;
+        wave_lds_fence();
+        if (win) {
+            vals[slot] = acc_step<FUSED>(vals[slot], a, b);
+            tags[bucket] = 0xffffffffu;
+            pending = false;
+        }
+        wave_lds_fence();
+    }
+}
+
+// Bitonic sort of n (power of two) keys with values, by the G lanes of a
+// group (G <= 64, so wave-synchronous) or by a whole block (GROUP_IS_BLOCK).
+template <int G>
+__device__ __forceinline__ void bitonic_sort_group(int *keys, double *vals, int n) {
+    const int l = __lane_id() & (G - 1);
+    for (int kk = 2; kk <= n; kk <<= 1) {
+        for (int j = kk >> 1; j > 0; j >>= 1) {
+            for (int i = l; i < n; i += G) {
+                int ixj = i ^ j;
+                if (ixj > i) {
+                    int ki = keys[i], kx = keys[ixj];
+                    bool asc = (i & kk) == 0;
+                    if ((ki > kx) == asc) {
+                        keys[i] = kx;
+                        keys[ixj] = ki;
+                        double t = vals[i];
+                        vals[i] = vals[ixj];
+                        vals[ixj] = t;
+                    }
+                }
+            }
+            wave_lds_fence();
+        }
+    }
+}
+
+__device__ __forceinline__ void bitonic_sort_keys_group64(int *keys, int n) {
+    const int l = __lane_id();
+    for (int kk = 2; kk <= n; kk <<= 1) {
+        for (int j = kk >> 1; j > 0; j >>= 1) {
+            for (int i = l; i < n; i += 64) {
+                int ixj = i ^ j;
+                if (ixj > i) {
+                    int ki = keys[i], kx = keys[ixj];
+                    bool asc = (i & kk) == 0;
+                    if ((ki > kx) == asc) { keys[i] = kx; keys[ixj] = ki; }
+                }
+            }
+            wave_lds_fence();
+        }
+    }
+}
+

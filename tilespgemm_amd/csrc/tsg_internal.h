@@ -1,0 +1,185 @@
+// Internal declarations shared by the tsg HIP sources (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+#include <string>
+#include <vector>
+
+#include "../../include/tsg.h"
+
+#define TSG_WAVE 64
+#define TSG_TILE 16
+
+// ------------------------------------------------------------ error plumbing
+struct TsgError {
+    int code;
+    std::string msg;
+};
+
+#define TSG_HIP_TRY(expr)                                                      \
+    do {                                                                       \
+        hipError_t _e = (expr);                                                \
+        if (_e != hipSuccess)                                                  \
+            throw TsgError{TSG_ERR_HIP, std::string(#expr) + ": " +            \
+                                            hipGetErrorString(_e)};            \
+    } while (0)
+
+#define TSG_CHECK(cond, code, text)                                            \
+    do {                                                                       \
+        if (!(cond)) throw TsgError{(code), (text)};                           \
+    } while (0)
+
+// Launch check: surfaces bad launch configurations immediately.
+#define TSG_LAUNCH_CHECK() TSG_HIP_TRY(hipGetLastError())
+
+// -------------------------------------------------------------- workspace
+// Chunked bump arena over hipMalloc'd blocks.  reset() keeps the blocks, so
+// repeated calls of the same size reach a steady state with no allocation.
+class Arena {
+public:
+    ~Arena();
+    void reset();
+    void *alloc(size_t bytes);
+    template <class T> T *alloc_n(size_t n) {
+        return static_cast<T *>(alloc(n * sizeof(T) + 1));
+    }
+    size_t held() const { return held_; }
+    void release();
+
+private:
+    struct Block { char *base; size_t size; size_t used; };
+    std::vector<Block> blocks_;
+    size_t cur_ = 0;
+    size_t held_ = 0;
+};
+
+// -------------------------------------------------------------- scans
+// Exclusive scan of n int64 counts in place into out[0..n] (out[n] = total).
+// in and out may alias.  tmp must hold scan_tmp_elems(n) int64.
+size_t scan_tmp_elems(size_t n);
+void exclusive_scan_i64(const int64_t *in, int64_t *out, size_t n, int64_t *tmp,
+                        hipStream_t s);
+void exclusive_scan_i32(const int32_t *in, int32_t *out, size_t n, int64_t *tmp,
+                        hipStream_t s);
+
+// -------------------------------------------------------------- helpers
+static inline unsigned grid_for(size_t work, unsigned per_block, unsigned cap = 1u << 30) {
+    size_t g = (work + per_block - 1) / per_block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
+int device_cu_count(int device);
+
+// Input description shared by both paths.
+struct CsrIn {
+    int rows, cols;
+    const int *ptr;
+    const int *idx;
+    const double *val;   // may be null in the symbolic phase
+    int64_t nnz;
+};
+
+// Stage timer: records (name, start, stop) HIP events on a stream.
+struct StageTimer {
+    bool enabled = false;
+    hipStream_t stream = nullptr;
+    std::vector<std::string> names;
+    std::vector<hipEvent_t> ev;          // 2 per stage
+    std::vector<double> ms;
+    void clear();
+    void begin(const char *name);
+    void end();
+    void finalize();                     // synchronizes, fills ms
+    ~StageTimer();
+};
+
+// ------------------------------------------------------------ row path
+struct RowPlan {
+    bool valid = false;
+    int m = 0, n = 0, k = 0;
+    // per-row products and output counts
+    int64_t *rowprod = nullptr;      // [m]
+    int64_t *rownnz = nullptr;       // [m+1] (scratch for the scan)
+    // bins: small rows (by products) and big rows
+    int *bin_rows = nullptr;         // [m] rows grouped by bin
+    int *bin_start = nullptr;        // [nbins+1] device
+    int bin_start_h[16] = {0};       // host copy (after symbolic)
+    int nbins = 0;
+    // big-row structures
+    int nbig = 0;
+    int nsub = 0;                    // sub-blocks per big row
+    int sub_shift = 0;               // log2 sub-block width (columns)
+    uint16_t *subcnt = nullptr;      // [nbig * nsub]
+    // B block-pointer table for big-row items
+    int blk_shift = 0;               // log2 block width (columns)
+    int nblk = 0;                    // blocks per B row (+1 stored)
+    int *bptr = nullptr;             // [(n) * (nblk+1)]
+    // items
+    int64_t nitems = 0;
+    int64_t *item_cnt = nullptr;     // [nbig+1] items per big row -> offsets
+    int4 *items = nullptr;           // {row, c0, c1, nnz}
+    int64_t *item_off = nullptr;     // output offset (within C) per item
+};
+
+void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic,
+                  int64_t *nnz_c, Arena &ws, hipStream_t s, StageTimer &tm, int cus);
+void row_numeric(RowPlan &p, const CsrIn &A, const CsrIn &B, const int64_t *d_ic,
+                 int *d_jc, double *d_c, int fused, Arena &ws, hipStream_t s,
+                 StageTimer &tm, int cus);
+
+// ------------------------------------------------------------ tile path
+struct TileMat {
+    int rows = 0, cols = 0, blkm = 0, blkn = 0, numblk = 0;
+    int *tile_ptr = nullptr;     // [blkm+1]
+    int *tile_col = nullptr;     // [numblk]
+    int *tile_off = nullptr;     // [numblk+1]
+    uint16_t *mask = nullptr;    // [16*numblk]
+    uint8_t *rowptr = nullptr;   // [16*numblk]
+    uint8_t *idx = nullptr;      // [nnz]
+    double *val = nullptr;       // [nnz]
+    int64_t nnz = 0;
+    // tile-CSC index (B only)
+    int *csc_ptr = nullptr;      // [blkn+1]
+    int *csc_row = nullptr;      // [numblk]
+    int *csc_tile = nullptr;     // [numblk]
+    // entry -> payload position permutation (values refreshed per numeric)
+    int *perm = nullptr;         // [nnz] CSR position -> tile payload position
+};
+
+struct TilePlan {
+    bool valid = false;
+    bool b_is_a = false;
+    TileMat A, B;
+    int numblkC = 0;
+    int *tptrC = nullptr;        // [blkm+1]
+    int *tcolC = nullptr;        // [numblkC]
+    int *trowC = nullptr;        // [numblkC]
+    uint16_t *maskC = nullptr;   // [16*numblkC]
+    uint8_t *rowptrC = nullptr;  // [16*numblkC]
+    int64_t *offC = nullptr;     // [numblkC+1] payload offsets (tile order)
+    int *rowoffC = nullptr;      // [numblkC*16] start of (tile,row) inside CSR row
+    int64_t *rownnz = nullptr;   // [m+1] CSR row counts
+    int64_t nnzC = 0;
+};
+
+void tile_symbolic(TilePlan &p, const CsrIn &A, const CsrIn &B, bool b_is_a,
+                   int64_t *d_ic, int64_t *nnz_c, Arena &ws, hipStream_t s,
+                   StageTimer &tm, int cus);
+void tile_numeric(TilePlan &p, const CsrIn &A, const CsrIn &B, const int64_t *d_ic,
+                  int *d_jc, double *d_c, int fused, Arena &ws, hipStream_t s,
+                  StageTimer &tm, int cus);
+
+// CSR utilities
+// Validates A and B on the device (one sync).  Returns flags of A in bits
+// 0-7 and of B in bits 8-15: 1 bad row pointers, 2 column out of range,
+// 4 a row not strictly ascending.
+int validate_csr_pair(const CsrIn &A, const CsrIn &B, Arena &ws, hipStream_t s);
+void csr_transpose(int m, int n, const int *ia, const int *ja, const double *a,
+                   int64_t nnz, int *tp, int *ti, double *tv, Arena &ws,
+                   hipStream_t s, int *tpos = nullptr);
+void csr_row_products(int m, const int *ia, const int *ja, const int *ib,
+                      int64_t *rowprod, hipStream_t s);

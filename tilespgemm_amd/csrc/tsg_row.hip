@@ -1,0 +1,550 @@
+// Row path: row-binned SpGEMM for matrices whose 16x16 tiles are too sparse
+// for the tile path (hypersparse / power-law inputs such as R-MAT).
+//
+// Lineage: the reference bins rows by intermediate products and runs per-bin
+// hash kernels (nsparse, spgemm_nsparse_kernel.h:221-311 binning, :331-687
+// symbolic hash, :696-1119 structure fill; its value-carrying variant is
+// commented out at :805-907).  This path is built for gfx950 instead:
+//
+//   * Every output entry is accumulated from +0.0 over its products in A-row
+//     order (thmkl_dcsrmultcsr.c:61-72), so results are bitwise equal to the
+//     CPU reference.  Products are expanded flat, a wave at a time in A-row
+//     order ("t-major"); lanes of one batch that hit the same accumulator slot
+//     are serialized in lane order by tag rounds (LDS atomicMin on a small
+//     bucket table), so the parallel sum equals the sequential one.
+//   * Small rows (<= SMALL_PROD products): one lane group per row, LDS hash
+//     table, output sorted by an in-LDS bitonic sort.
+//   * Big rows: the symbolic phase marks the row's columns in an LDS bitmap
+//     covering up to 2^20 columns and records the count of every 1024-column
+//     sub-block; the numeric phase splits the row into items of <= ITEM_CAP
+//     outputs and <= ITEM_WBITS columns, each accumulated by one wave in an
+//     LDS hash and emitted in column order through an LDS bitmap.  A B block
+//     pointer table gives each B row's sub-range of an item without a search.
+#include "tsg_internal.h"
+#include "tsg_device.h"
+#include "tsg_expand.h"
+
+#include <algorithm>
+
+namespace {
+
+constexpr int SMALL_PROD = 1024;       // rows up to this many products: small path
+constexpr int NBINS = 5;               // 0: empty, 1..3: small, 4: big
+constexpr int BIN_BIG = 4;
+constexpr int bin_limit[NBINS] = {0, 64, 256, 1024, 0x7fffffff};
+
+constexpr int SUB_SHIFT = 10;          // sub-block = 1024 columns
+constexpr int BIG_PASS_SHIFT = 20;     // one symbolic pass covers 2^20 columns
+constexpr int BIG_T = 1024;            // threads of the big symbolic kernel
+constexpr int ITEM_CAP = 1024;         // outputs per item
+constexpr int ITEM_SLOTS = 2048;       // hash slots per item (load <= 1/2)
+constexpr int ITEM_WBITS_SHIFT = 16;   // item width <= 65536 columns
+constexpr int ITEM_WORDS = (1 << ITEM_WBITS_SHIFT) / 32;
+constexpr int ITEM_TAGS = 256;
+constexpr int ITEM_WAVES = 4;
+
+struct BinStarts { int s[NBINS + 1]; };
+
+__device__ __forceinline__ int bin_of(int64_t prod) {
+    if (prod == 0) return 0;
+    if (prod <= 64) return 1;
+    if (prod <= 256) return 2;
+    if (prod <= SMALL_PROD) return 3;
+    return BIN_BIG;
+}
+
+// ------------------------------------------------------------------ binning
+__global__ void k_bin_count(int m, const int64_t *__restrict__ rowprod, int *__restrict__ cnt) {
+    __shared__ int s_cnt[NBINS];
+    if (threadIdx.x < NBINS) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) atomicAdd(&s_cnt[bin_of(rowprod[i])], 1);
+    __syncthreads();
+    if (threadIdx.x < NBINS && s_cnt[threadIdx.x]) atomicAdd(&cnt[threadIdx.x], s_cnt[threadIdx.x]);
+}
+
+__global__ void k_bin_scatter(int m, const int64_t *__restrict__ rowprod, BinStarts st,
+                              int *__restrict__ fill, int *__restrict__ bin_rows,
+                              int64_t *__restrict__ rownnz) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    int b = bin_of(rowprod[i]);
+    if (b == 0) { rownnz[i] = 0; return; }
+    int pos = atomicAdd(&fill[b], 1);
+    bin_rows[st.s[b] + pos] = i;
+}
+
+// ------------------------------------------------------ small rows: symbolic
+template <int G, int SLOTS>
+__global__ void __launch_bounds__(256) k_small_symbolic(int nrows, const int *__restrict__ rows,
+                                                        const int *__restrict__ ia,
+                                                        const int *__restrict__ ja,
+                                                        const int *__restrict__ ib,
+                                                        const int *__restrict__ jb,
+                                                        int64_t *__restrict__ rownnz) {
+    constexpr int RPB = 256 / G;
+    __shared__ int s_keys[RPB * SLOTS];
+    const int grp = threadIdx.x / G;
+    const int l = threadIdx.x & (G - 1);
+    int *keys = s_keys + grp * SLOTS;
+    const int r = blockIdx.x * RPB + grp;
+    for (int i = l; i < SLOTS; i += G) keys[i] = -1;
+    wave_lds_fence();
+    if (r >= nrows) return;
+    const int row = rows[r];
+    int created_cnt = 0;
+    for_products<G, false, false>(ia[row], ia[row + 1], 0, G, ja, nullptr, ib, jb, nullptr,
+                                  nullptr, 0, 0, 0, 0,
+                                  [&](bool act, int col, double, double) {
+                                      if (act) {
+                                          bool created;
+                                          hash_insert<SLOTS>(keys, col, created);
+                                          created_cnt += created;
+                                      }
+                                  });
+    created_cnt = group_sum<G>(created_cnt);
+    if (l == 0) rownnz[row] = created_cnt;
+}
+
+// ------------------------------------------------------- small rows: numeric
+template <int G, int SLOTS, bool FUSED>
+__global__ void __launch_bounds__(256) k_small_numeric(int nrows, const int *__restrict__ rows,
+                                                       const int *__restrict__ ia,
+                                                       const int *__restrict__ ja,
+                                                       const double *__restrict__ a,
+                                                       const int *__restrict__ ib,
+                                                       const int *__restrict__ jb,
+                                                       const double *__restrict__ b,
+                                                       const int64_t *__restrict__ ic,
+                                                       int *__restrict__ jc,
+                                                       double *__restrict__ c) {
+    constexpr int RPB = 256 / G;
+    constexpr int NT = 4 * G;
+    extern __shared__ __align__(16) unsigned char smem[];
+    double *s_vals = reinterpret_cast<double *>(smem);
+    int *s_keys = reinterpret_cast<int *>(s_vals + RPB * SLOTS);
+    unsigned *s_tags = reinterpret_cast<unsigned *>(s_keys + RPB * SLOTS);
+    const int grp = threadIdx.x / G;
+    const int l = threadIdx.x & (G - 1);
+    int *keys = s_keys + grp * SLOTS;
+    double *vals = s_vals + grp * SLOTS;
+    unsigned *tags = s_tags + grp * NT;
+    const int r = blockIdx.x * RPB + grp;
+    for (int i = l; i < SLOTS; i += G) { keys[i] = -1; vals[i] = 0.0; }
+    for (int i = l; i < NT; i += G) tags[i] = 0xffffffffu;
+    wave_lds_fence();
+    if (r >= nrows) return;
+    const int row = rows[r];
+    for_products<G, false, true>(ia[row], ia[row + 1], 0, G, ja, a, ib, jb, b, nullptr, 0, 0, 0,
+                                 0, [&](bool act, int col, double av, double bv) {
+                                     int slot = 0;
+                                     if (act) {
+                                         bool created;
+                                         slot = hash_insert<SLOTS>(keys, col, created);
+                                     }
+                                     wave_lds_fence();
+                                     ordered_accumulate<FUSED, NT>(act, slot, av, bv, vals, tags,
+                                                                   (unsigned)l);
+                                 });
+    // empty slots sort last
+    for (int i = l; i < SLOTS; i += G)
+        if (keys[i] == -1) keys[i] = 0x7fffffff;
+    wave_lds_fence();
+    bitonic_sort_group<G>(keys, vals, SLOTS);
+    const int64_t o = ic[row];
+    const int nnz = (int)(ic[row + 1] - o);
+    for (int i = l; i < nnz; i += G) {
+        jc[o + i] = keys[i];
+        c[o + i] = vals[i];
+    }
+}
+
+// -------------------------------------------------------- big rows: symbolic
+// One block per big row (grid-stride).  LDS bitmap of up to 2^20 columns;
+// per 1024-column sub-block counts go to subcnt.
+__global__ void __launch_bounds__(BIG_T) k_big_symbolic(int nbig, const int *__restrict__ rows,
+                                                        int k, const int *__restrict__ ia,
+                                                        const int *__restrict__ ja,
+                                                        const int *__restrict__ ib,
+                                                        const int *__restrict__ jb, int nsub,
+                                                        uint16_t *__restrict__ subcnt,
+                                                        int64_t *__restrict__ rownnz) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    unsigned *bits = reinterpret_cast<unsigned *>(smem);
+    __shared__ int s_total[BIG_T / 64];
+    const int wave = threadIdx.x >> 6;
+    constexpr int NW = BIG_T / 64;
+    for (int r = blockIdx.x; r < nbig; r += gridDim.x) {
+        const int row = rows[r];
+        const int pa = ia[row], pe = ia[row + 1];
+        int row_total = 0;
+        for (int pass0 = 0; pass0 < k; pass0 += (1 << BIG_PASS_SHIFT)) {
+            const int pass1 = min(k, pass0 + (1 << BIG_PASS_SHIFT));
+            const int words = (pass1 - pass0 + 31) >> 5;
+            for (int i = threadIdx.x; i < words; i += BIG_T) bits[i] = 0u;
+            __syncthreads();
+            for_products<64, false, false>(pa, pe, wave * 64, NW * 64, ja, nullptr, ib, jb,
+                                           nullptr, nullptr, 0, 0, 0, 0,
+                                           [&](bool act, int col, double, double) {
+                                               if (act && col >= pass0 && col < pass1) {
+                                                   int c = col - pass0;
+                                                   atomicOr(&bits[c >> 5], 1u << (c & 31));
+                                               }
+                                           });
+            __syncthreads();
+            const int nsb = (pass1 - pass0 + (1 << SUB_SHIFT) - 1) >> SUB_SHIFT;
+            int part = 0;
+            for (int sb = threadIdx.x; sb < nsb; sb += BIG_T) {
+                int w0 = sb << (SUB_SHIFT - 5);
+                int w1 = min(words, w0 + (1 << (SUB_SHIFT - 5)));
+                int cnt = 0;
+                for (int w = w0; w < w1; w++) cnt += __popc(bits[w]);
+                subcnt[(size_t)r * nsub + (pass0 >> SUB_SHIFT) + sb] = (uint16_t)cnt;
+                part += cnt;
+            }
+            part = wave_sum(part);
+            if ((threadIdx.x & 63) == 0) s_total[wave] = part;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                int t = 0;
+                for (int i = 0; i < NW; i++) t += s_total[i];
+                row_total += t;
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) rownnz[row] = row_total;
+    }
+}
+
+// ------------------------------------------------------ B block pointers
+// bptr[t*(nblk+1) + b] = first position of B row t with column >= b << bs.
+__global__ void __launch_bounds__(256) k_bptr(int n, const int *__restrict__ ib,
+                                              const int *__restrict__ jb, int nblk, int bs,
+                                              int *__restrict__ bptr) {
+    const int t = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int l = threadIdx.x & 63;
+    if (t >= n) return;
+    const int lo = ib[t], hi = ib[t + 1];
+    for (int b = l; b <= nblk; b += 64) {
+        int pos = b == nblk ? hi : lower_bound_i32(jb, lo, hi, b << bs);
+        bptr[(size_t)t * (nblk + 1) + b] = pos;
+    }
+}
+
+// ------------------------------------------------------------- item plan
+// Greedy split of each big row into items: runs of sub-blocks with at most
+// ITEM_CAP outputs and at most ITEM_WBITS columns; leading empty sub-blocks
+// are skipped.  FILL=false counts, FILL=true writes.
+template <bool FILL>
+__global__ void k_items(int nbig, const int *__restrict__ rows, int k, int nsub,
+                        const uint16_t *__restrict__ subcnt, const int64_t *__restrict__ ic,
+                        int64_t *__restrict__ item_cnt, int4 *__restrict__ items,
+                        int64_t *__restrict__ item_off) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nbig) return;
+    const int row = rows[r];
+    const uint16_t *sc = subcnt + (size_t)r * nsub;
+    constexpr int WMAX_SB = 1 << (ITEM_WBITS_SHIFT - SUB_SHIFT);
+    int64_t nitem = 0;
+    int64_t out = FILL ? ic[row] : 0;
+    int64_t w = FILL ? item_cnt[r] : 0;
+    int start = -1, cnt = 0;
+    for (int sb = 0; sb < nsub; sb++) {
+        const int cs = sc[sb];
+        if (start >= 0 && (cnt + cs > ITEM_CAP || sb - start >= WMAX_SB)) {
+            if (FILL) {
+                items[w] = make_int4(row, start << SUB_SHIFT, min(k, sb << SUB_SHIFT), cnt);
+                item_off[w] = out;
+                w++;
+                out += cnt;
+            }
+            nitem++;
+            start = -1;
+            cnt = 0;
+        }
+        if (cs == 0 && start < 0) continue;
+        if (start < 0) start = sb;
+        cnt += cs;
+    }
+    if (start >= 0 && cnt > 0) {
+        if (FILL) {
+            items[w] = make_int4(row, start << SUB_SHIFT, k, cnt);
+            item_off[w] = out;
+        }
+        nitem++;
+    }
+    if (!FILL) item_cnt[r] = nitem;
+}
+
+// --------------------------------------------------------- big rows: numeric
+// One wave per item (grid-stride over items), ITEM_WAVES waves per block,
+// each with its own LDS hash, column bitmap and tag table.
+struct ItemSmem {
+    double vals[ITEM_SLOTS];
+    int keys[ITEM_SLOTS];
+    unsigned bits[ITEM_WORDS];
+    unsigned tags[ITEM_TAGS];
+};
+
+template <bool FUSED>
+__global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric(
+    int64_t nitems, const int4 *__restrict__ items, const int64_t *__restrict__ item_off,
+    const int *__restrict__ ia, const int *__restrict__ ja, const double *__restrict__ a,
+    const int *__restrict__ ib, const int *__restrict__ jb, const double *__restrict__ b,
+    const int *__restrict__ bptr, int bstride, int bs, int *__restrict__ jc,
+    double *__restrict__ c) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    ItemSmem &S = reinterpret_cast<ItemSmem *>(smem)[threadIdx.x >> 6];
+    const int l = __lane_id();
+    for (int i = l; i < ITEM_TAGS; i += 64) S.tags[i] = 0xffffffffu;
+    const int64_t wave_id = (int64_t)blockIdx.x * ITEM_WAVES + (threadIdx.x >> 6);
+    const int64_t nwaves = (int64_t)gridDim.x * ITEM_WAVES;
+    for (int64_t it = wave_id; it < nitems; it += nwaves) {
+        const int4 item = items[it];
+        const int row = item.x, c0 = item.y, c1 = item.z;
+        const int nwords = (c1 - c0 + 31) >> 5;
+        for (int i = l; i < ITEM_SLOTS; i += 64) { S.keys[i] = -1; S.vals[i] = 0.0; }
+        for (int i = l; i < nwords; i += 64) S.bits[i] = 0u;
+        wave_lds_fence();
+        for_products<64, true, true>(
+            ia[row], ia[row + 1], 0, 64, ja, a, ib, jb, b, bptr, bstride, bs, c0, c1,
+            [&](bool act, int col, double av, double bv) {
+                int slot = 0;
+                if (act) {
+                    bool created;
+                    slot = hash_insert<ITEM_SLOTS>(S.keys, col, created);
+                    if (created) {
+                        int cc = col - c0;
+                        atomicOr(&S.bits[cc >> 5], 1u << (cc & 31));
+                    }
+                }
+                wave_lds_fence();
+                ordered_accumulate<FUSED, ITEM_TAGS>(act, slot, av, bv, S.vals, S.tags,
+                                                     (unsigned)l);
+            });
+        wave_lds_fence();
+        // emit in column order: bitmap words -> ranks -> hash lookups
+        int64_t o = item_off[it];
+        for (int w0 = 0; w0 < nwords; w0 += 64) {
+            const int w = w0 + l;
+            unsigned word = w < nwords ? S.bits[w] : 0u;
+            const int pc = __popc(word);
+            const int incl = wave_incl_scan(pc);
+            int64_t pos = o + incl - pc;
+            while (word) {
+                const int bit = __ffs(word) - 1;
+                word &= word - 1;
+                const int col = c0 + (w << 5) + bit;
+                const int slot = hash_find<ITEM_SLOTS>(S.keys, col);
+                jc[pos] = col;
+                c[pos] = slot >= 0 ? S.vals[slot] : 0.0;
+                pos++;
+            }
+            o += __shfl(incl, 63, 64);
+        }
+        wave_lds_fence();
+    }
+}
+
+template <class K>
+void set_smem(K kernel, size_t bytes) {
+    TSG_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
+
+}  // namespace
+
+// =================================================================== host
+void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int64_t *nnz_c,
+                  Arena &ws, hipStream_t s, StageTimer &tm, int cus) {
+    const int m = A.rows, n = A.cols, k = B.cols;
+    p = RowPlan();
+    p.m = m;
+    p.n = n;
+    p.k = k;
+    p.rowprod = ws.alloc_n<int64_t>((size_t)m + 1);
+    p.rownnz = ws.alloc_n<int64_t>((size_t)m + 1);
+    p.bin_rows = ws.alloc_n<int>((size_t)m + 1);
+    int *d_cnt = ws.alloc_n<int>(2 * NBINS + 2);
+    int64_t *scan_tmp = ws.alloc_n<int64_t>(scan_tmp_elems((size_t)m + 1));
+
+    tm.begin("row.products+bins");
+    csr_row_products(m, A.ptr, A.idx, B.ptr, p.rowprod, s);
+    TSG_HIP_TRY(hipMemsetAsync(d_cnt, 0, sizeof(int) * (2 * NBINS + 2), s));
+    if (m > 0) {
+        hipLaunchKernelGGL(k_bin_count, dim3(grid_for(m, 256)), dim3(256), 0, s, m, p.rowprod,
+                           d_cnt);
+        TSG_LAUNCH_CHECK();
+    }
+    int h_cnt[NBINS];
+    TSG_HIP_TRY(hipMemcpyAsync(h_cnt, d_cnt, sizeof(h_cnt), hipMemcpyDeviceToHost, s));
+    TSG_HIP_TRY(hipStreamSynchronize(s));
+    BinStarts st;
+    st.s[0] = 0;
+    for (int b = 0; b < NBINS; b++) st.s[b + 1] = st.s[b] + (b == 0 ? 0 : h_cnt[b]);
+    for (int b = 0; b <= NBINS; b++) p.bin_start_h[b] = st.s[b];
+    p.nbins = NBINS;
+    if (m > 0) {
+        hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(m, 256)), dim3(256), 0, s, m, p.rowprod,
+                           st, d_cnt + NBINS, p.bin_rows, p.rownnz);
+        TSG_LAUNCH_CHECK();
+    }
+    tm.end();
+
+    // small rows
+    tm.begin("row.small_symbolic");
+    {
+        const int *rows1 = p.bin_rows + st.s[1];
+        int n1 = st.s[2] - st.s[1];
+        if (n1 > 0) {
+            hipLaunchKernelGGL((k_small_symbolic<16, 128>), dim3(grid_for(n1, 16)), dim3(256), 0,
+                               s, n1, rows1, A.ptr, A.idx, B.ptr, B.idx, p.rownnz);
+            TSG_LAUNCH_CHECK();
+        }
+        const int *rows2 = p.bin_rows + st.s[2];
+        int n2 = st.s[3] - st.s[2];
+        if (n2 > 0) {
+            hipLaunchKernelGGL((k_small_symbolic<32, 512>), dim3(grid_for(n2, 8)), dim3(256), 0,
+                               s, n2, rows2, A.ptr, A.idx, B.ptr, B.idx, p.rownnz);
+            TSG_LAUNCH_CHECK();
+        }
+        const int *rows3 = p.bin_rows + st.s[3];
+        int n3 = st.s[4] - st.s[3];
+        if (n3 > 0) {
+            hipLaunchKernelGGL((k_small_symbolic<64, 2048>), dim3(grid_for(n3, 4)), dim3(256), 0,
+                               s, n3, rows3, A.ptr, A.idx, B.ptr, B.idx, p.rownnz);
+            TSG_LAUNCH_CHECK();
+        }
+    }
+    tm.end();
+
+    // big rows
+    p.nbig = st.s[5] - st.s[4];
+    p.sub_shift = SUB_SHIFT;
+    p.nsub = (k + (1 << SUB_SHIFT) - 1) >> SUB_SHIFT;
+    if (p.nbig > 0) {
+        tm.begin("row.big_symbolic");
+        p.subcnt = ws.alloc_n<uint16_t>((size_t)p.nbig * p.nsub);
+        const int *rowsb = p.bin_rows + st.s[4];
+        const size_t smem = sizeof(unsigned) * (size_t)((std::min(k, 1 << BIG_PASS_SHIFT) + 31) / 32);
+        set_smem(k_big_symbolic, smem);
+        const unsigned grid = (unsigned)std::min(p.nbig, cus * 1);
+        hipLaunchKernelGGL(k_big_symbolic, dim3(grid), dim3(BIG_T), smem, s, p.nbig, rowsb, k,
+                           A.ptr, A.idx, B.ptr, B.idx, p.nsub, p.subcnt, p.rownnz);
+        TSG_LAUNCH_CHECK();
+        tm.end();
+
+        // B block pointers: finest block width >= 1024 whose table stays <= 1 GiB
+        tm.begin("row.bptr");
+        int bs = SUB_SHIFT;
+        while (bs < 30) {
+            size_t nblk = ((size_t)k + (1u << bs) - 1) >> bs;
+            if ((size_t)n * (nblk + 1) * sizeof(int) <= (size_t(1) << 30)) break;
+            bs++;
+        }
+        p.blk_shift = bs;
+        p.nblk = (int)(((size_t)k + (1u << bs) - 1) >> bs);
+        p.bptr = ws.alloc_n<int>((size_t)n * (p.nblk + 1));
+        if (n > 0) {
+            hipLaunchKernelGGL(k_bptr, dim3(grid_for((size_t)n * 64, 256)), dim3(256), 0, s, n,
+                               B.ptr, B.idx, p.nblk, bs, p.bptr);
+            TSG_LAUNCH_CHECK();
+        }
+        tm.end();
+    }
+
+    tm.begin("row.scan");
+    exclusive_scan_i64(p.rownnz, d_ic, (size_t)m, scan_tmp, s);
+    tm.end();
+
+    int64_t h_tot[2] = {0, 0};
+    if (p.nbig > 0) {
+        tm.begin("row.items");
+        const int *rowsb = p.bin_rows + st.s[4];
+        p.item_cnt = ws.alloc_n<int64_t>((size_t)p.nbig + 1);
+        int64_t *tmp2 = ws.alloc_n<int64_t>(scan_tmp_elems((size_t)p.nbig + 1));
+        hipLaunchKernelGGL(k_items<false>, dim3(grid_for(p.nbig, 128)), dim3(128), 0, s, p.nbig,
+                           rowsb, k, p.nsub, p.subcnt, d_ic, p.item_cnt, nullptr, nullptr);
+        TSG_LAUNCH_CHECK();
+        exclusive_scan_i64(p.item_cnt, p.item_cnt, (size_t)p.nbig, tmp2, s);
+        TSG_HIP_TRY(hipMemcpyAsync(&h_tot[1], p.item_cnt + p.nbig, sizeof(int64_t),
+                                   hipMemcpyDeviceToHost, s));
+        TSG_HIP_TRY(hipMemcpyAsync(&h_tot[0], d_ic + m, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        TSG_HIP_TRY(hipStreamSynchronize(s));
+        p.nitems = h_tot[1];
+        p.items = ws.alloc_n<int4>((size_t)p.nitems + 1);
+        p.item_off = ws.alloc_n<int64_t>((size_t)p.nitems + 1);
+        hipLaunchKernelGGL(k_items<true>, dim3(grid_for(p.nbig, 128)), dim3(128), 0, s, p.nbig,
+                           rowsb, k, p.nsub, p.subcnt, d_ic, p.item_cnt, p.items, p.item_off);
+        TSG_LAUNCH_CHECK();
+        tm.end();
+    } else {
+        TSG_HIP_TRY(hipMemcpyAsync(&h_tot[0], d_ic + m, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        TSG_HIP_TRY(hipStreamSynchronize(s));
+    }
+    *nnz_c = h_tot[0];
+    p.valid = true;
+}
+
+template <bool FUSED>
+static void row_numeric_t(RowPlan &p, const CsrIn &A, const CsrIn &B, const int64_t *d_ic,
+                          int *d_jc, double *d_c, hipStream_t s, StageTimer &tm, int cus) {
+    const int *bins = p.bin_rows;
+    const int *st = p.bin_start_h;
+    tm.begin("row.small_numeric");
+    {
+        int n1 = st[2] - st[1];
+        if (n1 > 0) {
+            constexpr size_t sm = 16 * 128 * 12 + 16 * 64 * 4;
+            auto kern = k_small_numeric<16, 128, FUSED>;
+            set_smem(kern, sm);
+            hipLaunchKernelGGL(kern, dim3(grid_for(n1, 16)), dim3(256), sm, s, n1, bins + st[1],
+                               A.ptr, A.idx, A.val, B.ptr, B.idx, B.val, d_ic, d_jc, d_c);
+            TSG_LAUNCH_CHECK();
+        }
+        int n2 = st[3] - st[2];
+        if (n2 > 0) {
+            constexpr size_t sm = 8 * 512 * 12 + 8 * 128 * 4;
+            auto kern = k_small_numeric<32, 512, FUSED>;
+            set_smem(kern, sm);
+            hipLaunchKernelGGL(kern, dim3(grid_for(n2, 8)), dim3(256), sm, s, n2, bins + st[2],
+                               A.ptr, A.idx, A.val, B.ptr, B.idx, B.val, d_ic, d_jc, d_c);
+            TSG_LAUNCH_CHECK();
+        }
+        int n3 = st[4] - st[3];
+        if (n3 > 0) {
+            constexpr size_t sm = 4 * 2048 * 12 + 4 * 256 * 4;
+            auto kern = k_small_numeric<64, 2048, FUSED>;
+            set_smem(kern, sm);
+            hipLaunchKernelGGL(kern, dim3(grid_for(n3, 4)), dim3(256), sm, s, n3, bins + st[3],
+                               A.ptr, A.idx, A.val, B.ptr, B.idx, B.val, d_ic, d_jc, d_c);
+            TSG_LAUNCH_CHECK();
+        }
+    }
+    tm.end();
+    if (p.nitems > 0) {
+        tm.begin("row.big_numeric");
+        constexpr size_t sm = sizeof(ItemSmem) * ITEM_WAVES;
+        auto kern = k_big_numeric<FUSED>;
+        set_smem(kern, sm);
+        const unsigned grid = (unsigned)std::min<int64_t>(
+            (p.nitems + ITEM_WAVES - 1) / ITEM_WAVES, (int64_t)cus * 8);
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * ITEM_WAVES), sm, s, p.nitems, p.items,
+                           p.item_off, A.ptr, A.idx, A.val, B.ptr, B.idx, B.val, p.bptr,
+                           p.nblk + 1, p.blk_shift, d_jc, d_c);
+        TSG_LAUNCH_CHECK();
+        tm.end();
+    }
+}
+
+void row_numeric(RowPlan &p, const CsrIn &A, const CsrIn &B, const int64_t *d_ic, int *d_jc,
+                 double *d_c, int fused, Arena &ws, hipStream_t s, StageTimer &tm, int cus) {
+    (void)ws;
+    TSG_CHECK(p.valid && p.m == A.rows && p.n == A.cols && p.k == B.cols, TSG_ERR_STATE,
+              "row numeric phase without a matching symbolic phase");
+    if (fused)
+        row_numeric_t<true>(p, A, B, d_ic, d_jc, d_c, s, tm, cus);
+    else
+        row_numeric_t<false>(p, A, B, d_ic, d_jc, d_c, s, tm, cus);
+}

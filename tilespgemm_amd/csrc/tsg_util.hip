@@ -1,0 +1,329 @@
+// Utilities: workspace arena, device-wide scans, stage timer, CSR transpose,
+// intermediate-product counts.
+#include "tsg_internal.h"
+#include "tsg_device.h"
+
+#include <algorithm>
+#include <hipcub/hipcub.hpp>
+
+// ================================================================= arena
+Arena::~Arena() { release(); }
+
+void Arena::release() {
+    for (auto &b : blocks_) (void)hipFree(b.base);
+    blocks_.clear();
+    cur_ = 0;
+    held_ = 0;
+}
+
+void Arena::reset() {
+    for (auto &b : blocks_) b.used = 0;
+    cur_ = 0;
+}
+
+void *Arena::alloc(size_t bytes) {
+    bytes = (bytes + 255) & ~size_t(255);
+    while (cur_ < blocks_.size()) {
+        Block &b = blocks_[cur_];
+        if (b.size - b.used >= bytes) {
+            void *p = b.base + b.used;
+            b.used += bytes;
+            return p;
+        }
+        cur_++;
+    }
+    size_t sz = std::max(bytes, size_t(64) << 20);
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, sz);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw TsgError{TSG_ERR_ALLOC, "workspace hipMalloc of " + std::to_string(sz) +
+                                          " bytes failed: " + hipGetErrorString(e)};
+    }
+    blocks_.push_back(Block{static_cast<char *>(p), sz, bytes});
+    cur_ = blocks_.size() - 1;
+    held_ += sz;
+    return p;
+}
+
+// ================================================================= scans
+// Three-phase exclusive scan: per-block totals, recursive scan of the totals,
+// per-block rescan with the block's offset.  1024 elements per 256-thread
+// block (4 per thread).
+namespace {
+constexpr int SCAN_T = 256;
+constexpr int SCAN_PER = 4;
+constexpr int SCAN_BLK = SCAN_T * SCAN_PER;
+
+template <class T>
+__device__ __forceinline__ long long block_excl_scan(long long x, long long *total) {
+    __shared__ long long wsum[SCAN_T / 64];
+    long long inc = wave_incl_scan64(x);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 63) wsum[w] = inc;
+    __syncthreads();
+    long long off = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_T / 64; i++) {
+        if (i < w) off += wsum[i];
+        tot += wsum[i];
+    }
+    __syncthreads();
+    *total = tot;
+    return off + inc - x;
+}
+
+template <class T>
+__global__ void __launch_bounds__(SCAN_T) k_scan_reduce(const T *in, size_t n, long long *sums) {
+    size_t base = (size_t)blockIdx.x * SCAN_BLK + threadIdx.x * SCAN_PER;
+    long long s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_PER; i++)
+        if (base + i < n) s += (long long)in[base + i];
+    long long tot;
+    (void)block_excl_scan<T>(s, &tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+template <class T>
+__global__ void __launch_bounds__(SCAN_T) k_scan_apply(const T *in, T *out, size_t n,
+                                                        const long long *offs, int has_offs) {
+    size_t base = (size_t)blockIdx.x * SCAN_BLK + threadIdx.x * SCAN_PER;
+    T v[SCAN_PER];
+    long long s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_PER; i++) {
+        v[i] = base + i < n ? in[base + i] : T(0);
+        s += (long long)v[i];
+    }
+    long long tot;
+    long long ex = block_excl_scan<T>(s, &tot);
+    long long run = ex + (has_offs ? offs[blockIdx.x] : 0);
+#pragma unroll
+    for (int i = 0; i < SCAN_PER; i++) {
+        if (base + i < n) out[base + i] = (T)run;
+        run += (long long)v[i];
+    }
+    // out[n] = grand total, written by the thread that owns element n-1
+    if (n > 0 && base <= n - 1 && n - 1 < base + SCAN_PER) out[n] = (T)run;
+    if (n == 0 && blockIdx.x == 0 && threadIdx.x == 0) out[0] = T(0);
+}
+
+size_t level_elems(size_t n) { return (n + SCAN_BLK - 1) / SCAN_BLK; }
+
+template <class T>
+void scan_rec(const T *in, T *out, size_t n, long long *tmp, hipStream_t s) {
+    size_t nb = level_elems(n);
+    if (nb <= 1) {
+        hipLaunchKernelGGL(k_scan_apply<T>, dim3(1), dim3(SCAN_T), 0, s, in, out, n,
+                           (const long long *)nullptr, 0);
+        TSG_LAUNCH_CHECK();
+        return;
+    }
+    long long *sums = tmp;
+    long long *rest = tmp + nb + 1;
+    hipLaunchKernelGGL(k_scan_reduce<T>, dim3((unsigned)nb), dim3(SCAN_T), 0, s, in, n, sums);
+    TSG_LAUNCH_CHECK();
+    scan_rec<long long>(sums, sums, nb, rest, s);
+    hipLaunchKernelGGL(k_scan_apply<T>, dim3((unsigned)nb), dim3(SCAN_T), 0, s, in, out, n,
+                       (const long long *)sums, 1);
+    TSG_LAUNCH_CHECK();
+}
+}  // namespace
+
+size_t scan_tmp_elems(size_t n) {
+    size_t total = 0;
+    size_t nb = level_elems(n);
+    while (nb > 1) {
+        total += nb + 1;
+        nb = level_elems(nb);
+    }
+    return total + 8;
+}
+
+// The apply pass reads in[] before any block writes out[] only within its own
+// block range, so in == out is safe: every block reads its elements before
+// writing them, and out[n] lies outside every block's input range.
+void exclusive_scan_i64(const int64_t *in, int64_t *out, size_t n, int64_t *tmp, hipStream_t s) {
+    scan_rec<long long>((const long long *)in, (long long *)out, n, (long long *)tmp, s);
+}
+
+void exclusive_scan_i32(const int32_t *in, int32_t *out, size_t n, int64_t *tmp, hipStream_t s) {
+    scan_rec<int>(in, out, n, (long long *)tmp, s);
+}
+
+// ================================================================= timer
+void StageTimer::clear() {
+    for (auto e : ev) (void)hipEventDestroy(e);
+    ev.clear();
+    names.clear();
+    ms.clear();
+}
+
+void StageTimer::begin(const char *name) {
+    if (!enabled) return;
+    hipEvent_t a, b;
+    TSG_HIP_TRY(hipEventCreate(&a));
+    TSG_HIP_TRY(hipEventCreate(&b));
+    TSG_HIP_TRY(hipEventRecord(a, stream));
+    names.emplace_back(name);
+    ev.push_back(a);
+    ev.push_back(b);
+}
+
+void StageTimer::end() {
+    if (!enabled || ev.empty()) return;
+    TSG_HIP_TRY(hipEventRecord(ev.back(), stream));
+}
+
+void StageTimer::finalize() {
+    if (!enabled) return;
+    ms.assign(names.size(), 0.0);
+    for (size_t i = 0; i < names.size(); i++) {
+        TSG_HIP_TRY(hipEventSynchronize(ev[2 * i + 1]));
+        float t = 0.f;
+        TSG_HIP_TRY(hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]));
+        ms[i] = t;
+    }
+}
+
+StageTimer::~StageTimer() { clear(); }
+
+// ======================================================= validation
+// flags: bit 0 bad row pointers, bit 1 column out of range, bit 2 a row not
+// strictly ascending (unsorted or duplicate columns).  Reads stay inside
+// [0, nnz) whatever the row pointers hold.
+__global__ void __launch_bounds__(256) k_validate(int m, int n, const int *__restrict__ ia,
+                                                  const int *__restrict__ ja, int64_t nnz,
+                                                  int *__restrict__ flags) {
+    const int i = (blockIdx.x * blockDim.x + threadIdx.x) >> 3;
+    const int l = threadIdx.x & 7;
+    if (i >= m) return;
+    int f = 0;
+    const int64_t lo = ia[i], hi = ia[i + 1];
+    if (lo > hi || lo < 0 || hi > nnz) f |= 1;
+    if (i == 0 && lo != 0) f |= 1;
+    if (i == m - 1 && hi != nnz) f |= 1;
+    const int64_t a = lo < 0 ? 0 : (lo > nnz ? nnz : lo);
+    const int64_t b = hi < a ? a : (hi > nnz ? nnz : hi);
+    for (int64_t p = a + l; p < b; p += 8) {
+        const int c = ja[p];
+        if (c < 0 || c >= n) f |= 2;
+        if (p > a && ja[p - 1] >= c) f |= 4;
+    }
+    if (f) atomicOr(flags, f);
+}
+
+int validate_csr_pair(const CsrIn &A, const CsrIn &B, Arena &ws, hipStream_t s) {
+    int *d = ws.alloc_n<int>(2);
+    TSG_HIP_TRY(hipMemsetAsync(d, 0, 2 * sizeof(int), s));
+    if (A.rows > 0) {
+        hipLaunchKernelGGL(k_validate, dim3(grid_for((size_t)A.rows * 8, 256)), dim3(256), 0, s,
+                           A.rows, A.cols, A.ptr, A.idx, A.nnz, d);
+        TSG_LAUNCH_CHECK();
+    }
+    if (B.rows > 0 && (B.ptr != A.ptr || B.idx != A.idx || B.cols != A.cols)) {
+        hipLaunchKernelGGL(k_validate, dim3(grid_for((size_t)B.rows * 8, 256)), dim3(256), 0, s,
+                           B.rows, B.cols, B.ptr, B.idx, B.nnz, d + 1);
+        TSG_LAUNCH_CHECK();
+    }
+    int h[2] = {0, 0};
+    TSG_HIP_TRY(hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, s));
+    TSG_HIP_TRY(hipStreamSynchronize(s));
+    if (A.rows == 0 && A.nnz != 0) h[0] |= 1;
+    if (B.rows == 0 && B.nnz != 0) h[1] |= 1;
+    return h[0] | (h[1] << 8);
+}
+
+// ======================================================= row products
+// rowprod[i] = sum over A's entries a_ik in row i of |row k of B|
+// (main_spgemm-col.cu:4950-4955).  8 lanes per row.
+__global__ void __launch_bounds__(256) k_row_products(int m, const int *__restrict__ ia,
+                                                      const int *__restrict__ ja,
+                                                      const int *__restrict__ ib,
+                                                      int64_t *__restrict__ rowprod) {
+    const int g = (blockIdx.x * blockDim.x + threadIdx.x) >> 3;
+    const int l = threadIdx.x & 7;
+    int64_t s = 0;
+    if (g < m) {
+        for (int p = ia[g] + l; p < ia[g + 1]; p += 8) {
+            int t = ja[p];
+            s += ib[t + 1] - ib[t];
+        }
+    }
+    s = group_sum<8>(s);
+    if (g < m && l == 0) rowprod[g] = s;
+}
+
+void csr_row_products(int m, const int *ia, const int *ja, const int *ib, int64_t *rowprod,
+                      hipStream_t s) {
+    if (m <= 0) return;
+    hipLaunchKernelGGL(k_row_products, dim3(grid_for((size_t)m * 8, 256)), dim3(256), 0, s, m,
+                       ia, ja, ib, rowprod);
+    TSG_LAUNCH_CHECK();
+}
+
+// ============================================================ transpose
+// Stable CSR transpose: a radix sort of (column, position) pairs keeps equal
+// columns in position order, i.e. ascending source row (utils.h:158-195).
+__global__ void k_col_count(int64_t nnz, const int *__restrict__ ja, int *__restrict__ cnt) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nnz) atomicAdd(&cnt[ja[i]], 1);
+}
+
+__global__ void k_row_index(int m, const int *__restrict__ ia, int *__restrict__ rowidx) {
+    const int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 3;
+    const int l = threadIdx.x & 7;
+    if (row >= m) return;
+    for (int p = ia[row] + l; p < ia[row + 1]; p += 8) rowidx[p] = row;
+}
+
+__global__ void k_iota(int64_t n, int *__restrict__ v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = (int)i;
+}
+
+__global__ void k_gather_t(int64_t nnz, const int *__restrict__ pos, const int *__restrict__ rowidx,
+                           const double *__restrict__ a, int *__restrict__ ti,
+                           double *__restrict__ tv, int *__restrict__ tpos) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nnz) return;
+    int p = pos[q];
+    ti[q] = rowidx[p];
+    if (a) tv[q] = a[p];
+    if (tpos) tpos[q] = p;
+}
+
+void csr_transpose(int m, int n, const int *ia, const int *ja, const double *a, int64_t nnz,
+                   int *tp, int *ti, double *tv, Arena &ws, hipStream_t s, int *tpos) {
+    int *cnt = ws.alloc_n<int>((size_t)n + 1);
+    int64_t *tmp = ws.alloc_n<int64_t>(scan_tmp_elems((size_t)n));
+    TSG_HIP_TRY(hipMemsetAsync(cnt, 0, sizeof(int) * ((size_t)n + 1), s));
+    if (nnz > 0) {
+        hipLaunchKernelGGL(k_col_count, dim3(grid_for((size_t)nnz, 256)), dim3(256), 0, s, nnz,
+                           ja, cnt);
+        TSG_LAUNCH_CHECK();
+    }
+    exclusive_scan_i32(cnt, tp, (size_t)n, tmp, s);
+    if (nnz <= 0) return;
+    int *rowidx = ws.alloc_n<int>((size_t)nnz);
+    int *pos_in = ws.alloc_n<int>((size_t)nnz);
+    int *pos_out = ws.alloc_n<int>((size_t)nnz);
+    int *keys_out = ws.alloc_n<int>((size_t)nnz);
+    hipLaunchKernelGGL(k_row_index, dim3(grid_for((size_t)m * 8, 256)), dim3(256), 0, s, m, ia,
+                       rowidx);
+    TSG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_iota, dim3(grid_for((size_t)nnz, 256)), dim3(256), 0, s, nnz, pos_in);
+    TSG_LAUNCH_CHECK();
+    int end_bit = 1;
+    while (end_bit < 31 && (1ll << end_bit) < (long long)n) end_bit++;
+    size_t tbytes = 0;
+    TSG_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, ja, keys_out, pos_in, pos_out,
+                                                   (int)nnz, 0, end_bit, s));
+    void *tstore = ws.alloc(tbytes);
+    TSG_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tstore, tbytes, ja, keys_out, pos_in, pos_out,
+                                                   (int)nnz, 0, end_bit, s));
+    hipLaunchKernelGGL(k_gather_t, dim3(grid_for((size_t)nnz, 256)), dim3(256), 0, s, nnz,
+                       pos_out, rowidx, a, ti, tv, tpos);
+    TSG_LAUNCH_CHECK();
+}
