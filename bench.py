@@ -1,0 +1,220 @@
+"""Benchmark: FP64 SpGEMM throughput on one MI355X.
+
+Default workload: C = A * A^T with A the R-MAT matrix of n = 2^20 rows,
+16,000,000 edge draws, (a,b,c,d) = (.57,.19,.19,.05), duplicates merged,
+values U[-1,1], seed 1 (nnz(A) = 15,361,802; 19,250,644,998 intermediate
+products; nnz(C) ~ 9.1e9, so C's row pointers are int64).
+
+A step is one full SpGEMM through the device API -- symbolic phase (row
+pointers of C, nnz(C)) and numeric phase (columns and values) -- with A and
+B = A^T resident in HBM; C's column/value buffers are allocated once and
+rewritten every step.  GFLOP/s = 2 * intermediate products / step time
+(main_spgemm-col.cu:4787).
+
+Prints one JSON line (rank 0 only; this build runs on one GPU):
+  value         whole-job GFLOP/s
+  roofline      dominant kernel's algorithmic bytes / its average duration
+                (HIP events on the library's stream) against 8 TB/s
+  cpu_baseline  the reference's own CPU SpGEMM (oracle/_ref, OpenMP) on a
+                bounded, evenly strided sample of A's rows, same B
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_workload(name):
+    from tilespgemm_amd import gen
+    if name == "rmat20_aat":
+        A = gen.rmat(20, 16_000_000, seed=1)
+        return dict(name="R-MAT n=2^20, 16M draws, (.57,.19,.19,.05), U[-1,1], seed 1; C=A*A^T",
+                    m=1 << 20, n=1 << 20, A=A, aat=True)
+    if name.startswith("rmat") and name.endswith("_aat"):
+        s = int(name[4:-4])
+        A = gen.rmat(s, 16 * (1 << s) // 1, seed=1)
+        return dict(name=f"R-MAT n=2^{s}, {16 << s} draws, seed 1; C=A*A^T",
+                    m=1 << s, n=1 << s, A=A, aat=True)
+    if name == "laplacian256_aa":
+        A = gen.laplacian2d(256)
+        return dict(name="2D 5-point Laplacian 256x256; C=A*A", m=65536, n=65536, A=A, aat=False)
+    if name == "stencil27_64_aa":
+        A = gen.stencil27(64, seed=0)
+        return dict(name="27-point stencil 64^3, N(0,1) values, seed 0; C=A*A", m=262144,
+                    n=262144, A=A, aat=False)
+    raise SystemExit(f"unknown workload {name}")
+
+
+def csr_bytes(rows, nnz, ptr_bytes):
+    return (rows + 1) * ptr_bytes + nnz * 12
+
+
+def cpu_baseline(A, B, m, n, k, target_s=15.0):
+    """The reference thmkl_dcsrmultcsr (compiled from its sources into
+    oracle/_ref) on every stride-th row of A, timed as its main() does
+    (both phases plus the malloc between, thmkl_dcsrmultcsr.c:190-210)."""
+    from oracle import py_oracle as O
+    kind = "reference" if O.ref_available("fma") else "port"
+    ia, ja, a = (np.asarray(x) for x in A)
+    blen = np.diff(np.asarray(B[0], dtype=np.int64))
+
+    def sample(stride):
+        rows = np.arange(0, m, stride)
+        lens = (ia[rows + 1] - ia[rows]).astype(np.int64)
+        sp = np.zeros(rows.size + 1, dtype=np.int64)
+        np.cumsum(lens, out=sp[1:])
+        idx = np.concatenate([np.arange(ia[r], ia[r + 1]) for r in rows]) if rows.size else \
+            np.zeros(0, np.int64)
+        return rows.size, (sp, ja[idx], a[idx]), int(blen[ja[idx]].sum())
+
+    def run(ms, As):
+        t0 = time.perf_counter()
+        if kind == "reference":
+            O.ref_thmkl(ms, n, k, As, B, "fma")
+        else:
+            O.csr_spgemm(ms, k, As, B)
+        return time.perf_counter() - t0
+
+    stride = max(1, m // 256)
+    ms, As, prods = sample(stride)
+    t = run(ms, As)
+    scale = max(1.0, target_s / max(t, 1e-3))
+    stride2 = max(1, int(stride / scale))
+    if stride2 < stride:
+        ms, As, prods = sample(stride2)
+        t = run(ms, As)
+        stride = stride2
+    threads = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
+    return {"value": 2.0 * prods / t / 1e9, "unit": "GFLOP/s", "cores": threads, "kind": kind,
+            "sample": f"every {stride}th row of A ({ms} rows, {prods} intermediate products), "
+                      f"full B; {t:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="rmat20_aat")
+    ap.add_argument("--method", default="auto", choices=["auto", "tile", "row"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
+              "sharding of this path is a design note, not a build target.", file=sys.stderr)
+        raise SystemExit(2)
+
+    import torch
+    import tilespgemm_amd as tsg
+
+    t0 = time.time()
+    W = make_workload(args.workload)
+    m, n = W["m"], W["n"]
+    A = W["A"]
+    log(f"[bench] workload {args.workload}: nnz(A)={A[1].size} gen {time.time() - t0:.1f}s")
+    dev = "cuda:0"
+    dA = tsg.DeviceCsr.from_host(m, n, *A, device=dev)
+    method = {"auto": tsg.METHOD_AUTO, "tile": tsg.METHOD_TILE, "row": tsg.METHOD_ROW}[args.method]
+    h = tsg.Handle(0, method=method, timing=True)
+    if W["aat"]:
+        dB = h.transpose(dA)
+        k = m
+    else:
+        dB = dA
+        k = n
+    torch.cuda.synchronize()
+    products = h.intermediate_products(dA, dB)
+    log(f"[bench] intermediate products {products}")
+
+    # first symbolic: size C
+    ic, nnz_c = h.symbolic(dA, dB)
+    log(f"[bench] nnz(C)={nnz_c} method={h.method_used} "
+        f"workspace={h.workspace_bytes / 2**30:.2f} GiB")
+    jc = torch.empty(max(nnz_c, 1), dtype=torch.int32, device=dev)
+    c = torch.empty(max(nnz_c, 1), dtype=torch.float64, device=dev)
+
+    def step():
+        _, nz = h.symbolic(dA, dB, ic=ic)
+        assert nz == nnz_c
+        h.numeric(dA, dB, ic, jc, c)
+
+    for i in range(args.warmup):
+        step()
+        torch.cuda.synchronize()
+        log(f"[bench] warmup {i} done")
+    stage_sum = {}
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        step()
+        for kname, v in h.stage_times().items():   # waits for the step's events
+            stage_sum[kname] = stage_sum.get(kname, 0.0) + v
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    ms_step = elapsed * 1e3 / args.steps
+    gflops = 2.0 * products / (ms_step * 1e-3) / 1e9
+    stages = {kname: v / args.steps for kname, v in stage_sum.items()}
+    log(f"[bench] {ms_step:.2f} ms/step, {gflops:.1f} GFLOP/s, stages {stages}")
+
+    # roofline of the dominant kernel
+    dom = max(stages, key=stages.get)
+    ptr_c = 8
+    bytes_a = csr_bytes(m, A[1].size, 4)
+    bytes_b = csr_bytes(n, dB.nnz, 4)
+    bytes_c = csr_bytes(m, nnz_c, ptr_c)
+    whole = bytes_a + bytes_b + bytes_c
+    # the numeric kernels write C's columns and values and read A and B once
+    dom_bytes = {"row.big_numeric": bytes_a + bytes_b + 12 * nnz_c,
+                 "tile.step3": bytes_a + bytes_b + 12 * nnz_c}.get(dom, whole)
+    achieved = dom_bytes / (stages[dom] * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get(dom)
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": dom, "kernel_ms": round(stages[dom], 3),
+            "algorithmic_bytes": dom_bytes,
+            "whole_step_GBs": round(whole / (ms_step * 1e-3) / 1e9, 1)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        Bh = (dB.indptr.cpu().numpy(), dB.indices.cpu().numpy(), dB.data.cpu().numpy())
+        try:
+            cpu = cpu_baseline(A, Bh, m, n, k)
+            log(f"[bench] cpu baseline {cpu}")
+        except Exception as e:  # noqa: BLE001
+            log(f"[bench] cpu baseline failed: {e}")
+
+    out = {
+        "metric": "FP64 GFLOP/s", "value": round(gflops, 2), "unit": "GFLOP/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": W["name"], "nnz_a": int(A[1].size), "nnz_c": int(nnz_c),
+                   "intermediate_products": int(products),
+                   "method": {1: "tile", 2: "row"}.get(h.method_used, "?"),
+                   "stages_ms": {kk: round(v, 3) for kk, v in stages.items()}},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
