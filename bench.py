@@ -90,14 +90,13 @@ def cpu_baseline(A, B, m, n, k, target_s=15.0):
         return time.perf_counter() - t0
 
     stride = max(1, m // 256)
-    ms, As, prods = sample(stride)
-    t = run(ms, As)
-    scale = max(1.0, target_s / max(t, 1e-3))
-    stride2 = max(1, int(stride / scale))
-    if stride2 < stride:
-        ms, As, prods = sample(stride2)
+    while True:
+        ms, As, prods = sample(stride)
         t = run(ms, As)
-        stride = stride2
+        if t >= target_s / 3 or stride == 1:
+            break
+        # grow the sample towards target_s (at most 64x per round)
+        stride = max(1, int(stride * max(t, target_s / 64) / target_s))
     threads = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
     return {"value": 2.0 * prods / t / 1e9, "unit": "GFLOP/s", "cores": threads, "kind": kind,
             "sample": f"every {stride}th row of A ({ms} rows, {prods} intermediate products), "

@@ -222,6 +222,8 @@ int tsg_spgemm_symbolic(tsg_handle_t h, int m, int n, int k, const int *d_ia, co
             tile_symbolic(h->tile, A, B, b_is_a, d_ic, nnz_c, h->ws, h->stream, h->tm, h->cus);
         } else {
             row_symbolic(h->row, A, B, d_ic, nnz_c, h->ws, h->stream, h->tm, h->cus);
+            const bool b_validated = B.ptr != A.ptr || B.idx != A.idx || B.cols != A.cols;
+            h->row.b_strict = b_validated ? (vf & 0x400) == 0 : (vf & 0x4) == 0;
         }
     });
 }

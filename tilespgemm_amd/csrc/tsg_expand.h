@@ -21,10 +21,13 @@ __device__ __forceinline__ int brange_lo(const int *__restrict__ bptr, int strid
 // Visits the products of the A entries [pa, pe) of one row in A-row order,
 // G lanes at a time, optionally restricted to B columns [c0, c1).  The
 // A entries are taken in groups of G starting at pa + gofs, stride gstride.
-// f(active, col, a, b) is called once per batch by every lane of the group;
-// batches are issued in product order and lane order inside a batch is
-// product order.
-template <int G, bool RANGED, bool VALUES, class F>
+// f(active, col, a, b, single) is called once per batch by every lane of the
+// group; batches are issued in product order and lane order inside a batch is
+// product order.  U batches are gathered before the first is handed to f, so
+// each lane keeps U independent loads in flight.  single is true when all
+// active lanes of the batch come from the same A entry (one B row: distinct
+// columns whenever B's rows are duplicate-free).
+template <int G, int U, bool RANGED, bool VALUES, class F>
 __device__ __forceinline__ void for_products(int pa, int pe, int gofs, int gstride,
                                              const int *__restrict__ ja,
                                              const double *__restrict__ aval,
@@ -34,6 +37,8 @@ __device__ __forceinline__ void for_products(int pa, int pe, int gofs, int gstri
                                              const int *__restrict__ bptr, int bstride, int bs,
                                              int c0, int c1, F &&f) {
     const int l = __lane_id() & (G - 1);
+    const int gbase = __lane_id() & ~(G - 1);
+    const unsigned long long gmask = G == 64 ? ~0ull : ((1ull << G) - 1);
     for (int g = pa + gofs; g < pe; g += gstride) {
         const int p = g + l;
         const bool v = p < pe;
@@ -53,27 +58,43 @@ __device__ __forceinline__ void for_products(int pa, int pe, int gofs, int gstri
         const int cnt = hi - lo;
         const int incl = group_incl_scan<G>(cnt);
         const int total = __shfl(incl, G - 1, G);
-        for (int base = 0; base < total; base += G) {
-            const int q = base + l;
-            const bool act = q < total;
-            int s = 0;
+        for (int base = 0; base < total; base += G * U) {
+            int e[U], sl[U];
+            bool act[U];
+            double as[U];
 #pragma unroll
-            for (int step = G / 2; step > 0; step >>= 1) {
-                int probe = __shfl(incl, s + step - 1, G);
-                if (probe <= q) s += step;
+            for (int u = 0; u < U; u++) {
+                const int q = base + u * G + l;
+                act[u] = q < total;
+                int s = 0;
+#pragma unroll
+                for (int step = G / 2; step > 0; step >>= 1) {
+                    int probe = __shfl(incl, s + step - 1, G);
+                    if (probe <= q) s += step;
+                }
+                const int incl_s = __shfl(incl, s, G);
+                const int cnt_s = __shfl(cnt, s, G);
+                const int lo_s = __shfl(lo, s, G);
+                as[u] = VALUES ? __shfl(a, s, G) : 0.0;
+                e[u] = lo_s + (q - (incl_s - cnt_s));
+                sl[u] = s;
             }
-            const int incl_s = __shfl(incl, s, G);
-            const int cnt_s = __shfl(cnt, s, G);
-            const int lo_s = __shfl(lo, s, G);
-            const double a_s = VALUES ? __shfl(a, s, G) : 0.0;
-            const int e = lo_s + (q - (incl_s - cnt_s));
-            int col = 0;
-            double b = 0.0;
-            if (act) {
-                col = jb[e];
-                if (VALUES) b = bval[e];
+            int col[U];
+            double bv[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                col[u] = act[u] ? jb[e[u]] : 0;
+                bv[u] = (VALUES && act[u]) ? bval[e[u]] : 0.0;
             }
-            f(act, col, a_s, b);
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                if (base + u * G < total) {
+                    const int s0 = __shfl(sl[u], 0, G);
+                    const unsigned long long diff = __ballot(act[u] && sl[u] != s0);
+                    const bool single = ((diff >> gbase) & gmask) == 0;
+                    f(act[u], col[u], as[u], bv[u], single);
+                }
+            }
         }
     }
 }
@@ -111,10 +132,17 @@ __device__ __forceinline__ int hash_find(const int *keys, int col) {
 // Ordered read-modify-write of vals[slot] by the active lanes of a batch:
 // lanes sharing a tag bucket go one per round, lowest lane (earliest product)
 // first, so every slot sees its products in product order.
+// With single (all active lanes from one duplicate-free B row, hence distinct
+// slots) the update is done in one step.
 template <bool FUSED, int NTAGS>
 __device__ __forceinline__ void ordered_accumulate(bool act, int slot, double a, double b,
                                                    double *vals, unsigned *tags,
-                                                   unsigned me) {
+                                                   unsigned me, bool single) {
+    if (single) {
+        if (act && slot >= 0) vals[slot] = acc_step<FUSED>(vals[slot], a, b);
+        wave_lds_fence();
+        return;
+    }
     bool pending = act && slot >= 0;
     const unsigned bucket = (unsigned)slot & (NTAGS - 1);
     while (__ballot(pending)) {

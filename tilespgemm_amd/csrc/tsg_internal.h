@@ -100,6 +100,7 @@ struct StageTimer {
 // ------------------------------------------------------------ row path
 struct RowPlan {
     bool valid = false;
+    bool b_strict = false;           // B rows strictly ascending (no duplicates)
     int m = 0, n = 0, k = 0;
     // per-row products and output counts
     int64_t *rowprod = nullptr;      // [m]

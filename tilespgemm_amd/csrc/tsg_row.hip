@@ -38,10 +38,14 @@ constexpr int BIG_PASS_SHIFT = 20;     // one symbolic pass covers 2^20 columns
 constexpr int BIG_T = 1024;            // threads of the big symbolic kernel
 constexpr int ITEM_CAP = 1024;         // outputs per item
 constexpr int ITEM_SLOTS = 2048;       // hash slots per item (load <= 1/2)
-constexpr int ITEM_WBITS_SHIFT = 16;   // item width <= 65536 columns
+constexpr int ITEM_WBITS_SHIFT = 15;   // item width <= 32768 columns
 constexpr int ITEM_WORDS = (1 << ITEM_WBITS_SHIFT) / 32;
 constexpr int ITEM_TAGS = 256;
-constexpr int ITEM_WAVES = 4;
+constexpr int ITEM_WAVES = 1;          // waves per block (one item per wave)
+constexpr int ITEM_BLOCKS_PER_CU = 6;  // LDS-bound residency (~25 KB per wave)
+constexpr int SMALL_U = 2;             // product batches in flight per lane
+constexpr int BIG_SYM_U = 8;
+constexpr int ITEM_U = 4;
 
 struct BinStarts { int s[NBINS + 1]; };
 
@@ -94,9 +98,9 @@ __global__ void __launch_bounds__(256) k_small_symbolic(int nrows, const int *__
     if (r >= nrows) return;
     const int row = rows[r];
     int created_cnt = 0;
-    for_products<G, false, false>(ia[row], ia[row + 1], 0, G, ja, nullptr, ib, jb, nullptr,
-                                  nullptr, 0, 0, 0, 0,
-                                  [&](bool act, int col, double, double) {
+    for_products<G, SMALL_U, false, false>(ia[row], ia[row + 1], 0, G, ja, nullptr, ib, jb,
+                                           nullptr, nullptr, 0, 0, 0, 0,
+                                           [&](bool act, int col, double, double, bool) {
                                       if (act) {
                                           bool created;
                                           hash_insert<SLOTS>(keys, col, created);
@@ -118,7 +122,7 @@ __global__ void __launch_bounds__(256) k_small_numeric(int nrows, const int *__r
                                                        const double *__restrict__ b,
                                                        const int64_t *__restrict__ ic,
                                                        int *__restrict__ jc,
-                                                       double *__restrict__ c) {
+                                                       double *__restrict__ c, int b_strict) {
     constexpr int RPB = 256 / G;
     constexpr int NT = 4 * G;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -136,8 +140,9 @@ __global__ void __launch_bounds__(256) k_small_numeric(int nrows, const int *__r
     wave_lds_fence();
     if (r >= nrows) return;
     const int row = rows[r];
-    for_products<G, false, true>(ia[row], ia[row + 1], 0, G, ja, a, ib, jb, b, nullptr, 0, 0, 0,
-                                 0, [&](bool act, int col, double av, double bv) {
+    for_products<G, SMALL_U, false, true>(ia[row], ia[row + 1], 0, G, ja, a, ib, jb, b, nullptr,
+                                          0, 0, 0, 0,
+                                          [&](bool act, int col, double av, double bv, bool single) {
                                      int slot = 0;
                                      if (act) {
                                          bool created;
@@ -145,7 +150,7 @@ __global__ void __launch_bounds__(256) k_small_numeric(int nrows, const int *__r
                                      }
                                      wave_lds_fence();
                                      ordered_accumulate<FUSED, NT>(act, slot, av, bv, vals, tags,
-                                                                   (unsigned)l);
+                                                                   (unsigned)l, single && b_strict);
                                  });
     // empty slots sort last
     for (int i = l; i < SLOTS; i += G)
@@ -184,9 +189,9 @@ __global__ void __launch_bounds__(BIG_T) k_big_symbolic(int nbig, const int *__r
             const int words = (pass1 - pass0 + 31) >> 5;
             for (int i = threadIdx.x; i < words; i += BIG_T) bits[i] = 0u;
             __syncthreads();
-            for_products<64, false, false>(pa, pe, wave * 64, NW * 64, ja, nullptr, ib, jb,
-                                           nullptr, nullptr, 0, 0, 0, 0,
-                                           [&](bool act, int col, double, double) {
+            for_products<64, BIG_SYM_U, false, false>(pa, pe, wave * 64, NW * 64, ja, nullptr, ib,
+                                                      jb, nullptr, nullptr, 0, 0, 0, 0,
+                                                      [&](bool act, int col, double, double, bool) {
                                                if (act && col >= pass0 && col < pass1) {
                                                    int c = col - pass0;
                                                    atomicOr(&bits[c >> 5], 1u << (c & 31));
@@ -278,13 +283,17 @@ __global__ void k_items(int nbig, const int *__restrict__ rows, int k, int nsub,
 }
 
 // --------------------------------------------------------- big rows: numeric
-// One wave per item (grid-stride over items), ITEM_WAVES waves per block,
-// each with its own LDS hash, column bitmap and tag table.
+// One wave per item (grid-stride over items).  LDS per wave: a key hash with
+// a 16-bit id per slot, values compacted by insertion id (only the item's
+// <= ITEM_CAP outputs are stored), the item's column bitmap (emission order)
+// and the tag table of the ordered accumulation.
 struct ItemSmem {
-    double vals[ITEM_SLOTS];
+    double vals[ITEM_CAP];
     int keys[ITEM_SLOTS];
+    unsigned short ids[ITEM_SLOTS];
     unsigned bits[ITEM_WORDS];
     unsigned tags[ITEM_TAGS];
+    int nid;
 };
 
 template <bool FUSED>
@@ -293,7 +302,7 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric(
     const int *__restrict__ ia, const int *__restrict__ ja, const double *__restrict__ a,
     const int *__restrict__ ib, const int *__restrict__ jb, const double *__restrict__ b,
     const int *__restrict__ bptr, int bstride, int bs, int *__restrict__ jc,
-    double *__restrict__ c) {
+    double *__restrict__ c, int b_strict) {
     extern __shared__ __align__(16) unsigned char smem[];
     ItemSmem &S = reinterpret_cast<ItemSmem *>(smem)[threadIdx.x >> 6];
     const int l = __lane_id();
@@ -304,24 +313,29 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric(
         const int4 item = items[it];
         const int row = item.x, c0 = item.y, c1 = item.z;
         const int nwords = (c1 - c0 + 31) >> 5;
-        for (int i = l; i < ITEM_SLOTS; i += 64) { S.keys[i] = -1; S.vals[i] = 0.0; }
+        for (int i = l; i < ITEM_SLOTS; i += 64) S.keys[i] = -1;
         for (int i = l; i < nwords; i += 64) S.bits[i] = 0u;
+        if (l == 0) S.nid = 0;
         wave_lds_fence();
-        for_products<64, true, true>(
+        for_products<64, ITEM_U, true, true>(
             ia[row], ia[row + 1], 0, 64, ja, a, ib, jb, b, bptr, bstride, bs, c0, c1,
-            [&](bool act, int col, double av, double bv) {
-                int slot = 0;
+            [&](bool act, int col, double av, double bv, bool single) {
+                int slot = -1;
                 if (act) {
                     bool created;
                     slot = hash_insert<ITEM_SLOTS>(S.keys, col, created);
                     if (created) {
-                        int cc = col - c0;
+                        const int id = atomicAdd(&S.nid, 1);
+                        S.ids[slot] = (unsigned short)id;
+                        S.vals[id] = 0.0;
+                        const int cc = col - c0;
                         atomicOr(&S.bits[cc >> 5], 1u << (cc & 31));
                     }
                 }
                 wave_lds_fence();
-                ordered_accumulate<FUSED, ITEM_TAGS>(act, slot, av, bv, S.vals, S.tags,
-                                                     (unsigned)l);
+                const int id = slot >= 0 ? (int)S.ids[slot] : -1;
+                ordered_accumulate<FUSED, ITEM_TAGS>(act, id, av, bv, S.vals, S.tags,
+                                                     (unsigned)l, single && b_strict);
             });
         wave_lds_fence();
         // emit in column order: bitmap words -> ranks -> hash lookups
@@ -338,7 +352,7 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric(
                 const int col = c0 + (w << 5) + bit;
                 const int slot = hash_find<ITEM_SLOTS>(S.keys, col);
                 jc[pos] = col;
-                c[pos] = slot >= 0 ? S.vals[slot] : 0.0;
+                c[pos] = slot >= 0 ? S.vals[S.ids[slot]] : 0.0;
                 pos++;
             }
             o += __shfl(incl, 63, 64);
@@ -435,12 +449,13 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
         TSG_LAUNCH_CHECK();
         tm.end();
 
-        // B block pointers: finest block width >= 1024 whose table stays <= 1 GiB
+        // B block pointers at the sub-block width (item boundaries then need
+        // no search) while the table stays <= 6 GiB, coarser beyond.
         tm.begin("row.bptr");
         int bs = SUB_SHIFT;
         while (bs < 30) {
             size_t nblk = ((size_t)k + (1u << bs) - 1) >> bs;
-            if ((size_t)n * (nblk + 1) * sizeof(int) <= (size_t(1) << 30)) break;
+            if ((size_t)n * (nblk + 1) * sizeof(int) <= (size_t(6) << 30)) break;
             bs++;
         }
         p.blk_shift = bs;
@@ -500,7 +515,8 @@ static void row_numeric_t(RowPlan &p, const CsrIn &A, const CsrIn &B, const int6
             auto kern = k_small_numeric<16, 128, FUSED>;
             set_smem(kern, sm);
             hipLaunchKernelGGL(kern, dim3(grid_for(n1, 16)), dim3(256), sm, s, n1, bins + st[1],
-                               A.ptr, A.idx, A.val, B.ptr, B.idx, B.val, d_ic, d_jc, d_c);
+                               A.ptr, A.idx, A.val, B.ptr, B.idx, B.val, d_ic, d_jc, d_c,
+                               (int)p.b_strict);
             TSG_LAUNCH_CHECK();
         }
         int n2 = st[3] - st[2];
@@ -509,7 +525,8 @@ static void row_numeric_t(RowPlan &p, const CsrIn &A, const CsrIn &B, const int6
             auto kern = k_small_numeric<32, 512, FUSED>;
             set_smem(kern, sm);
             hipLaunchKernelGGL(kern, dim3(grid_for(n2, 8)), dim3(256), sm, s, n2, bins + st[2],
-                               A.ptr, A.idx, A.val, B.ptr, B.idx, B.val, d_ic, d_jc, d_c);
+                               A.ptr, A.idx, A.val, B.ptr, B.idx, B.val, d_ic, d_jc, d_c,
+                               (int)p.b_strict);
             TSG_LAUNCH_CHECK();
         }
         int n3 = st[4] - st[3];
@@ -518,7 +535,8 @@ static void row_numeric_t(RowPlan &p, const CsrIn &A, const CsrIn &B, const int6
             auto kern = k_small_numeric<64, 2048, FUSED>;
             set_smem(kern, sm);
             hipLaunchKernelGGL(kern, dim3(grid_for(n3, 4)), dim3(256), sm, s, n3, bins + st[3],
-                               A.ptr, A.idx, A.val, B.ptr, B.idx, B.val, d_ic, d_jc, d_c);
+                               A.ptr, A.idx, A.val, B.ptr, B.idx, B.val, d_ic, d_jc, d_c,
+                               (int)p.b_strict);
             TSG_LAUNCH_CHECK();
         }
     }
@@ -529,10 +547,10 @@ static void row_numeric_t(RowPlan &p, const CsrIn &A, const CsrIn &B, const int6
         auto kern = k_big_numeric<FUSED>;
         set_smem(kern, sm);
         const unsigned grid = (unsigned)std::min<int64_t>(
-            (p.nitems + ITEM_WAVES - 1) / ITEM_WAVES, (int64_t)cus * 8);
+            (p.nitems + ITEM_WAVES - 1) / ITEM_WAVES, (int64_t)cus * ITEM_BLOCKS_PER_CU);
         hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * ITEM_WAVES), sm, s, p.nitems, p.items,
                            p.item_off, A.ptr, A.idx, A.val, B.ptr, B.idx, B.val, p.bptr,
-                           p.nblk + 1, p.blk_shift, d_jc, d_c);
+                           p.nblk + 1, p.blk_shift, d_jc, d_c, (int)p.b_strict);
         TSG_LAUNCH_CHECK();
         tm.end();
     }
