@@ -89,14 +89,18 @@ def cpu_baseline(A, B, m, n, k, target_s=15.0):
             O.csr_spgemm(ms, k, As, B)
         return time.perf_counter() - t0
 
-    stride = max(1, m // 256)
+    # the reference keeps C's row pointers in int32 (common.h:15-17, no overflow
+    # check): keep the sample's intermediate products (>= its nnz(C)) < 1.5e9
+    total_prods = int(blen[ja].sum())
+    min_stride = max(1, -(-total_prods // 1_500_000_000))
+    stride = max(min_stride, m // 256)
     while True:
         ms, As, prods = sample(stride)
         t = run(ms, As)
-        if t >= target_s / 3 or stride == 1:
+        if t >= target_s / 3 or stride == min_stride:
             break
         # grow the sample towards target_s (at most 64x per round)
-        stride = max(1, int(stride * max(t, target_s / 64) / target_s))
+        stride = max(min_stride, int(stride * max(t, target_s / 64) / target_s))
     threads = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
     return {"value": 2.0 * prods / t / 1e9, "unit": "GFLOP/s", "cores": threads, "kind": kind,
             "sample": f"every {stride}th row of A ({ms} rows, {prods} intermediate products), "

@@ -39,14 +39,22 @@ __device__ __forceinline__ void for_products(int pa, int pe, int gofs, int gstri
     const int l = __lane_id() & (G - 1);
     const int gbase = __lane_id() & ~(G - 1);
     const unsigned long long gmask = G == 64 ? ~0ull : ((1ull << G) - 1);
-    for (int g = pa + gofs; g < pe; g += gstride) {
+    // Software pipeline over groups of A entries: the A entry (t, a) of group
+    // g+2 and the B range of group g+1 are loaded while group g's products
+    // are processed.
+    auto load_t = [&](int g, int &t, double &a) {
         const int p = g + l;
-        const bool v = p < pe;
-        int lo = 0, hi = 0;
-        double a = 0.0;
-        if (v) {
-            const int t = ja[p];
+        t = -1;
+        a = 0.0;
+        if (g < pe && p < pe) {
+            t = ja[p];
             if (VALUES) a = aval[p];
+        }
+    };
+    auto load_range = [&](int t, int &lo, int &hi) {
+        lo = 0;
+        hi = 0;
+        if (t >= 0) {
             if (RANGED) {
                 lo = brange_lo(bptr, bstride, bs, jb, t, c0);
                 hi = brange_lo(bptr, bstride, bs, jb, t, c1);
@@ -55,6 +63,20 @@ __device__ __forceinline__ void for_products(int pa, int pe, int gofs, int gstri
                 hi = ib[t + 1];
             }
         }
+    };
+    int g = pa + gofs;
+    int t0, t1;
+    double a0, a1;
+    load_t(g, t0, a0);
+    load_t(g + gstride, t1, a1);
+    int lo, hi;
+    load_range(t0, lo, hi);
+    for (; g < pe; g += gstride) {
+        const double a = a0;
+        int lo_n, hi_n, t2;
+        double a2;
+        load_range(t1, lo_n, hi_n);                 // next group's B ranges
+        load_t(g + 2 * gstride, t2, a2);            // group after next
         const int cnt = hi - lo;
         const int incl = group_incl_scan<G>(cnt);
         const int total = __shfl(incl, G - 1, G);
@@ -96,6 +118,11 @@ __device__ __forceinline__ void for_products(int pa, int pe, int gofs, int gstri
                 }
             }
         }
+        lo = lo_n;
+        hi = hi_n;
+        t1 = t2;
+        a0 = a1;
+        a1 = a2;
     }
 }
 
