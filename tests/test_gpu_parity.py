@@ -245,3 +245,18 @@ def test_reference_host_entry(name):
     ic64 = tsg.dcsrmultcsr_i64(1, m, n, k, A, B)
     jc64, c64 = tsg.dcsrmultcsr_i64(0, m, n, k, A, B, ic=ic64)
     assert_csr_equal((ic64, jc64, c64), (g["ic"], g["jc"], g["c_fma"]))
+
+
+def test_auto_method_choice(handle):
+    """AUTO takes the tile path for tile-dense A (stencils) and the row path
+    for hypersparse A (R-MAT), with identical results either way."""
+    L = gen.laplacian2d(48)
+    n = 48 * 48
+    got = run(handle, n, n, n, L, L, tsg.METHOD_AUTO, same=True)
+    assert handle.method_used == tsg.METHOD_TILE
+    assert_csr_equal(got, O.csr_spgemm(n, n, L, L))
+    R = gen.rmat(11, 20_000, seed=4)
+    RT = gen.transpose(2048, 2048, *R)
+    got = run(handle, 2048, 2048, 2048, R, RT, tsg.METHOD_AUTO)
+    assert handle.method_used == tsg.METHOD_ROW
+    assert_csr_equal(got, O.csr_spgemm(2048, 2048, R, RT))

@@ -27,6 +27,9 @@ __device__ __forceinline__ int brange_lo(const int *__restrict__ bptr, int strid
 // each lane keeps U independent loads in flight.  single is true when all
 // active lanes of the batch come from the same A entry (one B row: distinct
 // columns whenever B's rows are duplicate-free).
+// Batch striding (bofs, bstep): only batches bofs, bofs + bstep, ... of
+// every group are visited, so several waves can share one group's products
+// (used where the order of products does not matter).
 template <int G, int U, bool RANGED, bool VALUES, class F>
 __device__ __forceinline__ void for_products(int pa, int pe, int gofs, int gstride,
                                              const int *__restrict__ ja,
@@ -35,7 +38,8 @@ __device__ __forceinline__ void for_products(int pa, int pe, int gofs, int gstri
                                              const int *__restrict__ jb,
                                              const double *__restrict__ bval,
                                              const int *__restrict__ bptr, int bstride, int bs,
-                                             int c0, int c1, F &&f) {
+                                             int c0, int c1, F &&f, int bofs = 0,
+                                             int bstep = 1) {
     const int l = __lane_id() & (G - 1);
     const int gbase = __lane_id() & ~(G - 1);
     const unsigned long long gmask = G == 64 ? ~0ull : ((1ull << G) - 1);
@@ -80,13 +84,13 @@ __device__ __forceinline__ void for_products(int pa, int pe, int gofs, int gstri
         const int cnt = hi - lo;
         const int incl = group_incl_scan<G>(cnt);
         const int total = __shfl(incl, G - 1, G);
-        for (int base = 0; base < total; base += G * U) {
+        for (int base = bofs * G; base < total; base += G * U * bstep) {
             int e[U], sl[U];
             bool act[U];
             double as[U];
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                const int q = base + u * G + l;
+                const int q = base + u * G * bstep + l;
                 act[u] = q < total;
                 int s = 0;
 #pragma unroll
@@ -110,7 +114,7 @@ __device__ __forceinline__ void for_products(int pa, int pe, int gofs, int gstri
             }
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                if (base + u * G < total) {
+                if (base + u * G * bstep < total) {
                     const int s0 = __shfl(sl[u], 0, G);
                     const unsigned long long diff = __ballot(act[u] && sl[u] != s0);
                     const bool single = ((diff >> gbase) & gmask) == 0;

@@ -122,7 +122,11 @@ struct RowPlan {
     // items
     int64_t nitems = 0;
     int64_t *item_cnt = nullptr;     // [nbig+1] items per big row -> offsets
-    int4 *items = nullptr;           // {row, c0, c1, nnz}
+    int4 *items = nullptr;           // {row, c0, c1, big-row index}
+    // per big row column bitmaps kept from the symbolic phase (rank mode)
+    bool use_rank = false;
+    unsigned *rowbits = nullptr;     // [nbig * rowbits_words]
+    int64_t rowbits_words = 0;
     int64_t *item_off = nullptr;     // output offset (within C) per item
 };
 

@@ -43,6 +43,7 @@ constexpr int ITEM_WORDS = (1 << ITEM_WBITS_SHIFT) / 32;
 constexpr int ITEM_TAGS = 256;
 constexpr int ITEM_WAVES = 1;          // waves per block (one item per wave)
 constexpr int ITEM_BLOCKS_PER_CU = 6;  // LDS-bound residency (~25 KB per wave)
+constexpr int RANK_BLOCKS_PER_CU = 7;  // rank-indexed items: 20 KB per wave
 constexpr int SMALL_U = 2;             // product batches in flight per lane
 constexpr int BIG_SYM_U = 8;
 constexpr int ITEM_U = 4;
@@ -174,7 +175,9 @@ __global__ void __launch_bounds__(BIG_T) k_big_symbolic(int nbig, const int *__r
                                                         const int *__restrict__ ib,
                                                         const int *__restrict__ jb, int nsub,
                                                         uint16_t *__restrict__ subcnt,
-                                                        int64_t *__restrict__ rownnz) {
+                                                        int64_t *__restrict__ rownnz,
+                                                        unsigned *__restrict__ rowbits,
+                                                        int64_t rowbits_words) {
     extern __shared__ __align__(16) unsigned char smem[];
     unsigned *bits = reinterpret_cast<unsigned *>(smem);
     __shared__ int s_total[BIG_T / 64];
@@ -189,15 +192,23 @@ __global__ void __launch_bounds__(BIG_T) k_big_symbolic(int nbig, const int *__r
             const int words = (pass1 - pass0 + 31) >> 5;
             for (int i = threadIdx.x; i < words; i += BIG_T) bits[i] = 0u;
             __syncthreads();
-            for_products<64, BIG_SYM_U, false, false>(pa, pe, wave * 64, NW * 64, ja, nullptr, ib,
-                                                      jb, nullptr, nullptr, 0, 0, 0, 0,
-                                                      [&](bool act, int col, double, double, bool) {
-                                               if (act && col >= pass0 && col < pass1) {
-                                                   int c = col - pass0;
-                                                   atomicOr(&bits[c >> 5], 1u << (c & 31));
-                                               }
-                                           });
+            // every wave walks every group of A entries and takes every NW-th
+            // batch of its products: the row's work splits evenly over the
+            // block however few A entries the row has
+            for_products<64, BIG_SYM_U, false, false>(
+                pa, pe, 0, 64, ja, nullptr, ib, jb, nullptr, nullptr, 0, 0, 0, 0,
+                [&](bool act, int col, double, double, bool) {
+                    if (act && col >= pass0 && col < pass1) {
+                        int c = col - pass0;
+                        atomicOr(&bits[c >> 5], 1u << (c & 31));
+                    }
+                },
+                wave, NW);
             __syncthreads();
+            if (rowbits) {
+                unsigned *dst = rowbits + (size_t)r * rowbits_words + (pass0 >> 5);
+                for (int i = threadIdx.x; i < words; i += BIG_T) dst[i] = bits[i];
+            }
             const int nsb = (pass1 - pass0 + (1 << SUB_SHIFT) - 1) >> SUB_SHIFT;
             int part = 0;
             for (int sb = threadIdx.x; sb < nsb; sb += BIG_T) {
@@ -240,7 +251,8 @@ __global__ void __launch_bounds__(256) k_bptr(int n, const int *__restrict__ ib,
 // ------------------------------------------------------------- item plan
 // Greedy split of each big row into items: runs of sub-blocks with at most
 // ITEM_CAP outputs and at most ITEM_WBITS columns; leading empty sub-blocks
-// are skipped.  FILL=false counts, FILL=true writes.
+// are skipped.  FILL=false counts, FILL=true writes {row, c0, c1, big-row
+// index}.
 template <bool FILL>
 __global__ void k_items(int nbig, const int *__restrict__ rows, int k, int nsub,
                         const uint16_t *__restrict__ subcnt, const int64_t *__restrict__ ic,
@@ -259,7 +271,7 @@ __global__ void k_items(int nbig, const int *__restrict__ rows, int k, int nsub,
         const int cs = sc[sb];
         if (start >= 0 && (cnt + cs > ITEM_CAP || sb - start >= WMAX_SB)) {
             if (FILL) {
-                items[w] = make_int4(row, start << SUB_SHIFT, min(k, sb << SUB_SHIFT), cnt);
+                items[w] = make_int4(row, start << SUB_SHIFT, min(k, sb << SUB_SHIFT), r);
                 item_off[w] = out;
                 w++;
                 out += cnt;
@@ -274,7 +286,7 @@ __global__ void k_items(int nbig, const int *__restrict__ rows, int k, int nsub,
     }
     if (start >= 0 && cnt > 0) {
         if (FILL) {
-            items[w] = make_int4(row, start << SUB_SHIFT, k, cnt);
+            items[w] = make_int4(row, start << SUB_SHIFT, k, r);
             item_off[w] = out;
         }
         nitem++;
@@ -361,6 +373,102 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric(
     }
 }
 
+// Rank-indexed variant: the symbolic phase stored every big row's column
+// bitmap, so an item's outputs are known before its products are visited.
+// The rank of column j inside the item is prefix[w] + popc(bits[w] below j)
+// -- one LDS read of a {bits, prefix} pair -- and values accumulate in a
+// dense rank-ordered array that is written out coalesced.
+struct RankSmem {
+    uint2 words[ITEM_WORDS];      // {bitmap word, outputs before it}
+    double vals[ITEM_CAP];
+    unsigned tags[ITEM_CAP];
+};
+
+template <bool FUSED>
+__global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric_rank(
+    int64_t nitems, const int4 *__restrict__ items, const int64_t *__restrict__ item_off,
+    const unsigned *__restrict__ rowbits, int64_t rowbits_words,
+    const int *__restrict__ ia, const int *__restrict__ ja, const double *__restrict__ a,
+    const int *__restrict__ ib, const int *__restrict__ jb, const double *__restrict__ b,
+    const int *__restrict__ bptr, int bstride, int bs, int *__restrict__ jc,
+    double *__restrict__ c, int b_strict) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    RankSmem &S = reinterpret_cast<RankSmem *>(smem)[threadIdx.x >> 6];
+    const int l = __lane_id();
+    const int64_t wave_id = (int64_t)blockIdx.x * ITEM_WAVES + (threadIdx.x >> 6);
+    const int64_t nwaves = (int64_t)gridDim.x * ITEM_WAVES;
+    for (int64_t it = wave_id; it < nitems; it += nwaves) {
+        const int4 item = items[it];
+        const int row = item.x, c0 = item.y, c1 = item.z, big = item.w;
+        const int nwords = (c1 - c0 + 31) >> 5;
+        const unsigned *src = rowbits + (size_t)big * rowbits_words + (c0 >> 5);
+        int cnt = 0;
+        for (int w0 = 0; w0 < nwords; w0 += 64) {
+            const int w = w0 + l;
+            unsigned word = 0u;
+            if (w < nwords) {
+                word = src[w];
+                const int lastbits = (c1 - c0) - (w << 5);
+                if (lastbits < 32) word &= (1u << lastbits) - 1u;
+            }
+            const int pc = __popc(word);
+            const int incl = wave_incl_scan(pc);
+            if (w < nwords) S.words[w] = make_uint2(word, (unsigned)(cnt + incl - pc));
+            cnt += __shfl(incl, 63, 64);
+        }
+        for (int i = l; i < cnt; i += 64) {
+            S.vals[i] = 0.0;
+            S.tags[i] = 0xffffffffu;
+        }
+        wave_lds_fence();
+        for_products<64, ITEM_U, true, true>(
+            ia[row], ia[row + 1], 0, 64, ja, a, ib, jb, b, bptr, bstride, bs, c0, c1,
+            [&](bool act, int col, double av, double bv, bool single) {
+                int rank = -1;
+                if (act) {
+                    const int cc = col - c0;
+                    const uint2 wd = S.words[cc >> 5];
+                    rank = (int)wd.y + __popc(wd.x & ((1u << (cc & 31)) - 1u));
+                }
+                if (single && b_strict) {
+                    if (rank >= 0) S.vals[rank] = acc_step<FUSED>(S.vals[rank], av, bv);
+                    wave_lds_fence();
+                    return;
+                }
+                bool pending = rank >= 0;
+                while (__ballot(pending)) {
+                    if (pending) atomicMin(&S.tags[rank], (unsigned)l);
+                    wave_lds_fence();
+                    const bool win = pending && S.tags[rank] == (unsigned)l;
+                    wave_lds_fence();
+                    if (win) {
+                        S.vals[rank] = acc_step<FUSED>(S.vals[rank], av, bv);
+                        S.tags[rank] = 0xffffffffu;
+                        pending = false;
+                    }
+                    wave_lds_fence();
+                }
+            });
+        wave_lds_fence();
+        const int64_t o = item_off[it];
+        for (int i = l; i < cnt; i += 64) c[o + i] = S.vals[i];
+        for (int w0 = 0; w0 < nwords; w0 += 64) {
+            const int w = w0 + l;
+            if (w < nwords) {
+                uint2 wd = S.words[w];
+                int64_t pos = o + wd.y;
+                unsigned word = wd.x;
+                while (word) {
+                    const int bit = __ffs(word) - 1;
+                    word &= word - 1;
+                    jc[pos++] = c0 + (w << 5) + bit;
+                }
+            }
+        }
+        wave_lds_fence();
+    }
+}
+
 template <class K>
 void set_smem(K kernel, size_t bytes) {
     TSG_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
@@ -441,11 +549,22 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
         tm.begin("row.big_symbolic");
         p.subcnt = ws.alloc_n<uint16_t>((size_t)p.nbig * p.nsub);
         const int *rowsb = p.bin_rows + st.s[4];
+        // keep every big row's column bitmap for the rank-indexed numeric
+        // phase when it fits a budget of device memory; else the numeric
+        // phase uses per-item hash tables
+        p.rowbits_words = ((int64_t)k + 31) / 32;
+        const size_t bits_bytes = (size_t)p.nbig * (size_t)p.rowbits_words * 4;
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) { (void)hipGetLastError(); free_b = 0; }
+        const size_t budget = std::min<size_t>(size_t(48) << 30, free_b / 10 * 4);
+        p.use_rank = bits_bytes <= budget;
+        p.rowbits = p.use_rank ? ws.alloc_n<unsigned>(bits_bytes / 4) : nullptr;
         const size_t smem = sizeof(unsigned) * (size_t)((std::min(k, 1 << BIG_PASS_SHIFT) + 31) / 32);
         set_smem(k_big_symbolic, smem);
         const unsigned grid = (unsigned)std::min(p.nbig, cus * 1);
         hipLaunchKernelGGL(k_big_symbolic, dim3(grid), dim3(BIG_T), smem, s, p.nbig, rowsb, k,
-                           A.ptr, A.idx, B.ptr, B.idx, p.nsub, p.subcnt, p.rownnz);
+                           A.ptr, A.idx, B.ptr, B.idx, p.nsub, p.subcnt, p.rownnz, p.rowbits,
+                           p.rowbits_words);
         TSG_LAUNCH_CHECK();
         tm.end();
 
@@ -541,7 +660,20 @@ static void row_numeric_t(RowPlan &p, const CsrIn &A, const CsrIn &B, const int6
         }
     }
     tm.end();
-    if (p.nitems > 0) {
+    if (p.nitems > 0 && p.use_rank) {
+        tm.begin("row.big_numeric");
+        constexpr size_t sm = sizeof(RankSmem) * ITEM_WAVES;
+        auto kern = k_big_numeric_rank<FUSED>;
+        set_smem(kern, sm);
+        const unsigned grid = (unsigned)std::min<int64_t>(
+            (p.nitems + ITEM_WAVES - 1) / ITEM_WAVES, (int64_t)cus * RANK_BLOCKS_PER_CU);
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * ITEM_WAVES), sm, s, p.nitems, p.items,
+                           p.item_off, p.rowbits, p.rowbits_words, A.ptr, A.idx, A.val, B.ptr,
+                           B.idx, B.val, p.bptr, p.nblk + 1, p.blk_shift, d_jc, d_c,
+                           (int)p.b_strict);
+        TSG_LAUNCH_CHECK();
+        tm.end();
+    } else if (p.nitems > 0) {
         tm.begin("row.big_numeric");
         constexpr size_t sm = sizeof(ItemSmem) * ITEM_WAVES;
         auto kern = k_big_numeric<FUSED>;

@@ -100,6 +100,48 @@ __global__ void __launch_bounds__(256) k_tile_convert(
     if (FILL && r == 0 && I == blkm - 1) tile_off[t] = off;
 }
 
+// Tile count of each tile row through an LDS bitmap over its tile columns
+// (all entries of a tile row are contiguous in CSR).  One wave per tile row;
+// the bitmap window covers 65536 tile columns per pass.
+constexpr int TC_WAVES = 4;
+constexpr int TC_WORDS = 2048;
+
+__global__ void __launch_bounds__(64 * TC_WAVES) k_tile_count(int m, int blkm,
+                                                              const int *__restrict__ ia,
+                                                              const int *__restrict__ ja,
+                                                              int *__restrict__ tcount) {
+    __shared__ unsigned s_bits[TC_WAVES][TC_WORDS];
+    unsigned *bits = s_bits[threadIdx.x >> 6];
+    const int I = blockIdx.x * TC_WAVES + (threadIdx.x >> 6);
+    const int l = __lane_id();
+    if (I >= blkm) return;
+    const int p0 = ia[I * 16], p1 = ia[min(m, I * 16 + 16)];
+    int lo = INT_MAX, hi = -1;
+    for (int p = p0 + l; p < p1; p += 64) {
+        const int J = ja[p] >> 4;
+        lo = min(lo, J);
+        hi = max(hi, J);
+    }
+    lo = wave_min(lo);
+    hi = wave_max(hi);
+    int count = 0;
+    for (int w0 = lo; w0 <= hi; w0 += TC_WORDS * 32) {
+        const int w1 = min(hi + 1, w0 + TC_WORDS * 32);
+        const int nwords = (w1 - w0 + 31) >> 5;
+        for (int i = l; i < nwords; i += 64) bits[i] = 0u;
+        wave_lds_fence();
+        for (int p = p0 + l; p < p1; p += 64) {
+            const int J = ja[p] >> 4;
+            if (J >= w0 && J < w1) atomicOr(&bits[(J - w0) >> 5], 1u << ((J - w0) & 31));
+        }
+        wave_lds_fence();
+        for (int i = l; i < nwords; i += 64) count += __popc(bits[i]);
+        wave_lds_fence();
+    }
+    count = wave_sum(count);
+    if (l == 0) tcount[I] = count;
+}
+
 __global__ void k_perm_scatter(int64_t nnz, const int *__restrict__ perm,
                                const double *__restrict__ v, double *__restrict__ out) {
     int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -341,9 +383,8 @@ void convert(TileMat &T, const CsrIn &X, bool fill_values, Arena &ws, hipStream_
     int64_t *tmp = ws.alloc_n<int64_t>(scan_tmp_elems((size_t)T.blkm + 1));
     T.tile_ptr = ws.alloc_n<int>((size_t)T.blkm + 1);
     if (T.blkm > 0) {
-        hipLaunchKernelGGL(k_tile_convert<false>, dim3(grid_for((size_t)T.blkm * 16, 256)),
-                           dim3(256), 0, s, X.rows, T.blkm, X.ptr, X.idx, tcount, nullptr, nullptr,
-                           nullptr, nullptr, nullptr, nullptr, nullptr);
+        hipLaunchKernelGGL(k_tile_count, dim3(grid_for(T.blkm, TC_WAVES)), dim3(64 * TC_WAVES), 0,
+                           s, X.rows, T.blkm, X.ptr, X.idx, tcount);
         TSG_LAUNCH_CHECK();
     }
     exclusive_scan_i32(tcount, T.tile_ptr, (size_t)T.blkm, tmp, s);
@@ -503,9 +544,8 @@ int tile_count_only(const CsrIn &X, Arena &ws, hipStream_t s) {
     int *tcount = ws.alloc_n<int>((size_t)blkm + 1);
     int *tptr = ws.alloc_n<int>((size_t)blkm + 1);
     int64_t *tmp = ws.alloc_n<int64_t>(scan_tmp_elems((size_t)blkm + 1));
-    hipLaunchKernelGGL(k_tile_convert<false>, dim3(grid_for((size_t)blkm * 16, 256)), dim3(256),
-                       0, s, X.rows, blkm, X.ptr, X.idx, tcount, nullptr, nullptr, nullptr,
-                       nullptr, nullptr, nullptr, nullptr);
+    hipLaunchKernelGGL(k_tile_count, dim3(grid_for(blkm, TC_WAVES)), dim3(64 * TC_WAVES), 0, s,
+                       X.rows, blkm, X.ptr, X.idx, tcount);
     TSG_LAUNCH_CHECK();
     exclusive_scan_i32(tcount, tptr, (size_t)blkm, tmp, s);
     int nt = 0;
