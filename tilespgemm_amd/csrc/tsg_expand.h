@@ -84,6 +84,8 @@ __device__ __forceinline__ void for_products(int pa, int pe, int gofs, int gstri
         const int cnt = hi - lo;
         const int incl = group_incl_scan<G>(cnt);
         const int total = __shfl(incl, G - 1, G);
+        // position in B of product q of source lane s: off_s + q
+        const int off = lo - (incl - cnt);
         for (int base = bofs * G; base < total; base += G * U * bstep) {
             int e[U], sl[U];
             bool act[U];
@@ -98,11 +100,8 @@ __device__ __forceinline__ void for_products(int pa, int pe, int gofs, int gstri
                     int probe = __shfl(incl, s + step - 1, G);
                     if (probe <= q) s += step;
                 }
-                const int incl_s = __shfl(incl, s, G);
-                const int cnt_s = __shfl(cnt, s, G);
-                const int lo_s = __shfl(lo, s, G);
                 as[u] = VALUES ? __shfl(a, s, G) : 0.0;
-                e[u] = lo_s + (q - (incl_s - cnt_s));
+                e[u] = __shfl(off, s, G) + q;
                 sl[u] = s;
             }
             int col[U];

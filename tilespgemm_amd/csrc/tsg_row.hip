@@ -43,7 +43,8 @@ constexpr int ITEM_WORDS = (1 << ITEM_WBITS_SHIFT) / 32;
 constexpr int ITEM_TAGS = 256;
 constexpr int ITEM_WAVES = 1;          // waves per block (one item per wave)
 constexpr int ITEM_BLOCKS_PER_CU = 6;  // LDS-bound residency (~25 KB per wave)
-constexpr int RANK_BLOCKS_PER_CU = 7;  // rank-indexed items: 20 KB per wave
+constexpr int RANK_BLOCKS_PER_CU = 9;  // rank-indexed items: 17 KB per wave
+constexpr int RANK_U = 8;
 constexpr int SMALL_U = 2;             // product batches in flight per lane
 constexpr int BIG_SYM_U = 8;
 constexpr int ITEM_U = 4;
@@ -198,10 +199,22 @@ __global__ void __launch_bounds__(BIG_T) k_big_symbolic(int nbig, const int *__r
             for_products<64, BIG_SYM_U, false, false>(
                 pa, pe, 0, 64, ja, nullptr, ib, jb, nullptr, nullptr, 0, 0, 0, 0,
                 [&](bool act, int col, double, double, bool) {
-                    if (act && col >= pass0 && col < pass1) {
-                        int c = col - pass0;
-                        atomicOr(&bits[c >> 5], 1u << (c & 31));
+                    const bool in = act && col >= pass0 && col < pass1;
+                    const int c = col - pass0;
+                    const int w = in ? (c >> 5) : -1 - __lane_id();
+                    unsigned bitv = in ? 1u << (c & 31) : 0u;
+                    // lanes of a run with the same bitmap word (runs are
+                    // contiguous in a sorted B row) combine their bits; only
+                    // the run's last lane issues the atomic
+                    const int ln = __lane_id();
+#pragma unroll
+                    for (int d = 1; d < 64; d <<= 1) {
+                        const int wu = __shfl_up(w, d, 64);
+                        const unsigned bu = __shfl_up(bitv, d, 64);
+                        if (ln >= d && wu == w) bitv |= bu;
                     }
+                    const int wn = __shfl_down(w, 1, 64);
+                    if (in && (ln == 63 || wn != w)) atomicOr(&bits[w], bitv);
                 },
                 wave, NW);
             __syncthreads();
@@ -381,7 +394,7 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric(
 struct RankSmem {
     uint2 words[ITEM_WORDS];      // {bitmap word, outputs before it}
     double vals[ITEM_CAP];
-    unsigned tags[ITEM_CAP];
+    unsigned tags[ITEM_TAGS];     // bucketed by rank
 };
 
 template <bool FUSED>
@@ -395,6 +408,7 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric_rank(
     extern __shared__ __align__(16) unsigned char smem[];
     RankSmem &S = reinterpret_cast<RankSmem *>(smem)[threadIdx.x >> 6];
     const int l = __lane_id();
+    for (int i = l; i < ITEM_TAGS; i += 64) S.tags[i] = 0xffffffffu;
     const int64_t wave_id = (int64_t)blockIdx.x * ITEM_WAVES + (threadIdx.x >> 6);
     const int64_t nwaves = (int64_t)gridDim.x * ITEM_WAVES;
     for (int64_t it = wave_id; it < nitems; it += nwaves) {
@@ -416,12 +430,9 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric_rank(
             if (w < nwords) S.words[w] = make_uint2(word, (unsigned)(cnt + incl - pc));
             cnt += __shfl(incl, 63, 64);
         }
-        for (int i = l; i < cnt; i += 64) {
-            S.vals[i] = 0.0;
-            S.tags[i] = 0xffffffffu;
-        }
+        for (int i = l; i < cnt; i += 64) S.vals[i] = 0.0;
         wave_lds_fence();
-        for_products<64, ITEM_U, true, true>(
+        for_products<64, RANK_U, true, true>(
             ia[row], ia[row + 1], 0, 64, ja, a, ib, jb, b, bptr, bstride, bs, c0, c1,
             [&](bool act, int col, double av, double bv, bool single) {
                 int rank = -1;
@@ -435,19 +446,8 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric_rank(
                     wave_lds_fence();
                     return;
                 }
-                bool pending = rank >= 0;
-                while (__ballot(pending)) {
-                    if (pending) atomicMin(&S.tags[rank], (unsigned)l);
-                    wave_lds_fence();
-                    const bool win = pending && S.tags[rank] == (unsigned)l;
-                    wave_lds_fence();
-                    if (win) {
-                        S.vals[rank] = acc_step<FUSED>(S.vals[rank], av, bv);
-                        S.tags[rank] = 0xffffffffu;
-                        pending = false;
-                    }
-                    wave_lds_fence();
-                }
+                ordered_accumulate<FUSED, ITEM_TAGS>(rank >= 0, rank, av, bv, S.vals, S.tags,
+                                                     (unsigned)l, false);
             });
         wave_lds_fence();
         const int64_t o = item_off[it];
