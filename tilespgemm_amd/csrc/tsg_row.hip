@@ -182,6 +182,7 @@ __global__ void __launch_bounds__(BIG_T) k_big_symbolic(int nbig, const int *__r
     extern __shared__ __align__(16) unsigned char smem[];
     unsigned *bits = reinterpret_cast<unsigned *>(smem);
     __shared__ int s_total[BIG_T / 64];
+    __shared__ int s_seg[BIG_T / 64][128];
     const int wave = threadIdx.x >> 6;
     constexpr int NW = BIG_T / 64;
     for (int r = blockIdx.x; r < nbig; r += gridDim.x) {
@@ -194,29 +195,10 @@ __global__ void __launch_bounds__(BIG_T) k_big_symbolic(int nbig, const int *__r
             for (int i = threadIdx.x; i < words; i += BIG_T) bits[i] = 0u;
             __syncthreads();
             // every wave walks every group of A entries and takes every NW-th
-            // batch of its products: the row's work splits evenly over the
+            // window of its products: the row's work splits evenly over the
             // block however few A entries the row has
-            for_products<64, BIG_SYM_U, false, false>(
-                pa, pe, 0, 64, ja, nullptr, ib, jb, nullptr, nullptr, 0, 0, 0, 0,
-                [&](bool act, int col, double, double, bool) {
-                    const bool in = act && col >= pass0 && col < pass1;
-                    const int c = col - pass0;
-                    const int w = in ? (c >> 5) : -1 - __lane_id();
-                    unsigned bitv = in ? 1u << (c & 31) : 0u;
-                    // lanes of a run with the same bitmap word (runs are
-                    // contiguous in a sorted B row) combine their bits; only
-                    // the run's last lane issues the atomic
-                    const int ln = __lane_id();
-#pragma unroll
-                    for (int d = 1; d < 64; d <<= 1) {
-                        const int wu = __shfl_up(w, d, 64);
-                        const unsigned bu = __shfl_up(bitv, d, 64);
-                        if (ln >= d && wu == w) bitv |= bu;
-                    }
-                    const int wn = __shfl_down(w, 1, 64);
-                    if (in && (ln == 63 || wn != w)) atomicOr(&bits[w], bitv);
-                },
-                wave, NW);
+            mark_products<BIG_SYM_U>(pa, pe, ja, ib, jb, pass0, pass1, bits, wave, NW,
+                                     s_seg[wave]);
             __syncthreads();
             if (rowbits) {
                 unsigned *dst = rowbits + (size_t)r * rowbits_words + (pass0 >> 5);
