@@ -116,6 +116,37 @@ int tsg_spgemm_numeric(tsg_handle_t handle, int m, int n, int k,
                        int64_t nnz_b,
                        const int64_t *d_ic, int *d_jc, double *d_c);
 
+/* ====================================================================
+ * Batched products (no counterpart in the reference, whose driver runs one
+ * product per process).  The batch is laid out as one block-diagonal
+ * product diag(A_i) * diag(B_i) = diag(C_i), so every stage is one launch
+ * over all products.  Device pointers; int32 inputs.
+ * ==================================================================== */
+typedef struct {
+    int m, n, k;                 /* A_i is m x n, B_i is n x k            */
+    const int *ia, *ja;          /* A_i CSR (row pointers start at 0)     */
+    const double *a;
+    int64_t nnz_a;
+    const int *ib, *jb;          /* B_i CSR                               */
+    const double *b;
+    int64_t nnz_b;
+} tsg_product_t;
+
+/* Symbolic phase of a batch.  d_ic (sum of m_i, plus 1, int64) receives the
+ * row pointers of all C_i back to back: product i owns rows
+ * [sum_{j<i} m_j, sum_{j<=i} m_j) and entries [d_ic[first row], d_ic[last
+ * row + 1]).  *nnz_c_total = nnz of all C_i; nnz_c (host, count entries, may
+ * be NULL) receives each nnz(C_i).  Synchronizes the stream. */
+int tsg_batch_symbolic(tsg_handle_t handle, int count, const tsg_product_t *prods,
+                       int64_t *d_ic, int64_t *nnz_c_total, int64_t *nnz_c);
+
+/* Numeric phase of a batch (same products, same structure, values may
+ * differ).  Column indices in d_jc are local to each C_i.  Issues kernels
+ * only (no allocation, no synchronization), so it can be captured in a HIP
+ * graph and replayed. */
+int tsg_batch_numeric(tsg_handle_t handle, int count, const tsg_product_t *prods,
+                      const int64_t *d_ic, int *d_jc, double *d_c);
+
 /* Device CSR transpose (reference utils.h:158-195): row j of the output
  * holds column j of the input, entries by ascending source row.  d_tp has
  * n+1 entries.  Used for C = A * A^T, whose B the reference builds on the

@@ -61,6 +61,8 @@ SIGNATURES = {
     "tsg_spgemm_symbolic": (_i, [_vp, _i, _i, _i, _vp, _vp, _i64, _vp, _vp, _i64, _vp, _vp]),
     "tsg_spgemm_numeric": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _i64,
                                 _vp, _vp, _vp]),
+    "tsg_batch_symbolic": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
+    "tsg_batch_numeric": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
     "tsg_csr_transpose": (_i, [_vp, _i, _i, _vp, _vp, _vp, _i64, _vp, _vp, _vp]),
     "tsg_intermediate_products": (_i, [_vp, _i, _vp, _vp, _i64, _vp, _vp]),
     "tsg_plan_array_bytes": (_sz, [_vp, _i]),
@@ -145,6 +147,19 @@ def _ptr(t):
     return C.c_void_p(t.data_ptr()) if t is not None else C.c_void_p(0)
 
 
+class Product(C.Structure):
+    """tsg_product_t (include/tsg.h): one product C_i = A_i * B_i of a batch."""
+    _fields_ = [("m", C.c_int), ("n", C.c_int), ("k", C.c_int),
+                ("ia", C.c_void_p), ("ja", C.c_void_p), ("a", C.c_void_p), ("nnz_a", C.c_int64),
+                ("ib", C.c_void_p), ("jb", C.c_void_p), ("b", C.c_void_p), ("nnz_b", C.c_int64)]
+
+    @classmethod
+    def of(cls, A: "DeviceCsr", B: "DeviceCsr"):
+        return cls(A.rows, A.cols, B.cols, A.indptr.data_ptr(), A.indices.data_ptr(),
+                   A.data.data_ptr(), A.nnz, B.indptr.data_ptr(), B.indices.data_ptr(),
+                   B.data.data_ptr(), B.nnz)
+
+
 class Handle:
     """A tsg handle on one device.  Work is issued on torch's current stream
     of that device (or, for torch's default stream, on a blocking stream that
@@ -180,6 +195,12 @@ class Handle:
             self.close()
         except Exception:
             pass
+
+    def set_stream(self, stream):
+        """Issue work on a torch stream (None: the handle's own stream)."""
+        ptr = stream.cuda_stream if stream is not None else 0
+        _check(load().tsg_set_stream(self._h, C.c_void_p(ptr)), self._h)
+        self.stream = stream
 
     def set_method(self, method: int):
         _check(load().tsg_set_method(self._h, method), self._h)
@@ -223,6 +244,45 @@ class Handle:
         c = torch.empty(max(nnz, 1), dtype=torch.float64, device=dev)
         self.numeric(A, B, ic, jc, c)
         return ic, jc[:nnz], c[:nnz]
+
+    # ---------------------------------------------------------- batches
+    def batch_symbolic(self, pairs):
+        """pairs: list of (A_i, B_i) DeviceCsr.  Returns (products array, ic
+        (sum m_i + 1, int64, device), nnz per product (host int64 array),
+        row offsets per product)."""
+        torch = self._torch
+        arr = (Product * len(pairs))(*[Product.of(A, B) for A, B in pairs])
+        rows = sum(A.rows for A, _ in pairs)
+        dev = pairs[0][0].indptr.device
+        ic = torch.empty(rows + 1, dtype=torch.int64, device=dev)
+        tot = C.c_int64(0)
+        per = np.zeros(len(pairs), dtype=np.int64)
+        _check(load().tsg_batch_symbolic(self._h, len(pairs), arr, _ptr(ic), C.byref(tot),
+                                         per.ctypes.data_as(C.c_void_p)), self._h)
+        roff = np.concatenate([[0], np.cumsum([A.rows for A, _ in pairs])])
+        return arr, ic, per, roff, int(tot.value)
+
+    def batch_numeric(self, arr, ic, jc, c):
+        _check(load().tsg_batch_numeric(self._h, len(arr), arr, _ptr(ic), _ptr(jc), _ptr(c)),
+               self._h)
+
+    def batch_spgemm(self, pairs):
+        """Batch of C_i = A_i * B_i; returns a list of (ic_i, jc_i, c_i) device
+        tensors (ic_i int64 starting at 0, local columns)."""
+        torch = self._torch
+        arr, ic, per, roff, tot = self.batch_symbolic(pairs)
+        dev = ic.device
+        jc = torch.empty(max(tot, 1), dtype=torch.int32, device=dev)
+        c = torch.empty(max(tot, 1), dtype=torch.float64, device=dev)
+        self.batch_numeric(arr, ic, jc, c)
+        out = []
+        for i in range(len(pairs)):
+            r0, r1 = int(roff[i]), int(roff[i + 1])
+            sub = ic[r0:r1 + 1]
+            e0 = int(sub[0]) if r1 >= r0 else 0
+            e1 = e0 + int(per[i])
+            out.append((sub - e0, jc[e0:e1], c[e0:e1]))
+        return out
 
     def transpose(self, A: DeviceCsr) -> DeviceCsr:
         """A^T as a DeviceCsr (rows by ascending source row)."""

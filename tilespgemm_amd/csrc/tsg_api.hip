@@ -10,20 +10,6 @@
 
 int tile_count_only(const CsrIn &X, Arena &ws, hipStream_t s);   // tsg_tile.hip
 
-struct tsg_handle_s {
-    int device = 0;
-    int cus = 256;
-    hipStream_t stream = nullptr;
-    hipStream_t own_stream = nullptr;
-    int method = TSG_METHOD_AUTO;
-    int fp_mode = TSG_FP_FUSED;
-    int method_used = TSG_METHOD_AUTO;
-    Arena ws;
-    RowPlan row;
-    TilePlan tile;
-    StageTimer tm;
-    std::string err;
-};
 
 namespace {
 
@@ -263,6 +249,24 @@ int tsg_csr_transpose(tsg_handle_t h, int m, int n, const int *d_ia, const int *
         csr_transpose(m, n, d_ia, d_ja, d_a, nnz, d_tp, d_ti, d_a ? d_tv : nullptr, tmp,
                       h->stream);
         TSG_HIP_TRY(hipStreamSynchronize(h->stream));
+    });
+}
+
+int tsg_batch_symbolic(tsg_handle_t h, int count, const tsg_product_t *prods, int64_t *d_ic,
+                       int64_t *nnz_c_total, int64_t *nnz_c) {
+    if (!h) return TSG_ERR_INVALID;
+    return guarded(h, [&] {
+        DeviceGuard dg(h->device);
+        batch_symbolic(h, count, prods, d_ic, nnz_c_total, nnz_c);
+    });
+}
+
+int tsg_batch_numeric(tsg_handle_t h, int count, const tsg_product_t *prods, const int64_t *d_ic,
+                      int *d_jc, double *d_c) {
+    if (!h) return TSG_ERR_INVALID;
+    return guarded(h, [&] {
+        DeviceGuard dg(h->device);
+        batch_numeric(h, count, prods, d_ic, d_jc, d_c);
     });
 }
 

@@ -178,6 +178,44 @@ void tile_numeric(TilePlan &p, const CsrIn &A, const CsrIn &B, const int64_t *d_
                   int *d_jc, double *d_c, int fused, Arena &ws, hipStream_t s,
                   StageTimer &tm, int cus);
 
+// ------------------------------------------------------------ batches
+// A batch laid out as one block-diagonal product (tsg_batch.hip).
+struct BatchState {
+    bool valid = false;
+    int count = 0;
+    bool b_is_a = false;
+    int rows = 0, cols_a = 0, cols_b = 0;     // sum m, sum n, sum k
+    int64_t nnz_a = 0, nnz_b = 0;
+    void *desc = nullptr;                      // device descriptors
+    int64_t *row_off = nullptr;                // [count+1] C/A row offsets
+    int *ia = nullptr, *ja = nullptr, *ib = nullptr, *jb = nullptr;
+    double *a = nullptr, *b = nullptr;
+    std::vector<tsg_product_t> host;           // structure identity check
+};
+
+// ------------------------------------------------------------ handle
+struct tsg_handle_s {
+    int device = 0;
+    int cus = 256;
+    hipStream_t stream = nullptr;
+    hipStream_t own_stream = nullptr;
+    int method = TSG_METHOD_AUTO;
+    int fp_mode = TSG_FP_FUSED;
+    int method_used = TSG_METHOD_AUTO;
+    Arena ws;       // plan of the last symbolic phase
+    Arena bws;      // batch layout (outlives the plan's reset)
+    RowPlan row;
+    TilePlan tile;
+    BatchState batch;
+    StageTimer tm;
+    std::string err;
+};
+
+void batch_symbolic(tsg_handle_t h, int count, const tsg_product_t *prods, int64_t *d_ic,
+                    int64_t *nnz_c_total, int64_t *nnz_c);
+void batch_numeric(tsg_handle_t h, int count, const tsg_product_t *prods, const int64_t *d_ic,
+                   int *d_jc, double *d_c);
+
 // CSR utilities
 // Validates A and B on the device (one sync).  Returns flags of A in bits
 // 0-7 and of B in bits 8-15: 1 bad row pointers, 2 column out of range,

@@ -25,6 +25,8 @@
 #include "tsg_expand.h"
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 
 namespace {
 
@@ -154,11 +156,31 @@ __global__ void __launch_bounds__(256) k_small_numeric(int nrows, const int *__r
                                      ordered_accumulate<FUSED, NT>(act, slot, av, bv, vals, tags,
                                                                    (unsigned)l, single && b_strict);
                                  });
-    // empty slots sort last
-    for (int i = l; i < SLOTS; i += G)
-        if (keys[i] == -1) keys[i] = 0x7fffffff;
+    // compact the occupied slots to the front, in slot order: chunks of G
+    // slots are visited in ascending order and every entry moves to a lower
+    // or equal position, so no entry is overwritten before it is read
+    const int gb = __lane_id() & ~(G - 1);
+    const unsigned long long gm = G == 64 ? ~0ull : ((1ull << G) - 1);
+    int nocc = 0;
+    for (int c0 = 0; c0 < SLOTS; c0 += G) {
+        const int k = keys[c0 + l];
+        const double v = vals[c0 + l];
+        const bool occ = k != -1;
+        const unsigned long long bal = (__ballot(occ) >> gb) & gm;
+        const int before = __popcll(bal & ((1ull << l) - 1));
+        wave_lds_fence();
+        if (occ) {
+            keys[nocc + before] = k;
+            vals[nocc + before] = v;
+        }
+        nocc += __popcll(bal);
+        wave_lds_fence();
+    }
+    int n2 = 1;
+    while (n2 < nocc) n2 <<= 1;
+    for (int i = nocc + l; i < n2; i += G) keys[i] = 0x7fffffff;
     wave_lds_fence();
-    bitonic_sort_group<G>(keys, vals, SLOTS);
+    bitonic_sort_group<G>(keys, vals, n2);
     const int64_t o = ic[row];
     const int nnz = (int)(ic[row + 1] - o);
     for (int i = l; i < nnz; i += G) {
@@ -451,10 +473,18 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric_rank(
     }
 }
 
+// Raises a kernel's dynamic LDS limit once per process (later calls, e.g.
+// inside a captured numeric phase, touch no runtime state).
 template <class K>
 void set_smem(K kernel, size_t bytes) {
-    TSG_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    static std::mutex mu;
+    static std::map<const void *, size_t> done;
+    const void *key = reinterpret_cast<const void *>(kernel);
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = done.find(key);
+    if (it != done.end() && it->second >= bytes) return;
+    TSG_HIP_TRY(hipFuncSetAttribute(key, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    done[key] = bytes;
 }
 
 }  // namespace
