@@ -21,7 +21,7 @@ from __future__ import annotations
 import numpy as np
 
 __all__ = [
-    "laplacian2d", "stencil27", "rmat", "erdos_renyi", "from_coo",
+    "laplacian2d", "stencil27", "rmat", "erdos_renyi", "er_batch", "from_coo",
     "transpose", "intermediate_products", "ref_values",
 ]
 
@@ -119,6 +119,29 @@ def erdos_renyi(n: int, deg: float, seed: int = 0, ncols: int | None = None):
     indptr, indices, _ = from_coo(n, rows, cols, np.zeros(rows.size))
     data = rng.uniform(-1.0, 1.0, indices.size)
     return indptr, indices, data
+
+
+def er_batch(count: int, seed: int = 0, nmin: int = 1000, nmax: int = 65536,
+             dmin: float = 4.0, dmax: float = 32.0):
+    """``count`` independent ER matrices: n log-uniform in [nmin, nmax], mean
+    row degree U[dmin, dmax], values U[-1,1]; one seeded stream for the whole
+    batch.  Returns a list of (n, (indptr, indices, data))."""
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(np.log(nmin), np.log(nmax), count)).astype(np.int64)
+    degs = rng.uniform(dmin, dmax, count)
+    out = []
+    for n, d in zip(ns, degs):
+        n = int(n)
+        per_row = np.minimum(rng.poisson(d, n), n)
+        rows = np.repeat(np.arange(n, dtype=np.int64), per_row)
+        cols = rng.integers(0, n, rows.size)
+        key = np.unique(rows * n + cols)
+        counts = np.bincount(key // n, minlength=n)
+        indptr = np.zeros(n + 1, dtype=np.int64)
+        np.cumsum(counts, out=indptr[1:])
+        out.append((n, (indptr.astype(np.int32), (key % n).astype(np.int32),
+                        rng.uniform(-1.0, 1.0, key.size))))
+    return out
 
 
 def transpose(m: int, n: int, indptr, indices, data):
