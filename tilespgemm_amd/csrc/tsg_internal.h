@@ -112,9 +112,11 @@ struct RowPlan {
     int nbins = 0;
     // big-row structures
     int nbig = 0;
-    int nsub = 0;                    // sub-blocks per big row
     int sub_shift = 0;               // log2 sub-block width (columns)
-    uint16_t *subcnt = nullptr;      // [nbig * nsub]
+    int *span_sb0 = nullptr;         // [nbig] first sub-block of the row's span
+    int64_t *sb_off = nullptr;       // [nbig+1] span offsets (sub-blocks)
+    int64_t nsb_total = 0;
+    uint16_t *subcnt = nullptr;      // [nsb_total] outputs per span sub-block
     // B block-pointer table for big-row items
     int blk_shift = 0;               // log2 block width (columns)
     int nblk = 0;                    // blocks per B row (+1 stored)
@@ -123,10 +125,9 @@ struct RowPlan {
     int64_t nitems = 0;
     int64_t *item_cnt = nullptr;     // [nbig+1] items per big row -> offsets
     int4 *items = nullptr;           // {row, c0, c1, big-row index}
-    // per big row column bitmaps kept from the symbolic phase (rank mode)
+    // per big row span bitmaps kept from the symbolic phase (rank mode)
     bool use_rank = false;
-    unsigned *rowbits = nullptr;     // [nbig * rowbits_words]
-    int64_t rowbits_words = 0;
+    unsigned *rowbits = nullptr;     // [nsb_total * 32] words
     int64_t *item_off = nullptr;     // output offset (within C) per item
 };
 

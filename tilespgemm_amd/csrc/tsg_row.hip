@@ -189,18 +189,56 @@ __global__ void __launch_bounds__(256) k_small_numeric(int nrows, const int *__r
     }
 }
 
+// -------------------------------------------------------- big rows: span
+// Column span of each big row, in 1024-column sub-blocks: from the first
+// sub-block of the first B column to the last sub-block of the last B column
+// over the row's A entries.  One wave per big row.
+__global__ void __launch_bounds__(256) k_big_span(int nbig, const int *__restrict__ rows,
+                                                  const int *__restrict__ ia,
+                                                  const int *__restrict__ ja,
+                                                  const int *__restrict__ ib,
+                                                  const int *__restrict__ jb,
+                                                  int *__restrict__ sb0,
+                                                  int64_t *__restrict__ nsb) {
+    const int r = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int l = threadIdx.x & 63;
+    if (r >= nbig) return;
+    const int row = rows[r];
+    int lo = 0x7fffffff, hi = -1;
+    for (int p = ia[row] + l; p < ia[row + 1]; p += 64) {
+        const int t = ja[p];
+        const int b0 = ib[t], b1 = ib[t + 1];
+        if (b0 < b1) {
+            lo = min(lo, jb[b0] >> SUB_SHIFT);
+            hi = max(hi, jb[b1 - 1] >> SUB_SHIFT);
+        }
+    }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        lo = min(lo, __shfl_xor(lo, d, 64));
+        hi = max(hi, __shfl_xor(hi, d, 64));
+    }
+    if (l == 0) {
+        sb0[r] = hi >= lo ? lo : 0;
+        nsb[r] = hi >= lo ? (int64_t)(hi - lo + 1) : 0;
+    }
+}
+
 // -------------------------------------------------------- big rows: symbolic
-// One block per big row (grid-stride).  LDS bitmap of up to 2^20 columns;
-// per 1024-column sub-block counts go to subcnt.
+// One block per big row (grid-stride).  LDS bitmap over the row's column
+// span, up to 2^20 columns per pass; writes the count of every 1024-column
+// sub-block of the span (subcnt, at the row's span offset) and, when rowbits
+// is given, the span's bitmap.
 __global__ void __launch_bounds__(BIG_T) k_big_symbolic(int nbig, const int *__restrict__ rows,
                                                         int k, const int *__restrict__ ia,
                                                         const int *__restrict__ ja,
                                                         const int *__restrict__ ib,
-                                                        const int *__restrict__ jb, int nsub,
+                                                        const int *__restrict__ jb,
+                                                        const int *__restrict__ span_sb0,
+                                                        const int64_t *__restrict__ sb_off,
                                                         uint16_t *__restrict__ subcnt,
                                                         int64_t *__restrict__ rownnz,
-                                                        unsigned *__restrict__ rowbits,
-                                                        int64_t rowbits_words) {
+                                                        unsigned *__restrict__ rowbits) {
     extern __shared__ __align__(16) unsigned char smem[];
     unsigned *bits = reinterpret_cast<unsigned *>(smem);
     __shared__ int s_total[BIG_T / 64];
@@ -210,9 +248,13 @@ __global__ void __launch_bounds__(BIG_T) k_big_symbolic(int nbig, const int *__r
     for (int r = blockIdx.x; r < nbig; r += gridDim.x) {
         const int row = rows[r];
         const int pa = ia[row], pe = ia[row + 1];
+        const int64_t so = sb_off[r];
+        const int nsb_row = (int)(sb_off[r + 1] - so);
+        const int span0 = span_sb0[r] << SUB_SHIFT;
+        const int span1 = min(k, (span_sb0[r] + nsb_row) << SUB_SHIFT);
         int row_total = 0;
-        for (int pass0 = 0; pass0 < k; pass0 += (1 << BIG_PASS_SHIFT)) {
-            const int pass1 = min(k, pass0 + (1 << BIG_PASS_SHIFT));
+        for (int pass0 = span0; pass0 < span1; pass0 += (1 << BIG_PASS_SHIFT)) {
+            const int pass1 = min(span1, pass0 + (1 << BIG_PASS_SHIFT));
             const int words = (pass1 - pass0 + 31) >> 5;
             for (int i = threadIdx.x; i < words; i += BIG_T) bits[i] = 0u;
             __syncthreads();
@@ -222,8 +264,9 @@ __global__ void __launch_bounds__(BIG_T) k_big_symbolic(int nbig, const int *__r
             mark_products<BIG_SYM_U>(pa, pe, ja, ib, jb, pass0, pass1, bits, wave, NW,
                                      s_seg[wave]);
             __syncthreads();
+            const int64_t sbase = so + ((pass0 - span0) >> SUB_SHIFT);
             if (rowbits) {
-                unsigned *dst = rowbits + (size_t)r * rowbits_words + (pass0 >> 5);
+                unsigned *dst = rowbits + sbase * (1 << (SUB_SHIFT - 5));
                 for (int i = threadIdx.x; i < words; i += BIG_T) dst[i] = bits[i];
             }
             const int nsb = (pass1 - pass0 + (1 << SUB_SHIFT) - 1) >> SUB_SHIFT;
@@ -233,7 +276,7 @@ __global__ void __launch_bounds__(BIG_T) k_big_symbolic(int nbig, const int *__r
                 int w1 = min(words, w0 + (1 << (SUB_SHIFT - 5)));
                 int cnt = 0;
                 for (int w = w0; w < w1; w++) cnt += __popc(bits[w]);
-                subcnt[(size_t)r * nsub + (pass0 >> SUB_SHIFT) + sb] = (uint16_t)cnt;
+                subcnt[sbase + sb] = (uint16_t)cnt;
                 part += cnt;
             }
             part = wave_sum(part);
@@ -271,24 +314,28 @@ __global__ void __launch_bounds__(256) k_bptr(int n, const int *__restrict__ ib,
 // are skipped.  FILL=false counts, FILL=true writes {row, c0, c1, big-row
 // index}.
 template <bool FILL>
-__global__ void k_items(int nbig, const int *__restrict__ rows, int k, int nsub,
+__global__ void k_items(int nbig, const int *__restrict__ rows, int k,
+                        const int *__restrict__ span_sb0, const int64_t *__restrict__ sb_off,
                         const uint16_t *__restrict__ subcnt, const int64_t *__restrict__ ic,
                         int64_t *__restrict__ item_cnt, int4 *__restrict__ items,
                         int64_t *__restrict__ item_off) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nbig) return;
     const int row = rows[r];
-    const uint16_t *sc = subcnt + (size_t)r * nsub;
+    const uint16_t *sc = subcnt + sb_off[r];
+    const int nsb = (int)(sb_off[r + 1] - sb_off[r]);
+    const int first = span_sb0[r];
     constexpr int WMAX_SB = 1 << (ITEM_WBITS_SHIFT - SUB_SHIFT);
     int64_t nitem = 0;
     int64_t out = FILL ? ic[row] : 0;
     int64_t w = FILL ? item_cnt[r] : 0;
     int start = -1, cnt = 0;
-    for (int sb = 0; sb < nsub; sb++) {
-        const int cs = sc[sb];
-        if (start >= 0 && (cnt + cs > ITEM_CAP || sb - start >= WMAX_SB)) {
+    for (int j = 0; j < nsb; j++) {
+        const int cs = sc[j];
+        if (start >= 0 && (cnt + cs > ITEM_CAP || j - start >= WMAX_SB)) {
             if (FILL) {
-                items[w] = make_int4(row, start << SUB_SHIFT, min(k, sb << SUB_SHIFT), r);
+                items[w] = make_int4(row, (first + start) << SUB_SHIFT,
+                                     min(k, (first + j) << SUB_SHIFT), r);
                 item_off[w] = out;
                 w++;
                 out += cnt;
@@ -298,12 +345,13 @@ __global__ void k_items(int nbig, const int *__restrict__ rows, int k, int nsub,
             cnt = 0;
         }
         if (cs == 0 && start < 0) continue;
-        if (start < 0) start = sb;
+        if (start < 0) start = j;
         cnt += cs;
     }
     if (start >= 0 && cnt > 0) {
         if (FILL) {
-            items[w] = make_int4(row, start << SUB_SHIFT, k, r);
+            items[w] = make_int4(row, (first + start) << SUB_SHIFT,
+                                 min(k, (first + nsb) << SUB_SHIFT), r);
             item_off[w] = out;
         }
         nitem++;
@@ -404,8 +452,8 @@ struct RankSmem {
 template <bool FUSED>
 __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric_rank(
     int64_t nitems, const int4 *__restrict__ items, const int64_t *__restrict__ item_off,
-    const unsigned *__restrict__ rowbits, int64_t rowbits_words,
-    const int *__restrict__ ia, const int *__restrict__ ja, const double *__restrict__ a,
+    const unsigned *__restrict__ rowbits, const int *__restrict__ span_sb0,
+    const int64_t *__restrict__ sb_off, const int *__restrict__ ia, const int *__restrict__ ja, const double *__restrict__ a,
     const int *__restrict__ ib, const int *__restrict__ jb, const double *__restrict__ b,
     const int *__restrict__ bptr, int bstride, int bs, int *__restrict__ jc,
     double *__restrict__ c, int b_strict) {
@@ -419,7 +467,8 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric_rank(
         const int4 item = items[it];
         const int row = item.x, c0 = item.y, c1 = item.z, big = item.w;
         const int nwords = (c1 - c0 + 31) >> 5;
-        const unsigned *src = rowbits + (size_t)big * rowbits_words + (c0 >> 5);
+        const unsigned *src = rowbits + sb_off[big] * (1 << (SUB_SHIFT - 5)) +
+                              ((c0 >> 5) - (span_sb0[big] << (SUB_SHIFT - 5)));
         int cnt = 0;
         for (int w0 = 0; w0 < nwords; w0 += 64) {
             const int w = w0 + l;
@@ -556,27 +605,39 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
     // big rows
     p.nbig = st.s[5] - st.s[4];
     p.sub_shift = SUB_SHIFT;
-    p.nsub = (k + (1 << SUB_SHIFT) - 1) >> SUB_SHIFT;
     if (p.nbig > 0) {
         tm.begin("row.big_symbolic");
-        p.subcnt = ws.alloc_n<uint16_t>((size_t)p.nbig * p.nsub);
         const int *rowsb = p.bin_rows + st.s[4];
-        // keep every big row's column bitmap for the rank-indexed numeric
-        // phase when it fits a budget of device memory; else the numeric
-        // phase uses per-item hash tables
-        p.rowbits_words = ((int64_t)k + 31) / 32;
-        const size_t bits_bytes = (size_t)p.nbig * (size_t)p.rowbits_words * 4;
+        // column span of every big row, in sub-blocks, and span offsets
+        p.span_sb0 = ws.alloc_n<int>((size_t)p.nbig + 1);
+        p.sb_off = ws.alloc_n<int64_t>((size_t)p.nbig + 1);
+        int64_t *tmp_sb = ws.alloc_n<int64_t>(scan_tmp_elems((size_t)p.nbig + 1));
+        hipLaunchKernelGGL(k_big_span, dim3(grid_for((size_t)p.nbig * 64, 256)), dim3(256), 0, s,
+                           p.nbig, rowsb, A.ptr, A.idx, B.ptr, B.idx, p.span_sb0, p.sb_off);
+        TSG_LAUNCH_CHECK();
+        exclusive_scan_i64(p.sb_off, p.sb_off, (size_t)p.nbig, tmp_sb, s);
+        int64_t nsb_total = 0;
+        TSG_HIP_TRY(hipMemcpyAsync(&nsb_total, p.sb_off + p.nbig, sizeof(int64_t),
+                                   hipMemcpyDeviceToHost, s));
+        TSG_HIP_TRY(hipStreamSynchronize(s));
+        p.nsb_total = nsb_total;
+        p.subcnt = ws.alloc_n<uint16_t>((size_t)nsb_total + 1);
+        // keep every big row's span bitmap for the rank-indexed numeric phase
+        // when it fits a budget of device memory; else the numeric phase uses
+        // per-item hash tables
+        const size_t bits_bytes = (size_t)nsb_total * ((1u << SUB_SHIFT) / 8);
         size_t free_b = 0, total_b = 0;
         if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) { (void)hipGetLastError(); free_b = 0; }
         const size_t budget = std::min<size_t>(size_t(48) << 30, free_b / 10 * 4);
         p.use_rank = bits_bytes <= budget;
-        p.rowbits = p.use_rank ? ws.alloc_n<unsigned>(bits_bytes / 4) : nullptr;
-        const size_t smem = sizeof(unsigned) * (size_t)((std::min(k, 1 << BIG_PASS_SHIFT) + 31) / 32);
+        p.rowbits = p.use_rank ? ws.alloc_n<unsigned>(bits_bytes / 4 + 1) : nullptr;
+        const int span_max = std::min(k, 1 << BIG_PASS_SHIFT);
+        const size_t smem = sizeof(unsigned) * (size_t)((span_max + 31) / 32);
         set_smem(k_big_symbolic, smem);
         const unsigned grid = (unsigned)std::min(p.nbig, cus * 1);
         hipLaunchKernelGGL(k_big_symbolic, dim3(grid), dim3(BIG_T), smem, s, p.nbig, rowsb, k,
-                           A.ptr, A.idx, B.ptr, B.idx, p.nsub, p.subcnt, p.rownnz, p.rowbits,
-                           p.rowbits_words);
+                           A.ptr, A.idx, B.ptr, B.idx, p.span_sb0, p.sb_off, p.subcnt, p.rownnz,
+                           p.rowbits);
         TSG_LAUNCH_CHECK();
         tm.end();
 
@@ -611,7 +672,8 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
         p.item_cnt = ws.alloc_n<int64_t>((size_t)p.nbig + 1);
         int64_t *tmp2 = ws.alloc_n<int64_t>(scan_tmp_elems((size_t)p.nbig + 1));
         hipLaunchKernelGGL(k_items<false>, dim3(grid_for(p.nbig, 128)), dim3(128), 0, s, p.nbig,
-                           rowsb, k, p.nsub, p.subcnt, d_ic, p.item_cnt, nullptr, nullptr);
+                           rowsb, k, p.span_sb0, p.sb_off, p.subcnt, d_ic, p.item_cnt, nullptr,
+                           nullptr);
         TSG_LAUNCH_CHECK();
         exclusive_scan_i64(p.item_cnt, p.item_cnt, (size_t)p.nbig, tmp2, s);
         TSG_HIP_TRY(hipMemcpyAsync(&h_tot[1], p.item_cnt + p.nbig, sizeof(int64_t),
@@ -622,7 +684,8 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
         p.items = ws.alloc_n<int4>((size_t)p.nitems + 1);
         p.item_off = ws.alloc_n<int64_t>((size_t)p.nitems + 1);
         hipLaunchKernelGGL(k_items<true>, dim3(grid_for(p.nbig, 128)), dim3(128), 0, s, p.nbig,
-                           rowsb, k, p.nsub, p.subcnt, d_ic, p.item_cnt, p.items, p.item_off);
+                           rowsb, k, p.span_sb0, p.sb_off, p.subcnt, d_ic, p.item_cnt, p.items,
+                           p.item_off);
         TSG_LAUNCH_CHECK();
         tm.end();
     } else {
@@ -680,7 +743,7 @@ static void row_numeric_t(RowPlan &p, const CsrIn &A, const CsrIn &B, const int6
         const unsigned grid = (unsigned)std::min<int64_t>(
             (p.nitems + ITEM_WAVES - 1) / ITEM_WAVES, (int64_t)cus * RANK_BLOCKS_PER_CU);
         hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * ITEM_WAVES), sm, s, p.nitems, p.items,
-                           p.item_off, p.rowbits, p.rowbits_words, A.ptr, A.idx, A.val, B.ptr,
+                           p.item_off, p.rowbits, p.span_sb0, p.sb_off, A.ptr, A.idx, A.val, B.ptr,
                            B.idx, B.val, p.bptr, p.nblk + 1, p.blk_shift, d_jc, d_c,
                            (int)p.b_strict);
         TSG_LAUNCH_CHECK();
