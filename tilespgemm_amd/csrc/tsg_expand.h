@@ -296,6 +296,78 @@ __device__ __forceinline__ void bitonic_sort_group(int *keys, double *vals, int 
     }
 }
 
+// Stable LSD radix sort of n keys (with values) by the G lanes of a group:
+// 8-bit digits of (key - kmin) over `bits` bits; src/dst ping-pong buffers of
+// n entries each; hist is a 256-entry LDS array owned by the group.  Each
+// chunk of G entries is split by digit with ballots (lanes with equal digits
+// keep their order), so the sort is stable.  Returns true when the sorted
+// data ends in dst.
+template <int G>
+__device__ __forceinline__ bool radix_sort_group(int *skeys, double *svals, int *dkeys,
+                                                 double *dvals, int n, int kmin, int bits,
+                                                 unsigned *hist) {
+    const int l = __lane_id() & (G - 1);
+    const int gb = __lane_id() & ~(G - 1);
+    const unsigned long long gm = G == 64 ? ~0ull : ((1ull << G) - 1);
+    const unsigned long long below = (1ull << l) - 1;
+    bool in_dst = false;
+    for (int shift = 0; shift < bits; shift += 8) {
+        int *ik = in_dst ? dkeys : skeys;
+        double *iv = in_dst ? dvals : svals;
+        int *ok = in_dst ? skeys : dkeys;
+        double *ov = in_dst ? svals : dvals;
+        for (int i = l; i < 256; i += G) hist[i] = 0u;
+        wave_lds_fence();
+        // histogram: one add per digit group of a chunk
+        for (int c0 = 0; c0 < n; c0 += G) {
+            const bool act = c0 + l < n;
+            const unsigned d = act ? ((unsigned)(ik[c0 + l] - kmin) >> shift) & 255u : 0u;
+            unsigned long long m = (__ballot(act) >> gb) & gm;
+#pragma unroll
+            for (int b = 0; b < 8; b++) {
+                const unsigned long long bb = (__ballot((d >> b) & 1u) >> gb) & gm;
+                m &= ((d >> b) & 1u) ? bb : ~bb;
+            }
+            if (act && (m & below) == 0) hist[d] += (unsigned)__popcll(m);
+            wave_lds_fence();
+        }
+        // exclusive scan of the 256 counts
+        if (l == 0) {
+            unsigned run = 0;
+            for (int i = 0; i < 256; i++) {
+                const unsigned c = hist[i];
+                hist[i] = run;
+                run += c;
+            }
+        }
+        wave_lds_fence();
+        // stable scatter
+        for (int c0 = 0; c0 < n; c0 += G) {
+            const bool act = c0 + l < n;
+            const int key = act ? ik[c0 + l] : 0;
+            const double val = act ? iv[c0 + l] : 0.0;
+            const unsigned d = act ? ((unsigned)(key - kmin) >> shift) & 255u : 0u;
+            unsigned long long m = (__ballot(act) >> gb) & gm;
+#pragma unroll
+            for (int b = 0; b < 8; b++) {
+                const unsigned long long bb = (__ballot((d >> b) & 1u) >> gb) & gm;
+                m &= ((d >> b) & 1u) ? bb : ~bb;
+            }
+            const int rank = __popcll(m & below);
+            const unsigned base = act ? hist[d] : 0u;
+            wave_lds_fence();
+            if (act) {
+                ok[base + rank] = key;
+                ov[base + rank] = val;
+                if ((m >> l) == 1ull) hist[d] = base + (unsigned)__popcll(m);   // last of its digit
+            }
+            wave_lds_fence();
+        }
+        in_dst = !in_dst;
+    }
+    return in_dst;
+}
+
 __device__ __forceinline__ void bitonic_sort_keys_group64(int *keys, int n) {
     const int l = __lane_id();
     for (int kk = 2; kk <= n; kk <<= 1) {

@@ -133,11 +133,13 @@ __global__ void __launch_bounds__(256) k_small_numeric(int nrows, const int *__r
     double *s_vals = reinterpret_cast<double *>(smem);
     int *s_keys = reinterpret_cast<int *>(s_vals + RPB * SLOTS);
     unsigned *s_tags = reinterpret_cast<unsigned *>(s_keys + RPB * SLOTS);
+    unsigned *s_hist = s_tags + RPB * NT;
     const int grp = threadIdx.x / G;
     const int l = threadIdx.x & (G - 1);
     int *keys = s_keys + grp * SLOTS;
     double *vals = s_vals + grp * SLOTS;
     unsigned *tags = s_tags + grp * NT;
+    unsigned *hist = s_hist + grp * 256;
     const int r = blockIdx.x * RPB + grp;
     for (int i = l; i < SLOTS; i += G) { keys[i] = -1; vals[i] = 0.0; }
     for (int i = l; i < NT; i += G) tags[i] = 0xffffffffu;
@@ -176,16 +178,41 @@ __global__ void __launch_bounds__(256) k_small_numeric(int nrows, const int *__r
         nocc += __popcll(bal);
         wave_lds_fence();
     }
-    int n2 = 1;
-    while (n2 < nocc) n2 <<= 1;
-    for (int i = nocc + l; i < n2; i += G) keys[i] = 0x7fffffff;
-    wave_lds_fence();
-    bitonic_sort_group<G>(keys, vals, n2);
     const int64_t o = ic[row];
     const int nnz = (int)(ic[row + 1] - o);
+    if (nocc <= 64) {
+        int n2 = 1;
+        while (n2 < nocc) n2 <<= 1;
+        for (int i = nocc + l; i < n2; i += G) keys[i] = 0x7fffffff;
+        wave_lds_fence();
+        bitonic_sort_group<G>(keys, vals, n2);
+        for (int i = l; i < nnz; i += G) {
+            jc[o + i] = keys[i];
+            c[o + i] = vals[i];
+        }
+        return;
+    }
+    // radix sort on keys relative to the row's smallest column; the back
+    // half of the table (occupancy <= SLOTS/2) is the ping-pong buffer
+    int kmin = 0x7fffffff, kmax = 0;
+    for (int i = l; i < nocc; i += G) {
+        kmin = min(kmin, keys[i]);
+        kmax = max(kmax, keys[i]);
+    }
+#pragma unroll
+    for (int d = G / 2; d > 0; d >>= 1) {
+        kmin = min(kmin, __shfl_xor(kmin, d, G));
+        kmax = max(kmax, __shfl_xor(kmax, d, G));
+    }
+    const int span = kmax - kmin;
+    const int bits = span > 0 ? 32 - __clz(span) : 1;
+    const bool in_dst = radix_sort_group<G>(keys, vals, keys + SLOTS / 2, vals + SLOTS / 2, nocc,
+                                            kmin, bits, hist);
+    const int *sk = in_dst ? keys + SLOTS / 2 : keys;
+    const double *sv = in_dst ? vals + SLOTS / 2 : vals;
     for (int i = l; i < nnz; i += G) {
-        jc[o + i] = keys[i];
-        c[o + i] = vals[i];
+        jc[o + i] = sk[i];
+        c[o + i] = sv[i];
     }
 }
 
@@ -705,7 +732,7 @@ static void row_numeric_t(RowPlan &p, const CsrIn &A, const CsrIn &B, const int6
     {
         int n1 = st[2] - st[1];
         if (n1 > 0) {
-            constexpr size_t sm = 16 * 128 * 12 + 16 * 64 * 4;
+            constexpr size_t sm = 16 * 128 * 12 + 16 * 64 * 4 + 16 * 256 * 4;
             auto kern = k_small_numeric<16, 128, FUSED>;
             set_smem(kern, sm);
             hipLaunchKernelGGL(kern, dim3(grid_for(n1, 16)), dim3(256), sm, s, n1, bins + st[1],
@@ -715,7 +742,7 @@ static void row_numeric_t(RowPlan &p, const CsrIn &A, const CsrIn &B, const int6
         }
         int n2 = st[3] - st[2];
         if (n2 > 0) {
-            constexpr size_t sm = 8 * 512 * 12 + 8 * 128 * 4;
+            constexpr size_t sm = 8 * 512 * 12 + 8 * 128 * 4 + 8 * 256 * 4;
             auto kern = k_small_numeric<32, 512, FUSED>;
             set_smem(kern, sm);
             hipLaunchKernelGGL(kern, dim3(grid_for(n2, 8)), dim3(256), sm, s, n2, bins + st[2],
@@ -725,7 +752,7 @@ static void row_numeric_t(RowPlan &p, const CsrIn &A, const CsrIn &B, const int6
         }
         int n3 = st[4] - st[3];
         if (n3 > 0) {
-            constexpr size_t sm = 4 * 2048 * 12 + 4 * 256 * 4;
+            constexpr size_t sm = 4 * 2048 * 12 + 4 * 256 * 4 + 4 * 256 * 4;
             auto kern = k_small_numeric<64, 2048, FUSED>;
             set_smem(kern, sm);
             hipLaunchKernelGGL(kern, dim3(grid_for(n3, 4)), dim3(256), sm, s, n3, bins + st[3],
