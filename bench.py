@@ -194,6 +194,164 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def served_mix(count, seed, edge_factor):
+    """The served request mix: 70% ER (n uniform in [1000, 16000], mean degree
+    8) C=A*A, 25% 2D 5-point Laplacian (grid side log-uniform in [64, 512])
+    C=A*A, 5% R-MAT scale 18 (edge_factor * 2^18 draws) C=A*A^T.  Operands
+    come from a pool of distinct matrices (64 ER, 16 Laplacian, 2 R-MAT)
+    drawn per request.  Returns (kinds, index into the pool) per request and
+    the pools."""
+    from tilespgemm_amd import gen
+    rng = np.random.default_rng(seed)
+    er = [gen.erdos_renyi(int(n), 8.0, seed=1000 + i)
+          for i, n in enumerate(rng.integers(1000, 16001, 64))]
+    sides = sorted(set(int(x) for x in np.exp(rng.uniform(np.log(64), np.log(512), 16))))
+    lap = [(s, gen.laplacian2d(s)) for s in sides]
+    rm = [gen.rmat(18, edge_factor << 18, seed=s) for s in (1, 2)]
+    u = rng.random(count)
+    kinds = np.where(u < 0.70, 0, np.where(u < 0.95, 1, 2))
+    pick = np.array([rng.integers(0, (len(er), len(lap), len(rm))[k]) for k in kinds])
+    return kinds, pick, er, lap, rm
+
+
+def bench_served(args):
+    """Served form: an open-loop Poisson request stream against the request
+    server (device-resident operands, admission by the intermediate-product
+    bound, continuous batching on several workers) and against the same
+    server with one worker and batches of one (plain per-call launches).
+    Latency of a request = completion - scheduled arrival."""
+    import threading
+    import queue as pyqueue
+    import torch
+    import tilespgemm_amd as tsg
+    from tilespgemm_amd import gen
+    t0 = time.time()
+    nreq = args.requests
+    kinds, pick, er, lap, rm = served_mix(nreq, seed=11, edge_factor=args.edge_factor)
+    dev = "cuda:0"
+    h = tsg.Handle(0)
+    d_er = [tsg.DeviceCsr.from_host(X[0].size - 1, X[0].size - 1, *X, device=dev) for X in er]
+    d_lap = [tsg.DeviceCsr.from_host(s * s, s * s, *X, device=dev) for s, X in lap]
+    d_rm = []
+    for X in rm:
+        dA = tsg.DeviceCsr.from_host(1 << 18, 1 << 18, *X, device=dev)
+        d_rm.append((dA, h.transpose(dA)))
+    torch.cuda.synchronize()
+    ops = []
+    products = 0
+    for kd, p in zip(kinds, pick):
+        if kd == 0:
+            ops.append((d_er[p], d_er[p]))
+        elif kd == 1:
+            ops.append((d_lap[p], d_lap[p]))
+        else:
+            ops.append(d_rm[p])
+    prod_cache = {}
+    for A, B in ops:
+        key = (id(A), id(B))
+        if key not in prod_cache:
+            prod_cache[key] = h.intermediate_products(A, B)
+        products += prod_cache[key]
+    log(f"[bench] served mix: {nreq} requests (ER {int((kinds == 0).sum())}, Laplacian "
+        f"{int((kinds == 1).sum())}, R-MAT {int((kinds == 2).sum())}), "
+        f"{products} intermediate products, setup {time.time() - t0:.1f}s")
+
+    def run(workers, max_batch, rate):
+        """rate None: every request submitted at t=0 (saturation)."""
+        srv = tsg.Server(0, workers=workers, max_batch=max_batch,
+                         mem_budget=args.mem_budget << 30)
+        rng = np.random.default_rng(123)
+        if rate is None:
+            sched = np.zeros(nreq)
+        else:
+            sched = np.cumsum(rng.exponential(1.0 / rate, nreq))
+        lat = np.zeros(nreq)
+        bsz = np.zeros(nreq, dtype=np.int64)
+        q = pyqueue.Queue()
+        sub_lag = np.zeros(nreq)
+        errors = []
+
+        def waiter():
+            for _ in range(nreq):
+                i, t = q.get()
+                try:
+                    res = srv.wait(t)
+                    lat[i] = sub_lag[i] + res.total_ms * 1e-3
+                    bsz[i] = res.batch_size
+                    srv.release(t)
+                except Exception as e:  # noqa: BLE001 - reported below
+                    errors.append(e)
+                    return
+        th = threading.Thread(target=waiter)
+        th.start()
+        start = time.perf_counter()
+        for i, (A, B) in enumerate(ops):
+            target = start + sched[i]
+            while True:
+                d = target - time.perf_counter()
+                if d <= 0:
+                    break
+                time.sleep(d if d > 2e-3 else 0)
+            ts = time.perf_counter()
+            sub_lag[i] = ts - target
+            q.put((i, srv.submit(A, B)))
+        th.join()
+        span = time.perf_counter() - start
+        srv.close()
+        if errors:
+            raise errors[0]
+        return dict(req_s=nreq / span, p50_ms=float(np.percentile(lat, 50) * 1e3),
+                    p99_ms=float(np.percentile(lat, 99) * 1e3),
+                    mean_batch=float(bsz.mean()), span_s=span)
+
+    served = (args.workers, args.max_batch)
+    percall = (1, 1)
+    run(*served, None)  # warm the device pool and kernels
+    run(*percall, None)
+    sat_s = run(*served, None)
+    sat_p = run(*percall, None)
+    log(f"[bench] saturation: served {sat_s['req_s']:.0f} req/s (mean batch "
+        f"{sat_s['mean_batch']:.1f}), per-call {sat_p['req_s']:.0f} req/s")
+    sweep = []
+    for frac in (0.25, 0.5, 0.8):
+        rate = frac * sat_p["req_s"]
+        rs = run(*served, rate)
+        rp = run(*percall, rate)
+        sweep.append({"rate": round(rate, 1), "served_p50_ms": round(rs["p50_ms"], 3),
+                      "served_p99_ms": round(rs["p99_ms"], 3),
+                      "per_call_p50_ms": round(rp["p50_ms"], 3),
+                      "per_call_p99_ms": round(rp["p99_ms"], 3),
+                      "served_mean_batch": round(rs["mean_batch"], 2)})
+        log(f"[bench] rate {rate:.0f}/s: served p50 {rs['p50_ms']:.2f} p99 {rs['p99_ms']:.2f} ms"
+            f" | per-call p50 {rp['p50_ms']:.2f} p99 {rp['p99_ms']:.2f} ms")
+    for frac in (0.5, 0.8):
+        rate = frac * sat_s["req_s"]
+        rs = run(*served, rate)
+        sweep.append({"rate": round(rate, 1), "served_p50_ms": round(rs["p50_ms"], 3),
+                      "served_p99_ms": round(rs["p99_ms"], 3),
+                      "served_mean_batch": round(rs["mean_batch"], 2)})
+        log(f"[bench] rate {rate:.0f}/s: served p50 {rs['p50_ms']:.2f} p99 {rs['p99_ms']:.2f} ms")
+    out = {
+        "metric": "served req/s", "value": round(sat_s["req_s"], 1), "unit": "req/s",
+        "n_gpus": 1, "steps": nreq, "warmup": nreq,
+        "ms_per_step": round(1e3 / sat_s["req_s"], 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"served mix of {nreq} requests: 70% ER C=A*A (n U[1000,16000], "
+                               "deg 8), 25% 2D Laplacian C=A*A (side 64-512), 5% R-MAT 2^18 "
+                               f"C=A*A^T ({args.edge_factor}*2^18 draws); Poisson arrivals",
+                   "workers": args.workers, "max_batch": args.max_batch,
+                   "mem_budget_gib": args.mem_budget,
+                   "intermediate_products": products,
+                   "served_gflops": round(2.0 * products * sat_s["req_s"] / nreq / 1e9, 2),
+                   "per_call_req_s": round(sat_p["req_s"], 1),
+                   "speedup_vs_per_call": round(sat_s["req_s"] / sat_p["req_s"], 2),
+                   "saturation_served_p99_ms": round(sat_s["p99_ms"], 2),
+                   "saturation_per_call_p99_ms": round(sat_p["p99_ms"], 2),
+                   "sweep": sweep},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -203,6 +361,11 @@ def main():
     ap.add_argument("--method", default="auto", choices=["auto", "tile", "row"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch", type=int, default=512, help="products in the er_batch workload")
+    ap.add_argument("--requests", type=int, default=2000, help="requests in the served workload")
+    ap.add_argument("--edge-factor", type=int, default=4, help="R-MAT draws / n in the served mix")
+    ap.add_argument("--workers", type=int, default=3, help="server workers (served workload)")
+    ap.add_argument("--max-batch", type=int, default=64, help="server batch cap (served)")
+    ap.add_argument("--mem-budget", type=int, default=32, help="server budget, GiB (served)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -211,6 +374,8 @@ def main():
 
     if args.workload == "er_batch":
         return bench_batch(args)
+    if args.workload == "served":
+        return bench_served(args)
 
     import torch
     import tilespgemm_amd as tsg

@@ -147,6 +147,55 @@ int tsg_batch_symbolic(tsg_handle_t handle, int count, const tsg_product_t *prod
 int tsg_batch_numeric(tsg_handle_t handle, int count, const tsg_product_t *prods,
                       const int64_t *d_ic, int *d_jc, double *d_c);
 
+/* ====================================================================
+ * Serving (no counterpart in the reference).  A request loop with
+ * device-resident operands and continuous batching: `workers` threads, each
+ * with its own handle and stream, take in FIFO order every waiting request
+ * that fits the memory budget into one batched product, so requests that
+ * arrive while others run are admitted as memory frees up and their work
+ * overlaps on the device.  Admission charges a request
+ * 16*(m+2) + 12*min(products, m*k) bytes (products = intermediate products,
+ * computed on the device by an estimator thread after submission) from
+ * admission until release; when no batch is running the head request is
+ * admitted even if it exceeds the budget, so progress never depends on the
+ * caller releasing results first.  max_batch = 1 and workers = 1 give plain
+ * per-call processing in arrival order.
+ * Outputs are device buffers owned by the server until released; calls on
+ * one server are thread-safe.
+ * ==================================================================== */
+typedef struct tsg_server_s *tsg_server_t;
+
+typedef struct {
+    int workers;                 /* worker threads (handles, streams)       */
+    int max_batch;               /* requests per batch                      */
+    size_t mem_budget;           /* bytes charged to admitted, unreleased
+                                    requests; 0 = 64 GiB                    */
+    int method;                  /* TSG_METHOD_*                            */
+} tsg_server_config_t;
+
+typedef struct {
+    int status;                  /* TSG_OK or TSG_ERR_*                     */
+    int m;                       /* rows of C                               */
+    int64_t nnz;                 /* nnz(C)                                  */
+    const int64_t *ic;           /* device, m+1 entries, ic[0] = 0          */
+    const int *jc;               /* device, nnz entries                     */
+    const double *c;             /* device, nnz entries                     */
+    double queue_ms;             /* submission -> batch start               */
+    double total_ms;             /* submission -> result ready              */
+    int batch_size;              /* requests in the batch that served it    */
+} tsg_result_t;
+
+int tsg_server_create(tsg_server_t *server, int device, const tsg_server_config_t *config);
+/* Enqueues C = A * B for the device operands of *req (kept alive by the
+ * caller until the result is released).  Returns a ticket. */
+int tsg_server_submit(tsg_server_t server, const tsg_product_t *req, uint64_t *ticket);
+/* Blocks until the ticket's result is ready. */
+int tsg_server_wait(tsg_server_t server, uint64_t ticket, tsg_result_t *result);
+/* Returns the result's device buffers to the server. */
+int tsg_server_release(tsg_server_t server, uint64_t ticket);
+/* Drains the queue, stops the workers, frees everything. */
+int tsg_server_destroy(tsg_server_t server);
+
 /* Device CSR transpose (reference utils.h:158-195): row j of the output
  * holds column j of the input, entries by ascending source row.  d_tp has
  * n+1 entries.  Used for C = A * A^T, whose B the reference builds on the

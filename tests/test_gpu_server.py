@@ -1,0 +1,99 @@
+"""GPU tests of the continuous-batching server: every request's result is
+bitwise the same served in a mix as computed alone, for batched (several
+workers) and per-call (one worker, batch 1) configurations, including a
+request larger than the memory budget."""
+from __future__ import annotations
+
+import threading
+
+import numpy as np
+import pytest
+
+import tilespgemm_amd as tsg
+from oracle import py_oracle as O
+from tilespgemm_amd import gen
+from tests.util import assert_csr_equal
+
+pytestmark = pytest.mark.gpu
+
+
+def request_mix(seed=0, count=40):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        u = rng.random()
+        if u < 0.7:
+            n = int(rng.integers(1000, 16000))
+            A = gen.erdos_renyi(n, 8.0, seed=seed * 1000 + i)
+            out.append((n, n, n, A, A, True))
+        elif u < 0.95:
+            g = int(rng.integers(8, 128))
+            A = gen.laplacian2d(g)
+            out.append((g * g, g * g, g * g, A, A, True))
+        else:
+            A = gen.rmat(12, 4 << 12, seed=seed * 1000 + i)
+            AT = gen.transpose(4096, 4096, *A)
+            out.append((4096, 4096, 4096, A, AT, False))
+    return out
+
+
+def devs(mix):
+    out = []
+    for m, n, k, A, B, same in mix:
+        dA = tsg.DeviceCsr.from_host(m, n, *A)
+        out.append((dA, dA if same else tsg.DeviceCsr.from_host(n, k, *B)))
+    return out
+
+
+@pytest.mark.parametrize("workers,max_batch", [(2, 64), (1, 1), (3, 8)])
+def test_served_results_equal_alone(workers, max_batch):
+    import torch
+    assert torch.cuda.is_available()
+    mix = request_mix(seed=workers * 10 + max_batch)
+    ops = devs(mix)
+    srv = tsg.Server(0, workers=workers, max_batch=max_batch, mem_budget=1 << 30)
+    try:
+        tickets = [srv.submit(A, B) for A, B in ops]
+        got = [srv.fetch(t) for t in tickets]
+    finally:
+        srv.close()
+    h = tsg.Handle(0)
+    sizes = []
+    for (m, n, k, A, B, same), (dA, dB), (ic, jc, c, res) in zip(mix, ops, got):
+        ic1, jc1, c1 = h.spgemm(dA, dB)
+        alone = (ic1.cpu().numpy(), jc1.cpu().numpy(), c1.cpu().numpy())
+        assert_csr_equal((ic, jc, c), alone)
+        sizes.append(res.batch_size)
+    # the last request against the oracle too (structure and bits)
+    m, n, k, A, B, same = mix[-1]
+    assert_csr_equal(got[-1][:3], O.csr_spgemm(m, k, A, B))
+    if max_batch == 1:
+        assert max(sizes) == 1
+    h.close()
+
+
+def test_concurrent_submitters_and_budget():
+    """Two client threads submit while workers run; a budget smaller than one
+    request still serves it (alone)."""
+    import torch
+    assert torch.cuda.is_available()
+    mix = request_mix(seed=5, count=24)
+    ops = devs(mix)
+    srv = tsg.Server(0, workers=2, max_batch=16, mem_budget=4 << 20)
+    tickets = [None] * len(ops)
+
+    def client(idx):
+        for i in idx:
+            tickets[i] = srv.submit(*ops[i])
+
+    th = [threading.Thread(target=client, args=(range(j, len(ops), 2),)) for j in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    try:
+        got = [srv.fetch(t) for t in tickets]
+    finally:
+        srv.close()
+    for (m, n, k, A, B, same), g in zip(mix, got):
+        assert_csr_equal(g[:3], O.csr_spgemm(m, k, A, B))

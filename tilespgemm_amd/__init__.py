@@ -63,6 +63,11 @@ SIGNATURES = {
                                 _vp, _vp, _vp]),
     "tsg_batch_symbolic": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
     "tsg_batch_numeric": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
+    "tsg_server_create": (_i, [_vp, _i, _vp]),
+    "tsg_server_submit": (_i, [_vp, _vp, _vp]),
+    "tsg_server_wait": (_i, [_vp, C.c_uint64, _vp]),
+    "tsg_server_release": (_i, [_vp, C.c_uint64]),
+    "tsg_server_destroy": (_i, [_vp]),
     "tsg_csr_transpose": (_i, [_vp, _i, _i, _vp, _vp, _vp, _i64, _vp, _vp, _vp]),
     "tsg_intermediate_products": (_i, [_vp, _i, _vp, _vp, _i64, _vp, _vp]),
     "tsg_plan_array_bytes": (_sz, [_vp, _i]),
@@ -158,6 +163,92 @@ class Product(C.Structure):
         return cls(A.rows, A.cols, B.cols, A.indptr.data_ptr(), A.indices.data_ptr(),
                    A.data.data_ptr(), A.nnz, B.indptr.data_ptr(), B.indices.data_ptr(),
                    B.data.data_ptr(), B.nnz)
+
+
+class ServerConfig(C.Structure):
+    """tsg_server_config_t (include/tsg.h)."""
+    _fields_ = [("workers", C.c_int), ("max_batch", C.c_int), ("mem_budget", C.c_size_t),
+                ("method", C.c_int)]
+
+
+class Result(C.Structure):
+    """tsg_result_t (include/tsg.h)."""
+    _fields_ = [("status", C.c_int), ("m", C.c_int), ("nnz", C.c_int64),
+                ("ic", C.c_void_p), ("jc", C.c_void_p), ("c", C.c_void_p),
+                ("queue_ms", C.c_double), ("total_ms", C.c_double), ("batch_size", C.c_int)]
+
+
+class Server:
+    """Continuous-batching request server on one device (tsg_server_*).
+
+    submit(A, B) enqueues C = A*B on device-resident operands (keep them alive
+    until the result is released) and returns a ticket; wait(ticket) blocks
+    and returns the Result; fetch(ticket) copies it to host numpy arrays and
+    releases it."""
+
+    def __init__(self, device: int = 0, workers: int = 2, max_batch: int = 64,
+                 mem_budget: int = 32 << 30, method: int = METHOD_AUTO):
+        self._s = C.c_void_p()
+        cfg = ServerConfig(workers, max_batch, mem_budget, method)
+        _check(load().tsg_server_create(C.byref(self._s), device, C.byref(cfg)))
+        self._keep = {}
+
+    def submit(self, A: DeviceCsr, B: DeviceCsr) -> int:
+        req = Product.of(A, B)
+        t = C.c_uint64(0)
+        _check(load().tsg_server_submit(self._s, C.byref(req), C.byref(t)))
+        self._keep[t.value] = (A, B)
+        return t.value
+
+    def wait(self, ticket: int) -> Result:
+        res = Result()
+        rc = load().tsg_server_wait(self._s, ticket, C.byref(res))
+        if rc != 0:
+            raise TsgError(rc, f"request {ticket} failed")
+        return res
+
+    def release(self, ticket: int):
+        _check(load().tsg_server_release(self._s, ticket))
+        self._keep.pop(ticket, None)
+
+    def fetch(self, ticket: int):
+        """(ic int64, jc int32, c float64) on the host, then release."""
+        import torch
+        res = self.wait(ticket)
+        ic = np.empty(res.m + 1, dtype=np.int64)
+        jc = np.empty(res.nnz, dtype=np.int32)
+        c = np.empty(res.nnz, dtype=np.float64)
+        torch.cuda.synchronize()
+        _d2h(ic, res.ic)
+        if res.nnz:
+            _d2h(jc, res.jc)
+            _d2h(c, res.c)
+        self.release(ticket)
+        return ic, jc, c, res
+
+    def close(self):
+        if getattr(self, "_s", None):
+            load().tsg_server_destroy(self._s)
+            self._s = None
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _d2h(dst: np.ndarray, src_ptr: int):
+    """Copy device memory at src_ptr into the host array dst with the HIP
+    runtime libtsg itself is linked against (symbol lookup through the
+    library's handle searches its dependencies)."""
+    f = load().hipMemcpy
+    f.restype = C.c_int
+    f.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    rc = f(dst.ctypes.data_as(C.c_void_p), C.c_void_p(src_ptr), dst.nbytes, 2)
+    if rc != 0:
+        raise TsgError(-5, f"hipMemcpy device->host failed ({rc})")
 
 
 class Handle:

@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtsg.so")
-SOURCES = ["tsg_util.hip", "tsg_row.hip", "tsg_tile.hip", "tsg_batch.hip", "tsg_api.hip"]
+SOURCES = ["tsg_util.hip", "tsg_row.hip", "tsg_tile.hip", "tsg_batch.hip", "tsg_server.hip", "tsg_api.hip"]
 ARCH = os.environ.get("TSG_OFFLOAD_ARCH", "gfx950")
 
 

@@ -118,6 +118,8 @@ int tsg_destroy(tsg_handle_t h) {
         if (h->stream) (void)hipStreamSynchronize(h->stream);
         h->tm.clear();
         h->ws.release();
+        h->bws.release();
+        h->aux.release();
         if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
     }
     delete h;
@@ -245,8 +247,8 @@ int tsg_csr_transpose(tsg_handle_t h, int m, int n, const int *d_ia, const int *
         check_csr_args(m, n, d_ia, d_ja, nnz, "A");
         TSG_CHECK(d_tp && (nnz == 0 || d_ti), TSG_ERR_INVALID, "null output");
         DeviceGuard dg(h->device);
-        Arena tmp;
-        csr_transpose(m, n, d_ia, d_ja, d_a, nnz, d_tp, d_ti, d_a ? d_tv : nullptr, tmp,
+        h->aux.reset();
+        csr_transpose(m, n, d_ia, d_ja, d_a, nnz, d_tp, d_ti, d_a ? d_tv : nullptr, h->aux,
                       h->stream);
         TSG_HIP_TRY(hipStreamSynchronize(h->stream));
     });
@@ -276,9 +278,9 @@ int tsg_intermediate_products(tsg_handle_t h, int m, const int *d_ia, const int 
     return guarded(h, [&] {
         check_csr_args(m, 1, d_ia, d_ja, nnz_a, "A");
         DeviceGuard dg(h->device);
-        Arena tmp;
-        int64_t *rp = tmp.alloc_n<int64_t>((size_t)m + 1);
-        int64_t *sc = tmp.alloc_n<int64_t>(scan_tmp_elems((size_t)m + 1));
+        h->aux.reset();
+        int64_t *rp = h->aux.alloc_n<int64_t>((size_t)m + 1);
+        int64_t *sc = h->aux.alloc_n<int64_t>(scan_tmp_elems((size_t)m + 1));
         csr_row_products(m, d_ia, d_ja, d_ib, rp, h->stream);
         exclusive_scan_i64(rp, rp, (size_t)m, sc, h->stream);
         int64_t tot = 0;

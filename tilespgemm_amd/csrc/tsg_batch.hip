@@ -87,6 +87,22 @@ __global__ void k_gather_row_ptrs(int count, const int64_t *__restrict__ off,
     if (p <= count) out[p] = ic[off[OFF_ROW * (int64_t)(count + 1) + p]];
 }
 
+// Per-product row pointers starting at 0: product i's m_i+1 entries go to
+// out[row_off[i] + i ...].
+__global__ void k_rebase(int count, const int64_t *__restrict__ off, int64_t rows,
+                         const int64_t *__restrict__ ic, int64_t *__restrict__ out) {
+    const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= rows + count) return;
+    const int64_t *roff = off + OFF_ROW * (int64_t)(count + 1);
+    int lo = 0, hi = count - 1;                 // largest i with roff[i] + i <= o
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (roff[mid] + mid <= o) lo = mid; else hi = mid - 1;
+    }
+    const int64_t r = roff[lo] + (o - (roff[lo] + lo));
+    out[o] = ic[r] - ic[roff[lo]];
+}
+
 bool same_product(const tsg_product_t &x, const tsg_product_t &y) {
     return x.m == y.m && x.n == y.n && x.k == y.k && x.ia == y.ia && x.ja == y.ja &&
            x.a == y.a && x.nnz_a == y.nnz_a && x.ib == y.ib && x.jb == y.jb && x.b == y.b &&
@@ -220,4 +236,13 @@ void batch_numeric(tsg_handle_t h, int count, const tsg_product_t *prods, const 
                            count, S.row_off, (int64_t)S.rows, d_ic, d_jc);
         TSG_LAUNCH_CHECK();
     }
+}
+
+void batch_rebase(tsg_handle_t h, const int64_t *d_ic, int64_t *d_out) {
+    BatchState &S = h->batch;
+    TSG_CHECK(S.valid, TSG_ERR_STATE, "no batch");
+    const int64_t total = (int64_t)S.rows + S.count;
+    hipLaunchKernelGGL(k_rebase, dim3(grid_for((size_t)total, 256)), dim3(256), 0, h->stream,
+                       S.count, S.row_off, (int64_t)S.rows, d_ic, d_out);
+    TSG_LAUNCH_CHECK();
 }

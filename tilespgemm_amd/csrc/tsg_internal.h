@@ -205,6 +205,7 @@ struct tsg_handle_s {
     int method_used = TSG_METHOD_AUTO;
     Arena ws;       // plan of the last symbolic phase
     Arena bws;      // batch layout (outlives the plan's reset)
+    Arena aux;      // scratch of utility calls (transpose, product counts)
     RowPlan row;
     TilePlan tile;
     BatchState batch;
@@ -216,6 +217,9 @@ void batch_symbolic(tsg_handle_t h, int count, const tsg_product_t *prods, int64
                     int64_t *nnz_c_total, int64_t *nnz_c);
 void batch_numeric(tsg_handle_t h, int count, const tsg_product_t *prods, const int64_t *d_ic,
                    int *d_jc, double *d_c);
+// Row pointers of every product of the last batch rebased to start at 0,
+// m_i + 1 entries each, back to back (rows + count entries).
+void batch_rebase(tsg_handle_t h, const int64_t *d_ic, int64_t *d_out);
 
 // CSR utilities
 // Validates A and B on the device (one sync).  Returns flags of A in bits

@@ -1,0 +1,356 @@
+// Request server with continuous batching (include/tsg.h, tsg_server_*).
+//
+// Requests carry device-resident operands.  Submission only enqueues; an
+// estimator thread (own handle and stream) computes each request's output
+// upper bound from its intermediate products (the admission estimate) and
+// moves it, in arrival order, to the admission queue.  Each worker thread owns
+// a tsg handle (its own stream) and repeatedly takes, from the front of that
+// queue, every request that fits the memory budget and runs them as one
+// batched product (block-diagonal layout, tsg_batch.hip).  With several
+// workers, the next batch is formed and launched while earlier ones are still
+// on the device.  Output buffers come from a best-fit device cache and are
+// returned when the caller releases the result.
+#include "tsg_internal.h"
+
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double ms_between(Clock::time_point a, Clock::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+// Best-fit cache of device blocks (hipMalloc on a miss, reuse after put).
+class DevicePool {
+public:
+    ~DevicePool() {
+        std::lock_guard<std::mutex> lk(mu_);
+        for (auto &kv : size_) (void)hipFree(kv.first);
+        size_.clear();
+        free_.clear();
+    }
+    void *get(size_t bytes) {
+        bytes = std::max<size_t>(256, (bytes + 255) & ~size_t(255));
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            auto it = free_.lower_bound(bytes);
+            if (it != free_.end() && it->first <= 2 * bytes + (size_t(1) << 20)) {
+                void *p = it->second;
+                free_.erase(it);
+                return p;
+            }
+        }
+        void *p = nullptr;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            // drop the cache and retry once
+            clear();
+            e = hipMalloc(&p, bytes);
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                throw TsgError{TSG_ERR_ALLOC, "server output allocation failed"};
+            }
+        }
+        std::lock_guard<std::mutex> lk(mu_);
+        size_[p] = bytes;
+        return p;
+    }
+    void put(void *p) {
+        if (!p) return;
+        std::lock_guard<std::mutex> lk(mu_);
+        free_.emplace(size_[p], p);
+    }
+    void clear() {
+        std::lock_guard<std::mutex> lk(mu_);
+        for (auto &kv : free_) {
+            (void)hipFree(kv.second);
+            size_.erase(kv.second);
+        }
+        free_.clear();
+    }
+
+private:
+    std::mutex mu_;
+    std::multimap<size_t, void *> free_;
+    std::unordered_map<void *, size_t> size_;
+};
+
+struct BatchOut {
+    void *ic = nullptr, *jc = nullptr, *c = nullptr;
+    int refs = 0;
+};
+
+struct Req {
+    uint64_t ticket = 0;
+    tsg_product_t prod{};
+    size_t est = 0;
+    Clock::time_point t_submit, t_start, t_done;
+    bool done = false;
+    tsg_result_t res{};
+    std::shared_ptr<BatchOut> out;
+};
+
+}  // namespace
+
+struct tsg_server_s {
+    int device = 0;
+    tsg_server_config_t cfg{};
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done, cv_est;
+    std::deque<Req *> pending;         // submitted, not yet estimated
+    std::deque<Req *> queue;           // estimated, waiting for admission
+    std::unordered_map<uint64_t, std::unique_ptr<Req>> reqs;
+    uint64_t next_ticket = 1;
+    bool stop = false;
+    bool est_done = false;             // estimator has drained and exited
+    size_t in_flight = 0;              // estimated bytes of admitted, unreleased requests
+    int running = 0;                   // batches on the device now
+    std::vector<std::thread> workers;
+    std::thread estimator;
+    tsg_handle_t est = nullptr;        // owned by the estimator thread
+    DevicePool pool;
+
+    void run_batch(tsg_handle_t h, std::vector<Req *> &batch);
+    void worker_main();
+    void estimator_main();
+};
+
+void tsg_server_s::estimator_main() {
+    const bool dev_ok = hipSetDevice(device) == hipSuccess;
+    for (;;) {
+        std::deque<Req *> take;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv_est.wait(lk, [&] { return stop || !pending.empty(); });
+            if (pending.empty()) {
+                est_done = true;
+                cv_work.notify_all();
+                return;
+            }
+            take.swap(pending);
+        }
+        std::vector<int> status(take.size(), TSG_OK);
+        for (size_t i = 0; i < take.size(); i++) {
+            Req *r = take[i];
+            uint64_t products = 0;
+            int rc = dev_ok ? tsg_intermediate_products(est, r->prod.m, r->prod.ia, r->prod.ja,
+                                                        r->prod.nnz_a, r->prod.ib, &products)
+                            : TSG_ERR_NODEVICE;
+            status[i] = rc;
+            const uint64_t cap =
+                std::min<uint64_t>(products, (uint64_t)r->prod.m * (uint64_t)r->prod.k);
+            r->est = rc == TSG_OK ? 16 * ((size_t)r->prod.m + 2) + 12 * (size_t)cap : 0;
+        }
+        std::lock_guard<std::mutex> lk(mu);
+        bool failed = false;
+        for (size_t i = 0; i < take.size(); i++) {
+            Req *r = take[i];
+            if (status[i] == TSG_OK) {
+                queue.push_back(r);
+                continue;
+            }
+            r->done = true;
+            r->t_start = r->t_done = Clock::now();
+            r->res.status = status[i];
+            r->res.m = r->prod.m;
+            failed = true;
+        }
+        cv_work.notify_all();
+        if (failed) cv_done.notify_all();
+    }
+}
+
+void tsg_server_s::run_batch(tsg_handle_t h, std::vector<Req *> &batch) {
+    const int n = (int)batch.size();
+    std::vector<tsg_product_t> prods(n);
+    int rows = 0;
+    for (int i = 0; i < n; i++) {
+        prods[i] = batch[i]->prod;
+        rows += batch[i]->prod.m;
+    }
+    auto t0 = Clock::now();
+    for (auto *r : batch) r->t_start = t0;
+    int status = TSG_OK;
+    auto out = std::make_shared<BatchOut>();
+    std::vector<int64_t> per(n, 0);
+    void *ic_glob = nullptr;
+    try {
+        ic_glob = pool.get(sizeof(int64_t) * ((size_t)rows + 1));
+        int64_t tot = 0;
+        int rc = tsg_batch_symbolic(h, n, prods.data(), (int64_t *)ic_glob, &tot, per.data());
+        if (rc != TSG_OK) throw TsgError{rc, tsg_last_error(h)};
+        out->ic = pool.get(sizeof(int64_t) * ((size_t)rows + n));
+        out->jc = pool.get(sizeof(int) * ((size_t)tot + 1));
+        out->c = pool.get(sizeof(double) * ((size_t)tot + 1));
+        rc = tsg_batch_numeric(h, n, prods.data(), (const int64_t *)ic_glob, (int *)out->jc,
+                               (double *)out->c);
+        if (rc != TSG_OK) throw TsgError{rc, tsg_last_error(h)};
+        batch_rebase(h, (const int64_t *)ic_glob, (int64_t *)out->ic);
+        TSG_HIP_TRY(hipStreamSynchronize(h->stream));
+    } catch (const TsgError &e) {
+        status = e.code;
+    }
+    pool.put(ic_glob);
+    auto t1 = Clock::now();
+    std::lock_guard<std::mutex> lk(mu);
+    running--;
+    cv_work.notify_all();
+    int64_t e0 = 0, r0 = 0;
+    out->refs = n;
+    for (int i = 0; i < n; i++) {
+        Req *r = batch[i];
+        r->t_done = t1;
+        r->done = true;
+        r->out = out;
+        r->res.status = status;
+        r->res.m = r->prod.m;
+        r->res.nnz = status == TSG_OK ? per[i] : 0;
+        r->res.ic = status == TSG_OK ? (const int64_t *)out->ic + r0 + i : nullptr;
+        r->res.jc = status == TSG_OK ? (const int *)out->jc + e0 : nullptr;
+        r->res.c = status == TSG_OK ? (const double *)out->c + e0 : nullptr;
+        r->res.queue_ms = ms_between(r->t_submit, r->t_start);
+        r->res.total_ms = ms_between(r->t_submit, t1);
+        r->res.batch_size = n;
+        e0 += per[i];
+        r0 += r->prod.m;
+    }
+    cv_done.notify_all();
+}
+
+void tsg_server_s::worker_main() {
+    if (hipSetDevice(device) != hipSuccess) return;
+    tsg_handle_t h = nullptr;
+    if (tsg_create(&h, device) != TSG_OK) return;
+    tsg_set_method(h, cfg.method);
+    for (;;) {
+        std::vector<Req *> batch;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            for (;;) {
+                if (est_done && queue.empty()) {
+                    lk.unlock();
+                    tsg_destroy(h);
+                    return;
+                }
+                // admission: FIFO, every waiting request that fits the budget
+                // (estimates of admitted requests count until released).  When
+                // no batch is running, the head request is admitted even if it
+                // does not fit, so a caller holding results it has not released
+                // while waiting for a later ticket still makes progress; at
+                // shutdown the queue drains regardless of the budget
+                while (!queue.empty() && (int)batch.size() < cfg.max_batch) {
+                    Req *r = queue.front();
+                    const bool fits = stop || in_flight + r->est <= cfg.mem_budget;
+                    const bool alone = running == 0 && batch.empty();
+                    if (!fits && !alone) break;
+                    queue.pop_front();
+                    in_flight += r->est;
+                    batch.push_back(r);
+                }
+                if (!batch.empty()) {
+                    running++;
+                    break;
+                }
+                cv_work.wait(lk);
+            }
+        }
+        run_batch(h, batch);
+    }
+}
+
+extern "C" {
+
+int tsg_server_create(tsg_server_t *server, int device, const tsg_server_config_t *config) {
+    if (!server || !config || config->workers < 1 || config->max_batch < 1) return TSG_ERR_INVALID;
+    *server = nullptr;
+    auto s = std::make_unique<tsg_server_s>();
+    s->device = device;
+    s->cfg = *config;
+    if (s->cfg.mem_budget == 0) s->cfg.mem_budget = size_t(64) << 30;
+    int rc = tsg_create(&s->est, device);
+    if (rc != TSG_OK) return rc;
+    s->estimator = std::thread([p = s.get()] { p->estimator_main(); });
+    for (int i = 0; i < s->cfg.workers; i++)
+        s->workers.emplace_back([p = s.get()] { p->worker_main(); });
+    *server = s.release();
+    return TSG_OK;
+}
+
+int tsg_server_submit(tsg_server_t s, const tsg_product_t *req, uint64_t *ticket) {
+    if (!s || !req || !ticket) return TSG_ERR_INVALID;
+    if (req->m < 0 || req->n < 0 || req->k < 0 || req->nnz_a < 0 || req->nnz_b < 0)
+        return TSG_ERR_INVALID;
+    if (!req->ia || !req->ib || (req->nnz_a > 0 && (!req->ja || !req->a)) ||
+        (req->nnz_b > 0 && (!req->jb || !req->b)))
+        return TSG_ERR_INVALID;
+    auto r = std::make_unique<Req>();
+    r->prod = *req;
+    r->t_submit = Clock::now();
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) return TSG_ERR_STATE;
+    r->ticket = s->next_ticket++;
+    *ticket = r->ticket;
+    s->pending.push_back(r.get());
+    s->reqs.emplace(r->ticket, std::move(r));
+    s->cv_est.notify_one();
+    return TSG_OK;
+}
+
+int tsg_server_wait(tsg_server_t s, uint64_t ticket, tsg_result_t *result) {
+    if (!s || !result) return TSG_ERR_INVALID;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return TSG_ERR_INVALID;
+    Req *r = it->second.get();
+    s->cv_done.wait(lk, [&] { return r->done; });
+    *result = r->res;
+    return r->res.status;
+}
+
+int tsg_server_release(tsg_server_t s, uint64_t ticket) {
+    if (!s) return TSG_ERR_INVALID;
+    std::shared_ptr<BatchOut> out;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->reqs.find(ticket);
+        if (it == s->reqs.end() || !it->second->done) return TSG_ERR_INVALID;
+        Req *r = it->second.get();
+        s->in_flight -= r->est;
+        out = r->out;
+        if (out && --out->refs == 0) {
+            s->pool.put(out->ic);
+            s->pool.put(out->jc);
+            s->pool.put(out->c);
+        }
+        s->reqs.erase(it);
+        s->cv_work.notify_all();
+    }
+    return TSG_OK;
+}
+
+int tsg_server_destroy(tsg_server_t s) {
+    if (!s) return TSG_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+        s->cv_est.notify_all();
+        s->cv_work.notify_all();
+    }
+    s->estimator.join();
+    for (auto &t : s->workers) t.join();
+    tsg_destroy(s->est);
+    delete s;
+    return TSG_OK;
+}
+
+}  // extern "C"
