@@ -45,9 +45,11 @@ def make_workload(name):
         return dict(name="R-MAT n=2^20, 16M draws, (.57,.19,.19,.05), U[-1,1], seed 1; C=A*A^T",
                     m=1 << 20, n=1 << 20, A=A, aat=True)
     if name.startswith("rmat") and name.endswith("_aat"):
-        s = int(name[4:-4])
-        A = gen.rmat(s, 16 * (1 << s) // 1, seed=1)
-        return dict(name=f"R-MAT n=2^{s}, {16 << s} draws, seed 1; C=A*A^T",
+        # rmat<scale>_aat (16 draws per row) or rmat<scale>e<factor>_aat
+        spec = name[4:-4]
+        s, ef = (int(x) for x in spec.split("e")) if "e" in spec else (int(spec), 16)
+        A = gen.rmat(s, ef << s, seed=1)
+        return dict(name=f"R-MAT n=2^{s}, {ef << s} draws, seed 1; C=A*A^T",
                     m=1 << s, n=1 << s, A=A, aat=True)
     if name == "laplacian256_aa":
         A = gen.laplacian2d(256)

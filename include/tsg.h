@@ -205,9 +205,11 @@ int tsg_csr_transpose(tsg_handle_t handle, int m, int n,
                       int64_t nnz, int *d_tp, int *d_ti, double *d_tv);
 
 /* Sum over A's entries a_ik of |row k of B| -- the intermediate-product
- * count (reference main_spgemm-col.cu:4950-4955); flops = 2 * this.
+ * count (reference main_spgemm-col.cu:4950-4955); flops = 2 * this.  A is
+ * m x n (n = rows of B, d_ib has n+1 entries) and is validated first
+ * (TSG_ERR_INVALID for bad row pointers or columns outside [0, n)).
  * Synchronizes the stream. */
-int tsg_intermediate_products(tsg_handle_t handle, int m, const int *d_ia,
+int tsg_intermediate_products(tsg_handle_t handle, int m, int n, const int *d_ia,
                               const int *d_ja, int64_t nnz_a, const int *d_ib,
                               uint64_t *products);
 

@@ -228,6 +228,55 @@ def test_intermediate_products(handle):
     S = gen.stencil27(11, seed=0)
     dS = to_dev(1331, 1331, S)
     assert handle.intermediate_products(dS, dS) == (9 * 11 - 10) ** 3
+    # heavy-tailed rows (one row spans many threads' entry slices) and empty rows
+    R = gen.rmat(14, 200_000, seed=4)
+    RT = gen.transpose(1 << 14, 1 << 14, *R)
+    dR, dRT = to_dev(1 << 14, 1 << 14, R), to_dev(1 << 14, 1 << 14, RT)
+    assert np.diff(R[0]).max() > 1000 and (np.diff(R[0]) == 0).any()
+    assert handle.intermediate_products(dR, dRT) == gen.intermediate_products(R[0], R[1], RT[0])
+    # A with a column outside B's rows is rejected before any product is counted
+    bad = (R[0], R[1].copy(), R[2])
+    bad[1][5] = 1 << 14
+    with pytest.raises(tsg.TsgError):
+        handle.intermediate_products(to_dev(1 << 14, 1 << 14, bad), dRT)
+
+
+@pytest.mark.parametrize("where", ["slice_edge", "mid_long_row", "duplicate", "row_start"])
+def test_sortedness_detection_on_long_rows(handle, where):
+    """The tile path needs sorted, duplicate-free rows; the entry-parallel
+    check must see a descent anywhere inside a long row (including across
+    the 8-entry slices its threads walk) and must not flag the descent from
+    one row's last column to the next row's first."""
+    R = gen.rmat(12, 60_000, seed=6)
+    ip, ix, d = (x.copy() for x in R)
+    lens = np.diff(ip)
+    r = int(np.argmax(lens))
+    lo = int(ip[r])
+    if where == "row_start":
+        # rows untouched: every row start is a descent but the input is valid
+        run(handle, 4096, 4096, 4096, (ip, ix, d), R, tsg.METHOD_TILE)
+        assert handle.method_used == tsg.METHOD_TILE
+        return
+    if where == "slice_edge":
+        q = lo + ((8 - lo % 8) % 8) + 8          # first entry of a thread slice inside row r
+    elif where == "mid_long_row":
+        q = lo + lens[r] // 2 + 3
+    else:
+        q = lo + lens[r] // 3
+    if where == "duplicate":
+        ix[q] = ix[q - 1]
+        U = (ip, ix, d)
+    else:
+        ix[q - 1], ix[q] = ix[q], ix[q - 1]
+        d[q - 1], d[q] = d[q], d[q - 1]
+        U = (ip, ix, d)
+    with pytest.raises(tsg.TsgError):
+        run(handle, 4096, 4096, 4096, U, R, tsg.METHOD_TILE)
+    got = run(handle, 4096, 4096, 4096, U, R, tsg.METHOD_AUTO)
+    assert handle.method_used == tsg.METHOD_ROW
+    want = O.ref_thmkl(4096, 4096, 4096, U, R, "fma") if O.ref_available() else \
+        O.csr_spgemm(4096, 4096, U, R)
+    assert_csr_equal(got, want)
 
 
 @pytest.mark.parametrize("name", ["lap16_AA_refvals", "rmat9_AAT", "rect_203x157x91"])

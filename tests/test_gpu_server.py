@@ -97,3 +97,26 @@ def test_concurrent_submitters_and_budget():
         srv.close()
     for (m, n, k, A, B, same), g in zip(mix, got):
         assert_csr_equal(g[:3], O.csr_spgemm(m, k, A, B))
+
+
+def test_malformed_request_fails_alone():
+    """A request whose A has a column outside B's rows is rejected by the
+    estimator with an error of its own; the requests around it are served."""
+    import torch
+    assert torch.cuda.is_available()
+    A = gen.erdos_renyi(2000, 6.0, seed=3)
+    bad = (A[0], A[1].copy(), A[2])
+    bad[1][7] = 5000
+    dA = tsg.DeviceCsr.from_host(2000, 2000, *A)
+    dbad = tsg.DeviceCsr.from_host(2000, 2000, *bad)
+    srv = tsg.Server(0, workers=2, max_batch=8)
+    try:
+        t = [srv.submit(dA, dA), srv.submit(dbad, dA), srv.submit(dA, dA)]
+        with pytest.raises(tsg.TsgError):
+            srv.wait(t[1])
+        srv.release(t[1])
+        want = O.csr_spgemm(2000, 2000, A, A)
+        for x in (t[0], t[2]):
+            assert_csr_equal(srv.fetch(x)[:3], want)
+    finally:
+        srv.close()

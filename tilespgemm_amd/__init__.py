@@ -69,7 +69,7 @@ SIGNATURES = {
     "tsg_server_release": (_i, [_vp, C.c_uint64]),
     "tsg_server_destroy": (_i, [_vp]),
     "tsg_csr_transpose": (_i, [_vp, _i, _i, _vp, _vp, _vp, _i64, _vp, _vp, _vp]),
-    "tsg_intermediate_products": (_i, [_vp, _i, _vp, _vp, _i64, _vp, _vp]),
+    "tsg_intermediate_products": (_i, [_vp, _i, _i, _vp, _vp, _i64, _vp, _vp]),
     "tsg_plan_array_bytes": (_sz, [_vp, _i]),
     "tsg_plan_array_copy": (_i, [_vp, _i, _vp, _sz]),
     "tsg_enable_timing": (_i, [_vp, _i]),
@@ -389,7 +389,9 @@ class Handle:
 
     def intermediate_products(self, A: DeviceCsr, B: DeviceCsr) -> int:
         out = C.c_uint64(0)
-        _check(load().tsg_intermediate_products(self._h, A.rows, _ptr(A.indptr),
+        if A.cols != B.rows:
+            raise TsgError(-1, "A.cols != B.rows")
+        _check(load().tsg_intermediate_products(self._h, A.rows, A.cols, _ptr(A.indptr),
                                                 _ptr(A.indices), A.nnz, _ptr(B.indptr),
                                                 C.byref(out)), self._h)
         return int(out.value)

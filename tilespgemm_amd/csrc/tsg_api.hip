@@ -272,16 +272,22 @@ int tsg_batch_numeric(tsg_handle_t h, int count, const tsg_product_t *prods, con
     });
 }
 
-int tsg_intermediate_products(tsg_handle_t h, int m, const int *d_ia, const int *d_ja,
+int tsg_intermediate_products(tsg_handle_t h, int m, int n, const int *d_ia, const int *d_ja,
                               int64_t nnz_a, const int *d_ib, uint64_t *products) {
     if (!h || !products) return TSG_ERR_INVALID;
     return guarded(h, [&] {
-        check_csr_args(m, 1, d_ia, d_ja, nnz_a, "A");
+        check_csr_args(m, n, d_ia, d_ja, nnz_a, "A");
+        TSG_CHECK(d_ib != nullptr, TSG_ERR_INVALID, "B: null row pointer");
         DeviceGuard dg(h->device);
         h->aux.reset();
+        CsrIn A{m, n, d_ia, d_ja, nullptr, nnz_a};
+        const int vf = validate_csr_pair(A, A, h->aux, h->stream);
+        TSG_CHECK((vf & 0x3) == 0, TSG_ERR_INVALID,
+                  std::string("invalid CSR input: A") +
+                      ((vf & 0x1) ? " has bad row pointers" : " has a column out of range"));
         int64_t *rp = h->aux.alloc_n<int64_t>((size_t)m + 1);
         int64_t *sc = h->aux.alloc_n<int64_t>(scan_tmp_elems((size_t)m + 1));
-        csr_row_products(m, d_ia, d_ja, d_ib, rp, h->stream);
+        csr_row_products(m, d_ia, d_ja, nnz_a, d_ib, rp, h->aux, h->stream);
         exclusive_scan_i64(rp, rp, (size_t)m, sc, h->stream);
         int64_t tot = 0;
         TSG_HIP_TRY(hipMemcpyAsync(&tot, rp + m, sizeof(int64_t), hipMemcpyDeviceToHost,

@@ -90,19 +90,28 @@ __device__ __forceinline__ void for_products(int pa, int pe, int gofs, int gstri
             int e[U], sl[U];
             bool act[U];
             double as[U];
+            // batches of this chunk holding products: the rest are skipped
+            // (uniform over the group; with G == 64, over the wave)
+            const int nb = (total - base + G * bstep - 1) / (G * bstep);
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                const int q = base + u * G * bstep + l;
-                act[u] = q < total;
-                int s = 0;
+                act[u] = false;
+                as[u] = 0.0;
+                e[u] = 0;
+                sl[u] = 0;
+                if (u < nb) {
+                    const int q = base + u * G * bstep + l;
+                    act[u] = q < total;
+                    int s = 0;
 #pragma unroll
-                for (int step = G / 2; step > 0; step >>= 1) {
-                    int probe = __shfl(incl, s + step - 1, G);
-                    if (probe <= q) s += step;
+                    for (int step = G / 2; step > 0; step >>= 1) {
+                        int probe = __shfl(incl, s + step - 1, G);
+                        if (probe <= q) s += step;
+                    }
+                    as[u] = VALUES ? __shfl(a, s, G) : 0.0;
+                    e[u] = __shfl(off, s, G) + q;
+                    sl[u] = s;
                 }
-                as[u] = VALUES ? __shfl(a, s, G) : 0.0;
-                e[u] = __shfl(off, s, G) + q;
-                sl[u] = s;
             }
             int col[U];
             double bv[U];

@@ -229,5 +229,5 @@ int validate_csr_pair(const CsrIn &A, const CsrIn &B, Arena &ws, hipStream_t s);
 void csr_transpose(int m, int n, const int *ia, const int *ja, const double *a,
                    int64_t nnz, int *tp, int *ti, double *tv, Arena &ws,
                    hipStream_t s, int *tpos = nullptr);
-void csr_row_products(int m, const int *ia, const int *ja, const int *ib,
-                      int64_t *rowprod, hipStream_t s);
+void csr_row_products(int m, const int *ia, const int *ja, int64_t nnz, const int *ib,
+                      int64_t *rowprod, Arena &ws, hipStream_t s);

@@ -42,6 +42,8 @@ constexpr int ITEM_CAP = 1024;         // outputs per item
 constexpr int ITEM_SLOTS = 2048;       // hash slots per item (load <= 1/2)
 constexpr int ITEM_WBITS_SHIFT = 15;   // item width <= 32768 columns
 constexpr int ITEM_WORDS = (1 << ITEM_WBITS_SHIFT) / 32;
+// items are whole sub-blocks, and one sub-block can hold 1 << SUB_SHIFT outputs
+static_assert(ITEM_CAP >= (1 << SUB_SHIFT), "an item must hold a full sub-block");
 constexpr int ITEM_TAGS = 256;
 constexpr int ITEM_WAVES = 1;          // waves per block (one item per wave)
 constexpr int ITEM_BLOCKS_PER_CU = 6;  // LDS-bound residency (~25 KB per wave)
@@ -72,15 +74,36 @@ __global__ void k_bin_count(int m, const int64_t *__restrict__ rowprod, int *__r
     if (threadIdx.x < NBINS && s_cnt[threadIdx.x]) atomicAdd(&cnt[threadIdx.x], s_cnt[threadIdx.x]);
 }
 
-__global__ void k_bin_scatter(int m, const int64_t *__restrict__ rowprod, BinStarts st,
-                              int *__restrict__ fill, int *__restrict__ bin_rows,
-                              int64_t *__restrict__ rownnz) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
-    int b = bin_of(rowprod[i]);
-    if (b == 0) { rownnz[i] = 0; return; }
-    int pos = atomicAdd(&fill[b], 1);
-    bin_rows[st.s[b] + pos] = i;
+// Rows go to their bin's list in any order: per wave, one ballot per bin
+// gives each lane its slot among the wave's rows of that bin; per block, one
+// LDS atomic per wave and bin; per grid, one global atomic per block and bin.
+__global__ void __launch_bounds__(256) k_bin_scatter(int m, const int64_t *__restrict__ rowprod,
+                                                     BinStarts st, int *__restrict__ fill,
+                                                     int *__restrict__ bin_rows,
+                                                     int64_t *__restrict__ rownnz) {
+    __shared__ int s_cnt[NBINS], s_base[NBINS];
+    if (threadIdx.x < NBINS) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int b = i < m ? bin_of(rowprod[i]) : -1;
+    if (b == 0) rownnz[i] = 0;
+    const uint64_t below = (uint64_t(1) << lane) - 1;
+    int pos = 0;
+    for (int q = 1; q < NBINS; q++) {
+        const uint64_t mask = __ballot(b == q);
+        if (!mask) continue;
+        const int leader = __ffsll((unsigned long long)mask) - 1;
+        int wbase = 0;
+        if (lane == leader) wbase = atomicAdd(&s_cnt[q], __popcll(mask));
+        wbase = __shfl(wbase, leader);
+        if (b == q) pos = wbase + __popcll(mask & below);
+    }
+    __syncthreads();
+    if (threadIdx.x > 0 && threadIdx.x < NBINS && s_cnt[threadIdx.x])
+        s_base[threadIdx.x] = atomicAdd(&fill[threadIdx.x], s_cnt[threadIdx.x]);
+    __syncthreads();
+    if (b > 0) bin_rows[st.s[b] + s_base[b] + pos] = i;
 }
 
 // ------------------------------------------------------ small rows: symbolic
@@ -297,14 +320,20 @@ __global__ void __launch_bounds__(BIG_T) k_big_symbolic(int nbig, const int *__r
                 for (int i = threadIdx.x; i < words; i += BIG_T) dst[i] = bits[i];
             }
             const int nsb = (pass1 - pass0 + (1 << SUB_SHIFT) - 1) >> SUB_SHIFT;
+            // sub-block counts: a sub-block's 32 words are one half-wave, read
+            // consecutively (no bank conflicts) and summed across the half
+            static_assert(SUB_SHIFT - 5 == 5, "a sub-block is 32 bitmap words");
             int part = 0;
-            for (int sb = threadIdx.x; sb < nsb; sb += BIG_T) {
-                int w0 = sb << (SUB_SHIFT - 5);
-                int w1 = min(words, w0 + (1 << (SUB_SHIFT - 5)));
-                int cnt = 0;
-                for (int w = w0; w < w1; w++) cnt += __popc(bits[w]);
-                subcnt[sbase + sb] = (uint16_t)cnt;
-                part += cnt;
+            const int lane = threadIdx.x & 63;
+            for (int w0 = wave * 64; w0 < (nsb << 5); w0 += BIG_T) {
+                const int w = w0 + lane;
+                int pc = w < words ? __popc(bits[w]) : 0;
+#pragma unroll
+                for (int d = 16; d > 0; d >>= 1) pc += __shfl_xor(pc, d, 64);
+                if ((lane & 31) == 0 && (w >> 5) < nsb) {
+                    subcnt[sbase + (w >> 5)] = (uint16_t)pc;
+                    part += pc;
+                }
             }
             part = wave_sum(part);
             if ((threadIdx.x & 63) == 0) s_total[wave] = part;
@@ -490,25 +519,39 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric_rank(
     for (int i = l; i < ITEM_TAGS; i += 64) S.tags[i] = 0xffffffffu;
     const int64_t wave_id = (int64_t)blockIdx.x * ITEM_WAVES + (threadIdx.x >> 6);
     const int64_t nwaves = (int64_t)gridDim.x * ITEM_WAVES;
+    constexpr int WCH = ITEM_WORDS / 64;
     for (int64_t it = wave_id; it < nitems; it += nwaves) {
         const int4 item = items[it];
-        const int row = item.x, c0 = item.y, c1 = item.z, big = item.w;
-        const int nwords = (c1 - c0 + 31) >> 5;
+        const int row = item.x, big = item.w;
         const unsigned *src = rowbits + sb_off[big] * (1 << (SUB_SHIFT - 5)) +
-                              ((c0 >> 5) - (span_sb0[big] << (SUB_SHIFT - 5)));
+                              ((item.y >> 5) - (span_sb0[big] << (SUB_SHIFT - 5)));
+        const int c0 = item.y, c1 = item.z;
+        const int nwords = (c1 - c0 + 31) >> 5;
+        // the item's bitmap words (<= ITEM_WORDS): every load is issued before
+        // the first is used
+        unsigned wv[WCH];
+#pragma unroll
+        for (int j = 0; j < WCH; j++) {
+            const int w = j * 64 + l;
+            wv[j] = w < nwords ? src[w] : 0u;
+        }
+        // rank table: the 64-word chunks are scanned independently
+        int incl[WCH], tot[WCH];
+#pragma unroll
+        for (int j = 0; j < WCH; j++) {
+            const int w = j * 64 + l;
+            const int lastbits = (c1 - c0) - (w << 5);
+            if (w < nwords && lastbits < 32) wv[j] &= (1u << lastbits) - 1u;
+            incl[j] = wave_incl_scan(__popc(wv[j]));
+            tot[j] = __shfl(incl[j], 63, 64);
+        }
         int cnt = 0;
-        for (int w0 = 0; w0 < nwords; w0 += 64) {
-            const int w = w0 + l;
-            unsigned word = 0u;
-            if (w < nwords) {
-                word = src[w];
-                const int lastbits = (c1 - c0) - (w << 5);
-                if (lastbits < 32) word &= (1u << lastbits) - 1u;
-            }
-            const int pc = __popc(word);
-            const int incl = wave_incl_scan(pc);
-            if (w < nwords) S.words[w] = make_uint2(word, (unsigned)(cnt + incl - pc));
-            cnt += __shfl(incl, 63, 64);
+#pragma unroll
+        for (int j = 0; j < WCH; j++) {
+            const int w = j * 64 + l;
+            if (w < nwords)
+                S.words[w] = make_uint2(wv[j], (unsigned)(cnt + incl[j] - __popc(wv[j])));
+            cnt += tot[j];
         }
         for (int i = l; i < cnt; i += 64) S.vals[i] = 0.0;
         wave_lds_fence();
@@ -532,19 +575,26 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric_rank(
         wave_lds_fence();
         const int64_t o = item_off[it];
         for (int i = l; i < cnt; i += 64) c[o + i] = S.vals[i];
-        for (int w0 = 0; w0 < nwords; w0 += 64) {
-            const int w = w0 + l;
+        // columns: expanded from the words into LDS (over the values, which are
+        // written), then stored coalesced
+        wave_lds_fence();
+        int *cols = reinterpret_cast<int *>(S.vals);
+#pragma unroll
+        for (int j = 0; j < WCH; j++) {
+            const int w = j * 64 + l;
             if (w < nwords) {
-                uint2 wd = S.words[w];
-                int64_t pos = o + wd.y;
+                const uint2 wd = S.words[w];
+                int pos = (int)wd.y;
                 unsigned word = wd.x;
                 while (word) {
                     const int bit = __ffs(word) - 1;
                     word &= word - 1;
-                    jc[pos++] = c0 + (w << 5) + bit;
+                    cols[pos++] = c0 + (w << 5) + bit;
                 }
             }
         }
+        wave_lds_fence();
+        for (int i = l; i < cnt; i += 64) jc[o + i] = cols[i];
         wave_lds_fence();
     }
 }
@@ -580,7 +630,7 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
     int64_t *scan_tmp = ws.alloc_n<int64_t>(scan_tmp_elems((size_t)m + 1));
 
     tm.begin("row.products+bins");
-    csr_row_products(m, A.ptr, A.idx, B.ptr, p.rowprod, s);
+    csr_row_products(m, A.ptr, A.idx, A.nnz, B.ptr, p.rowprod, ws, s);
     TSG_HIP_TRY(hipMemsetAsync(d_cnt, 0, sizeof(int) * (2 * NBINS + 2), s));
     if (m > 0) {
         hipLaunchKernelGGL(k_bin_count, dim3(grid_for(m, 256)), dim3(256), 0, s, m, p.rowprod,

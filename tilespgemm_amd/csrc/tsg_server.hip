@@ -143,9 +143,23 @@ void tsg_server_s::estimator_main() {
         for (size_t i = 0; i < take.size(); i++) {
             Req *r = take[i];
             uint64_t products = 0;
-            int rc = dev_ok ? tsg_intermediate_products(est, r->prod.m, r->prod.ia, r->prod.ja,
-                                                        r->prod.nnz_a, r->prod.ib, &products)
+            int rc = dev_ok ? tsg_intermediate_products(est, r->prod.m, r->prod.n, r->prod.ia,
+                                                        r->prod.ja, r->prod.nnz_a, r->prod.ib,
+                                                        &products)
                             : TSG_ERR_NODEVICE;
+            if (rc == TSG_OK) {
+                // B checked here too, so a malformed request fails alone
+                // instead of failing the batch it would join
+                try {
+                    TSG_CHECK(r->prod.nnz_b < (int64_t(1) << 31), TSG_ERR_OVERFLOW,
+                              "B: inputs are limited to 2^31-1 entries");
+                    CsrIn B{r->prod.n, r->prod.k, r->prod.ib, r->prod.jb, nullptr, r->prod.nnz_b};
+                    if (validate_csr_pair(B, B, est->aux, est->stream) & 0x3)
+                        rc = TSG_ERR_INVALID;
+                } catch (const TsgError &e) {
+                    rc = e.code;
+                }
+            }
             status[i] = rc;
             const uint64_t cap =
                 std::min<uint64_t>(products, (uint64_t)r->prod.m * (uint64_t)r->prod.k);

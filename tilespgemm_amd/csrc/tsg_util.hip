@@ -193,73 +193,111 @@ StageTimer::~StageTimer() { clear(); }
 // flags: bit 0 bad row pointers, bit 1 column out of range, bit 2 a row not
 // strictly ascending (unsorted or duplicate columns).  Reads stay inside
 // [0, nnz) whatever the row pointers hold.
+// The kernels below run between kernels that stream gigabytes, so their
+// operands are cold in every cache: each keeps its chain of dependent loads
+// to two or three (no searches, no per-row loops), which is what sets their
+// duration.
+
+// Thread t checks row pointer t (t < m) and entry t (t < nnz).  Rows are
+// sorted and duplicate-free iff every descent ja[p-1] >= ja[p] (p > 0) sits at
+// the first entry of a non-empty row, i.e. iff the number of all descents
+// equals the number of descents at row starts: cnt[0] counts the former,
+// cnt[1] the latter.  Every access stays inside ia[0..m] and ja[0..nnz)
+// whatever the contents.
 __global__ void __launch_bounds__(256) k_validate(int m, int n, const int *__restrict__ ia,
                                                   const int *__restrict__ ja, int64_t nnz,
-                                                  int *__restrict__ flags) {
-    const int i = (blockIdx.x * blockDim.x + threadIdx.x) >> 3;
-    const int l = threadIdx.x & 7;
-    if (i >= m) return;
+                                                  int *__restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int f = 0;
-    const int64_t lo = ia[i], hi = ia[i + 1];
-    if (lo > hi || lo < 0 || hi > nnz) f |= 1;
-    if (i == 0 && lo != 0) f |= 1;
-    if (i == m - 1 && hi != nnz) f |= 1;
-    const int64_t a = lo < 0 ? 0 : (lo > nnz ? nnz : lo);
-    const int64_t b = hi < a ? a : (hi > nnz ? nnz : hi);
-    for (int64_t p = a + l; p < b; p += 8) {
-        const int c = ja[p];
-        if (c < 0 || c >= n) f |= 2;
-        if (p > a && ja[p - 1] >= c) f |= 4;
+    int at_start = 0, descent = 0;
+    if (t < m) {
+        const int64_t lo = ia[t], hi = ia[t + 1];
+        if (lo > hi || lo < 0 || hi > nnz) f |= 1;
+        if (t == 0 && lo != 0) f |= 1;
+        if (t == m - 1 && hi != nnz) f |= 1;
+        if (lo > 0 && lo < hi && hi <= nnz) at_start = ja[lo - 1] >= ja[lo];
     }
-    if (f) atomicOr(flags, f);
+    if (t < nnz) {
+        const int c = ja[t];
+        if (c < 0 || c >= n) f |= 2;
+        if (t > 0) descent = ja[t - 1] >= c;
+    }
+    const uint64_t md = __ballot(descent), ms = __ballot(at_start);
+    if ((threadIdx.x & 63) == 0) {
+        if (md) atomicAdd(&out[1], (int)__popcll(md));
+        if (ms) atomicAdd(&out[2], (int)__popcll(ms));
+    }
+    if (f) atomicOr(&out[0], f);
 }
 
 int validate_csr_pair(const CsrIn &A, const CsrIn &B, Arena &ws, hipStream_t s) {
-    int *d = ws.alloc_n<int>(2);
-    TSG_HIP_TRY(hipMemsetAsync(d, 0, 2 * sizeof(int), s));
+    int *d = ws.alloc_n<int>(6);
+    TSG_HIP_TRY(hipMemsetAsync(d, 0, 6 * sizeof(int), s));
+    auto threads = [](const CsrIn &X) { return std::max((size_t)X.rows, (size_t)X.nnz); };
     if (A.rows > 0) {
-        hipLaunchKernelGGL(k_validate, dim3(grid_for((size_t)A.rows * 8, 256)), dim3(256), 0, s,
-                           A.rows, A.cols, A.ptr, A.idx, A.nnz, d);
+        hipLaunchKernelGGL(k_validate, dim3(grid_for(threads(A), 256)), dim3(256), 0, s, A.rows,
+                           A.cols, A.ptr, A.idx, A.nnz, d);
         TSG_LAUNCH_CHECK();
     }
     if (B.rows > 0 && (B.ptr != A.ptr || B.idx != A.idx || B.cols != A.cols)) {
-        hipLaunchKernelGGL(k_validate, dim3(grid_for((size_t)B.rows * 8, 256)), dim3(256), 0, s,
-                           B.rows, B.cols, B.ptr, B.idx, B.nnz, d + 1);
+        hipLaunchKernelGGL(k_validate, dim3(grid_for(threads(B), 256)), dim3(256), 0, s, B.rows,
+                           B.cols, B.ptr, B.idx, B.nnz, d + 3);
         TSG_LAUNCH_CHECK();
     }
-    int h[2] = {0, 0};
+    int h[6] = {0, 0, 0, 0, 0, 0};
     TSG_HIP_TRY(hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, s));
     TSG_HIP_TRY(hipStreamSynchronize(s));
-    if (A.rows == 0 && A.nnz != 0) h[0] |= 1;
-    if (B.rows == 0 && B.nnz != 0) h[1] |= 1;
-    return h[0] | (h[1] << 8);
+    int fa = h[0], fb = h[3];
+    if (h[1] != h[2]) fa |= 4;
+    if (h[4] != h[5]) fb |= 4;
+    if (A.rows == 0 && A.nnz != 0) fa |= 1;
+    if (B.rows == 0 && B.nnz != 0) fb |= 1;
+    return fa | (fb << 8);
 }
 
 // ======================================================= row products
 // rowprod[i] = sum over A's entries a_ik in row i of |row k of B|
-// (main_spgemm-col.cu:4950-4955).  8 lanes per row.
-__global__ void __launch_bounds__(256) k_row_products(int m, const int *__restrict__ ia,
-                                                      const int *__restrict__ ja,
-                                                      const int *__restrict__ ib,
-                                                      int64_t *__restrict__ rowprod) {
-    const int g = (blockIdx.x * blockDim.x + threadIdx.x) >> 3;
-    const int l = threadIdx.x & 7;
-    int64_t s = 0;
-    if (g < m) {
-        for (int p = ia[g] + l; p < ia[g + 1]; p += 8) {
-            int t = ja[p];
-            s += ib[t + 1] - ib[t];
-        }
-    }
-    s = group_sum<8>(s);
-    if (g < m && l == 0) rowprod[g] = s;
+// (main_spgemm-col.cu:4950-4955), as P[ia[i+1]] - P[ia[i]] with P the
+// exclusive scan of the per-entry counts |row ja[p] of B|.  A must be a valid
+// CSR with columns indexing rows of B.
+__global__ void __launch_bounds__(256) k_entry_products(int64_t nnz, const int *__restrict__ ja,
+                                                        const int *__restrict__ ib,
+                                                        int64_t *__restrict__ ep) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nnz) return;
+    const int k = ja[p];
+    ep[p] = ib[k + 1] - ib[k];
 }
 
-void csr_row_products(int m, const int *ia, const int *ja, const int *ib, int64_t *rowprod,
-                      hipStream_t s) {
+__global__ void __launch_bounds__(256) k_row_diff(int m, const int *__restrict__ ia,
+                                                  const int64_t *__restrict__ P,
+                                                  int64_t *__restrict__ rowprod) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) rowprod[i] = P[ia[i + 1]] - P[ia[i]];
+}
+
+__global__ void k_zero_i64(int64_t n, int64_t *__restrict__ v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = 0;
+}
+
+void csr_row_products(int m, const int *ia, const int *ja, int64_t nnz, const int *ib,
+                      int64_t *rowprod, Arena &ws, hipStream_t s) {
     if (m <= 0) return;
-    hipLaunchKernelGGL(k_row_products, dim3(grid_for((size_t)m * 8, 256)), dim3(256), 0, s, m,
-                       ia, ja, ib, rowprod);
+    if (nnz <= 0) {
+        hipLaunchKernelGGL(k_zero_i64, dim3(grid_for((size_t)m, 256)), dim3(256), 0, s,
+                           (int64_t)m, rowprod);
+        TSG_LAUNCH_CHECK();
+        return;
+    }
+    int64_t *P = ws.alloc_n<int64_t>((size_t)nnz + 1);
+    int64_t *tmp = ws.alloc_n<int64_t>(scan_tmp_elems((size_t)nnz + 1));
+    hipLaunchKernelGGL(k_entry_products, dim3(grid_for((size_t)nnz, 256)), dim3(256), 0, s, nnz,
+                       ja, ib, P);
+    TSG_LAUNCH_CHECK();
+    exclusive_scan_i64(P, P, (size_t)nnz, tmp, s);
+    hipLaunchKernelGGL(k_row_diff, dim3(grid_for((size_t)m, 256)), dim3(256), 0, s, m, ia, P,
+                       rowprod);
     TSG_LAUNCH_CHECK();
 }
 
