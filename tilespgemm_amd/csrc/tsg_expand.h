@@ -30,6 +30,11 @@ __device__ __forceinline__ int brange_lo(const int *__restrict__ bptr, int strid
 // Batch striding (bofs, bstep): only batches bofs, bofs + bstep, ... of
 // every group are visited, so several waves can share one group's products
 // (used where the order of products does not matter).
+// s_mark: the group's G ints of LDS, all -1 on entry and on return.  A
+// batch finds the A entry of each of its products there: every non-empty
+// segment (the products of one A entry) that overlaps the batch's window of
+// G products writes its lane at its first position in the window, and a
+// running maximum over the window gives each position its segment.
 template <int G, int U, bool RANGED, bool VALUES, class F>
 __device__ __forceinline__ void for_products(int pa, int pe, int gofs, int gstride,
                                              const int *__restrict__ ja,
@@ -38,8 +43,9 @@ __device__ __forceinline__ void for_products(int pa, int pe, int gofs, int gstri
                                              const int *__restrict__ jb,
                                              const double *__restrict__ bval,
                                              const int *__restrict__ bptr, int bstride, int bs,
-                                             int c0, int c1, F &&f, int bofs = 0,
+                                             int c0, int c1, int *s_mark, F &&f, int bofs = 0,
                                              int bstep = 1) {
+    static_assert(G >= 16, "segment lookup scans 16, 32 or 64 lanes");
     const int l = __lane_id() & (G - 1);
     const int gbase = __lane_id() & ~(G - 1);
     const unsigned long long gmask = G == 64 ? ~0ull : ((1ull << G) - 1);
@@ -83,9 +89,10 @@ __device__ __forceinline__ void for_products(int pa, int pe, int gofs, int gstri
         load_t(g + 2 * gstride, t2, a2);            // group after next
         const int cnt = hi - lo;
         const int incl = group_incl_scan<G>(cnt);
-        const int total = __shfl(incl, G - 1, G);
+        const int total = group_last<G>(incl);
+        const int start = incl - cnt;
         // position in B of product q of source lane s: off_s + q
-        const int off = lo - (incl - cnt);
+        const int off = lo - start;
         for (int base = bofs * G; base < total; base += G * U * bstep) {
             int e[U], sl[U];
             bool act[U];
@@ -100,14 +107,16 @@ __device__ __forceinline__ void for_products(int pa, int pe, int gofs, int gstri
                 e[u] = 0;
                 sl[u] = 0;
                 if (u < nb) {
-                    const int q = base + u * G * bstep + l;
+                    const int qb = base + u * G * bstep;
+                    const int q = qb + l;
                     act[u] = q < total;
-                    int s = 0;
-#pragma unroll
-                    for (int step = G / 2; step > 0; step >>= 1) {
-                        int probe = __shfl(incl, s + step - 1, G);
-                        if (probe <= q) s += step;
-                    }
+                    wave_lds_fence();
+                    if (cnt > 0 && start < qb + G && incl > qb)
+                        s_mark[start > qb ? start - qb : 0] = l;
+                    wave_lds_fence();
+                    const int v = s_mark[l];
+                    s_mark[l] = -1;
+                    const int s = dpp_incl_max<G>(v);
                     as[u] = VALUES ? __shfl(a, s, G) : 0.0;
                     e[u] = __shfl(off, s, G) + q;
                     sl[u] = s;
@@ -123,7 +132,7 @@ __device__ __forceinline__ void for_products(int pa, int pe, int gofs, int gstri
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 if (base + u * G * bstep < total) {
-                    const int s0 = __shfl(sl[u], 0, G);
+                    const int s0 = group_first<G>(sl[u]);
                     const unsigned long long diff = __ballot(act[u] && sl[u] != s0);
                     const bool single = ((diff >> gbase) & gmask) == 0;
                     f(act[u], col[u], as[u], bv[u], single);
@@ -165,7 +174,7 @@ __device__ __forceinline__ void mark_products(int pa, int pe, const int *__restr
         }
         const int cnt = hi - lo;
         const int incl = wave_incl_scan(cnt);
-        const int total = __shfl(incl, 63, 64);
+        const int total = group_last<64>(incl);
         const int off = lo - (incl - cnt);
         wave_lds_fence();
         s_seg[l] = incl;

@@ -116,17 +116,19 @@ __global__ void __launch_bounds__(256) k_small_symbolic(int nrows, const int *__
                                                         int64_t *__restrict__ rownnz) {
     constexpr int RPB = 256 / G;
     __shared__ int s_keys[RPB * SLOTS];
+    __shared__ int s_mark[256];
     const int grp = threadIdx.x / G;
     const int l = threadIdx.x & (G - 1);
     int *keys = s_keys + grp * SLOTS;
     const int r = blockIdx.x * RPB + grp;
     for (int i = l; i < SLOTS; i += G) keys[i] = -1;
+    s_mark[threadIdx.x] = -1;
     wave_lds_fence();
     if (r >= nrows) return;
     const int row = rows[r];
     int created_cnt = 0;
     for_products<G, SMALL_U, false, false>(ia[row], ia[row + 1], 0, G, ja, nullptr, ib, jb,
-                                           nullptr, nullptr, 0, 0, 0, 0,
+                                           nullptr, nullptr, 0, 0, 0, 0, s_mark + grp * G,
                                            [&](bool act, int col, double, double, bool) {
                                       if (act) {
                                           bool created;
@@ -139,8 +141,20 @@ __global__ void __launch_bounds__(256) k_small_symbolic(int nrows, const int *__
 }
 
 // ------------------------------------------------------- small rows: numeric
-template <int G, int SLOTS, bool FUSED>
-__global__ void __launch_bounds__(256) k_small_numeric(int nrows, const int *__restrict__ rows,
+// RPB rows per block of RPB*G threads; LDS per row: SLOTS keys and values,
+// then the tags of the ordered accumulation, reused as the 256-entry radix
+// histogram once accumulation is over (rows of <= 64 products never need
+// the radix sort: at most 64 outputs go to the bitonic sort).
+template <int G, int SLOTS>
+constexpr int small_tag_words() {
+    return SLOTS > 128 ? (4 * G > 256 ? 4 * G : 256) : 4 * G;
+}
+template <int G, int SLOTS>
+constexpr size_t small_numeric_row_bytes() {
+    return (size_t)SLOTS * 12 + (size_t)small_tag_words<G, SLOTS>() * 4;
+}
+template <int G, int SLOTS, bool FUSED, int RPB>
+__global__ void __launch_bounds__(RPB * G) k_small_numeric(int nrows, const int *__restrict__ rows,
                                                        const int *__restrict__ ia,
                                                        const int *__restrict__ ja,
                                                        const double *__restrict__ a,
@@ -150,27 +164,28 @@ __global__ void __launch_bounds__(256) k_small_numeric(int nrows, const int *__r
                                                        const int64_t *__restrict__ ic,
                                                        int *__restrict__ jc,
                                                        double *__restrict__ c, int b_strict) {
-    constexpr int RPB = 256 / G;
     constexpr int NT = 4 * G;
+    constexpr int TW = small_tag_words<G, SLOTS>();
     extern __shared__ __align__(16) unsigned char smem[];
     double *s_vals = reinterpret_cast<double *>(smem);
     int *s_keys = reinterpret_cast<int *>(s_vals + RPB * SLOTS);
     unsigned *s_tags = reinterpret_cast<unsigned *>(s_keys + RPB * SLOTS);
-    unsigned *s_hist = s_tags + RPB * NT;
     const int grp = threadIdx.x / G;
     const int l = threadIdx.x & (G - 1);
     int *keys = s_keys + grp * SLOTS;
     double *vals = s_vals + grp * SLOTS;
-    unsigned *tags = s_tags + grp * NT;
-    unsigned *hist = s_hist + grp * 256;
+    unsigned *tags = s_tags + grp * TW;
+    unsigned *hist = tags;      // after the accumulation
     const int r = blockIdx.x * RPB + grp;
+    __shared__ int s_mark[RPB * G];
     for (int i = l; i < SLOTS; i += G) { keys[i] = -1; vals[i] = 0.0; }
     for (int i = l; i < NT; i += G) tags[i] = 0xffffffffu;
+    s_mark[threadIdx.x] = -1;
     wave_lds_fence();
     if (r >= nrows) return;
     const int row = rows[r];
     for_products<G, SMALL_U, false, true>(ia[row], ia[row + 1], 0, G, ja, a, ib, jb, b, nullptr,
-                                          0, 0, 0, 0,
+                                          0, 0, 0, 0, s_mark + grp * G,
                                           [&](bool act, int col, double av, double bv, bool single) {
                                      int slot = 0;
                                      if (act) {
@@ -217,25 +232,27 @@ __global__ void __launch_bounds__(256) k_small_numeric(int nrows, const int *__r
     }
     // radix sort on keys relative to the row's smallest column; the back
     // half of the table (occupancy <= SLOTS/2) is the ping-pong buffer
-    int kmin = 0x7fffffff, kmax = 0;
-    for (int i = l; i < nocc; i += G) {
-        kmin = min(kmin, keys[i]);
-        kmax = max(kmax, keys[i]);
-    }
+    if constexpr (SLOTS > 128) {
+        int kmin = 0x7fffffff, kmax = 0;
+        for (int i = l; i < nocc; i += G) {
+            kmin = min(kmin, keys[i]);
+            kmax = max(kmax, keys[i]);
+        }
 #pragma unroll
-    for (int d = G / 2; d > 0; d >>= 1) {
-        kmin = min(kmin, __shfl_xor(kmin, d, G));
-        kmax = max(kmax, __shfl_xor(kmax, d, G));
-    }
-    const int span = kmax - kmin;
-    const int bits = span > 0 ? 32 - __clz(span) : 1;
-    const bool in_dst = radix_sort_group<G>(keys, vals, keys + SLOTS / 2, vals + SLOTS / 2, nocc,
-                                            kmin, bits, hist);
-    const int *sk = in_dst ? keys + SLOTS / 2 : keys;
-    const double *sv = in_dst ? vals + SLOTS / 2 : vals;
-    for (int i = l; i < nnz; i += G) {
-        jc[o + i] = sk[i];
-        c[o + i] = sv[i];
+        for (int d = G / 2; d > 0; d >>= 1) {
+            kmin = min(kmin, __shfl_xor(kmin, d, G));
+            kmax = max(kmax, __shfl_xor(kmax, d, G));
+        }
+        const int span = kmax - kmin;
+        const int bits = span > 0 ? 32 - __clz(span) : 1;
+        const bool in_dst = radix_sort_group<G>(keys, vals, keys + SLOTS / 2, vals + SLOTS / 2,
+                                                nocc, kmin, bits, hist);
+        const int *sk = in_dst ? keys + SLOTS / 2 : keys;
+        const double *sv = in_dst ? vals + SLOTS / 2 : vals;
+        for (int i = l; i < nnz; i += G) {
+            jc[o + i] = sk[i];
+            c[o + i] = sv[i];
+        }
     }
 }
 
@@ -426,6 +443,7 @@ struct ItemSmem {
     unsigned short ids[ITEM_SLOTS];
     unsigned bits[ITEM_WORDS];
     unsigned tags[ITEM_TAGS];
+    int mark[64];                 // segment marks of the product expansion
     int nid;
 };
 
@@ -440,6 +458,7 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric(
     ItemSmem &S = reinterpret_cast<ItemSmem *>(smem)[threadIdx.x >> 6];
     const int l = __lane_id();
     for (int i = l; i < ITEM_TAGS; i += 64) S.tags[i] = 0xffffffffu;
+    S.mark[l] = -1;
     const int64_t wave_id = (int64_t)blockIdx.x * ITEM_WAVES + (threadIdx.x >> 6);
     const int64_t nwaves = (int64_t)gridDim.x * ITEM_WAVES;
     for (int64_t it = wave_id; it < nitems; it += nwaves) {
@@ -451,7 +470,7 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric(
         if (l == 0) S.nid = 0;
         wave_lds_fence();
         for_products<64, ITEM_U, true, true>(
-            ia[row], ia[row + 1], 0, 64, ja, a, ib, jb, b, bptr, bstride, bs, c0, c1,
+            ia[row], ia[row + 1], 0, 64, ja, a, ib, jb, b, bptr, bstride, bs, c0, c1, S.mark,
             [&](bool act, int col, double av, double bv, bool single) {
                 int slot = -1;
                 if (act) {
@@ -503,6 +522,7 @@ struct RankSmem {
     uint2 words[ITEM_WORDS];      // {bitmap word, outputs before it}
     double vals[ITEM_CAP];
     unsigned tags[ITEM_TAGS];     // bucketed by rank
+    int mark[64];                 // segment marks of the product expansion
 };
 
 template <bool FUSED>
@@ -517,6 +537,7 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric_rank(
     RankSmem &S = reinterpret_cast<RankSmem *>(smem)[threadIdx.x >> 6];
     const int l = __lane_id();
     for (int i = l; i < ITEM_TAGS; i += 64) S.tags[i] = 0xffffffffu;
+    S.mark[l] = -1;
     const int64_t wave_id = (int64_t)blockIdx.x * ITEM_WAVES + (threadIdx.x >> 6);
     const int64_t nwaves = (int64_t)gridDim.x * ITEM_WAVES;
     constexpr int WCH = ITEM_WORDS / 64;
@@ -543,7 +564,7 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric_rank(
             const int lastbits = (c1 - c0) - (w << 5);
             if (w < nwords && lastbits < 32) wv[j] &= (1u << lastbits) - 1u;
             incl[j] = wave_incl_scan(__popc(wv[j]));
-            tot[j] = __shfl(incl[j], 63, 64);
+            tot[j] = group_last<64>(incl[j]);
         }
         int cnt = 0;
 #pragma unroll
@@ -556,7 +577,7 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric_rank(
         for (int i = l; i < cnt; i += 64) S.vals[i] = 0.0;
         wave_lds_fence();
         for_products<64, RANK_U, true, true>(
-            ia[row], ia[row + 1], 0, 64, ja, a, ib, jb, b, bptr, bstride, bs, c0, c1,
+            ia[row], ia[row + 1], 0, 64, ja, a, ib, jb, b, bptr, bstride, bs, c0, c1, S.mark,
             [&](bool act, int col, double av, double bv, bool single) {
                 int rank = -1;
                 if (act) {
@@ -780,36 +801,20 @@ static void row_numeric_t(RowPlan &p, const CsrIn &A, const CsrIn &B, const int6
     const int *st = p.bin_start_h;
     tm.begin("row.small_numeric");
     {
-        int n1 = st[2] - st[1];
-        if (n1 > 0) {
-            constexpr size_t sm = 16 * 128 * 12 + 16 * 64 * 4 + 16 * 256 * 4;
-            auto kern = k_small_numeric<16, 128, FUSED>;
+        // rows of <= 64, <= 256 and <= 1024 products: 16, 2 and 1 rows per block
+        auto launch = [&](auto kern, int rpb, int g, size_t per_row, int b) {
+            const int nr = st[b + 1] - st[b];
+            if (nr <= 0) return;
+            const size_t sm = per_row * rpb;
             set_smem(kern, sm);
-            hipLaunchKernelGGL(kern, dim3(grid_for(n1, 16)), dim3(256), sm, s, n1, bins + st[1],
-                               A.ptr, A.idx, A.val, B.ptr, B.idx, B.val, d_ic, d_jc, d_c,
-                               (int)p.b_strict);
+            hipLaunchKernelGGL(kern, dim3(grid_for(nr, rpb)), dim3(rpb * g), sm, s, nr,
+                               bins + st[b], A.ptr, A.idx, A.val, B.ptr, B.idx, B.val, d_ic,
+                               d_jc, d_c, (int)p.b_strict);
             TSG_LAUNCH_CHECK();
-        }
-        int n2 = st[3] - st[2];
-        if (n2 > 0) {
-            constexpr size_t sm = 8 * 512 * 12 + 8 * 128 * 4 + 8 * 256 * 4;
-            auto kern = k_small_numeric<32, 512, FUSED>;
-            set_smem(kern, sm);
-            hipLaunchKernelGGL(kern, dim3(grid_for(n2, 8)), dim3(256), sm, s, n2, bins + st[2],
-                               A.ptr, A.idx, A.val, B.ptr, B.idx, B.val, d_ic, d_jc, d_c,
-                               (int)p.b_strict);
-            TSG_LAUNCH_CHECK();
-        }
-        int n3 = st[4] - st[3];
-        if (n3 > 0) {
-            constexpr size_t sm = 4 * 2048 * 12 + 4 * 256 * 4 + 4 * 256 * 4;
-            auto kern = k_small_numeric<64, 2048, FUSED>;
-            set_smem(kern, sm);
-            hipLaunchKernelGGL(kern, dim3(grid_for(n3, 4)), dim3(256), sm, s, n3, bins + st[3],
-                               A.ptr, A.idx, A.val, B.ptr, B.idx, B.val, d_ic, d_jc, d_c,
-                               (int)p.b_strict);
-            TSG_LAUNCH_CHECK();
-        }
+        };
+        launch(k_small_numeric<16, 128, FUSED, 16>, 16, 16, small_numeric_row_bytes<16, 128>(), 1);
+        launch(k_small_numeric<32, 512, FUSED, 2>, 2, 32, small_numeric_row_bytes<32, 512>(), 2);
+        launch(k_small_numeric<64, 2048, FUSED, 1>, 1, 64, small_numeric_row_bytes<64, 2048>(), 3);
     }
     tm.end();
     if (p.nitems > 0 && p.use_rank) {

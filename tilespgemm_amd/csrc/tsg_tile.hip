@@ -158,9 +158,11 @@ __global__ void __launch_bounds__(64 * S1_WAVES) k_step1(
     const int *__restrict__ tptrB, const int *__restrict__ tcolB, int *__restrict__ cnt,
     const int *__restrict__ tptrC, int *__restrict__ tcolC, int *__restrict__ trowC) {
     __shared__ unsigned s_bits[S1_WAVES][S1_WORDS];
+    __shared__ int s_mark[S1_WAVES][64];
     unsigned *bits = s_bits[threadIdx.x >> 6];
     const int I = blockIdx.x * S1_WAVES + (threadIdx.x >> 6);
     const int l = __lane_id();
+    s_mark[threadIdx.x >> 6][l] = -1;
     if (I >= blkm) return;
     const int pa = tptrA[I], pe = tptrA[I + 1];
     int lo = INT_MAX, hi = -1;
@@ -182,7 +184,7 @@ __global__ void __launch_bounds__(64 * S1_WAVES) k_step1(
         for (int i = l; i < nwords; i += 64) bits[i] = 0u;
         wave_lds_fence();
         for_products<64, 4, false, false>(pa, pe, 0, 64, tcolA, nullptr, tptrB, tcolB, nullptr,
-                                          nullptr, 0, 0, 0, 0,
+                                          nullptr, 0, 0, 0, 0, s_mark[threadIdx.x >> 6],
                                           [&](bool act, int J, double, double, bool) {
                                            if (act && J >= w0 && J < w1) {
                                                const int c = J - w0;
