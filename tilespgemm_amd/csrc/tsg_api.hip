@@ -8,7 +8,9 @@
 #include <memory>
 #include <mutex>
 
-int tile_count_only(const CsrIn &X, Arena &ws, hipStream_t s);   // tsg_tile.hip
+// tsg_tile.hip: entries per 16x16 tile of X (a sample decides when clear of
+// the threshold)
+double tile_density(const CsrIn &X, double threshold, Arena &ws, hipStream_t s);
 
 
 namespace {
@@ -73,8 +75,7 @@ int choose_method(tsg_handle_t h, const CsrIn &A, bool sorted) {
     if (A.nnz == 0 || A.rows == 0) return TSG_METHOD_ROW;
     // Tile path when A's 16x16 tiles hold enough entries to amortize the
     // per-tile metadata (~60 bytes per tile), row path otherwise.
-    const int nt = tile_count_only(A, h->ws, h->stream);
-    const double per_tile = nt > 0 ? (double)A.nnz / nt : 0.0;
+    const double per_tile = tile_density(A, 6.0, h->ws, h->stream);
     return per_tile >= 6.0 ? TSG_METHOD_TILE : TSG_METHOD_ROW;
 }
 

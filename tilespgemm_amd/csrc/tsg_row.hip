@@ -382,54 +382,92 @@ __global__ void __launch_bounds__(256) k_bptr(int n, const int *__restrict__ ib,
 }
 
 // ------------------------------------------------------------- item plan
-// Greedy split of each big row into items: runs of sub-blocks with at most
-// ITEM_CAP outputs and at most ITEM_WBITS columns; leading empty sub-blocks
-// are skipped.  FILL=false counts, FILL=true writes {row, c0, c1, big-row
-// index}.
+// Split of each big row into items: the row's span is cut into windows of
+// ITEM_WBITS columns (32 sub-blocks); inside a window, runs of sub-blocks
+// with at most ITEM_CAP outputs, leading empty sub-blocks skipped.  One wave
+// per big row, one lane per window (64 windows per pass): a lane loads its
+// window's 32 counts at once and splits them in registers, and wave scans
+// place the items and their outputs.  FILL=false counts items per row;
+// FILL=true (item_cnt = exclusive item offsets) writes {row, c0, c1,
+// big-row index} and each item's first output position.
 template <bool FILL>
-__global__ void k_items(int nbig, const int *__restrict__ rows, int k,
-                        const int *__restrict__ span_sb0, const int64_t *__restrict__ sb_off,
-                        const uint16_t *__restrict__ subcnt, const int64_t *__restrict__ ic,
-                        int64_t *__restrict__ item_cnt, int4 *__restrict__ items,
-                        int64_t *__restrict__ item_off) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_items(int nbig, const int *__restrict__ rows, int k,
+                                               const int *__restrict__ span_sb0,
+                                               const int64_t *__restrict__ sb_off,
+                                               const uint16_t *__restrict__ subcnt,
+                                               const int64_t *__restrict__ ic,
+                                               int64_t *__restrict__ item_cnt,
+                                               int4 *__restrict__ items,
+                                               int64_t *__restrict__ item_off) {
+    constexpr int WSB = 1 << (ITEM_WBITS_SHIFT - SUB_SHIFT);
+    const int r = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int l = threadIdx.x & 63;
     if (r >= nbig) return;
     const int row = rows[r];
     const uint16_t *sc = subcnt + sb_off[r];
     const int nsb = (int)(sb_off[r + 1] - sb_off[r]);
     const int first = span_sb0[r];
-    constexpr int WMAX_SB = 1 << (ITEM_WBITS_SHIFT - SUB_SHIFT);
-    int64_t nitem = 0;
-    int64_t out = FILL ? ic[row] : 0;
-    int64_t w = FILL ? item_cnt[r] : 0;
-    int start = -1, cnt = 0;
-    for (int j = 0; j < nsb; j++) {
-        const int cs = sc[j];
-        if (start >= 0 && (cnt + cs > ITEM_CAP || j - start >= WMAX_SB)) {
-            if (FILL) {
-                items[w] = make_int4(row, (first + start) << SUB_SHIFT,
-                                     min(k, (first + j) << SUB_SHIFT), r);
-                item_off[w] = out;
-                w++;
-                out += cnt;
+    const int nwin = (nsb + WSB - 1) / WSB;
+    int64_t wbase = FILL ? item_cnt[r] : 0;
+    int64_t obase = FILL ? ic[row] : 0;
+    for (int w0 = 0; w0 < nwin; w0 += 64) {
+        const int win = w0 + l;
+        const int j0 = win * WSB;
+        const int len = win < nwin ? min(WSB, nsb - j0) : 0;
+        int cs[WSB];
+#pragma unroll
+        for (int q = 0; q < WSB; q++) cs[q] = q < len ? (int)sc[j0 + q] : 0;
+        int nit = 0, tot = 0;
+        {
+            int start = -1, cnt = 0;
+#pragma unroll
+            for (int q = 0; q < WSB; q++) {
+                if (start >= 0 && cnt + cs[q] > ITEM_CAP) {
+                    nit++;
+                    start = -1;
+                    cnt = 0;
+                }
+                if (cs[q] != 0 || start >= 0) {
+                    if (start < 0) start = q;
+                    cnt += cs[q];
+                }
+                tot += cs[q];
             }
-            nitem++;
-            start = -1;
-            cnt = 0;
+            if (start >= 0 && cnt > 0) nit++;
         }
-        if (cs == 0 && start < 0) continue;
-        if (start < 0) start = j;
-        cnt += cs;
-    }
-    if (start >= 0 && cnt > 0) {
-        if (FILL) {
-            items[w] = make_int4(row, (first + start) << SUB_SHIFT,
-                                 min(k, (first + nsb) << SUB_SHIFT), r);
-            item_off[w] = out;
+        const int it_incl = wave_incl_scan(nit);
+        const int out_incl = wave_incl_scan(tot);
+        if (FILL && nit > 0) {
+            int64_t w = wbase + it_incl - nit;
+            int64_t out = obase + out_incl - tot;
+            const int cbase = first + j0;
+            int start = -1, cnt = 0;
+#pragma unroll
+            for (int q = 0; q < WSB; q++) {
+                if (start >= 0 && cnt + cs[q] > ITEM_CAP) {
+                    items[w] = make_int4(row, (cbase + start) << SUB_SHIFT,
+                                         min(k, (cbase + q) << SUB_SHIFT), r);
+                    item_off[w] = out;
+                    w++;
+                    out += cnt;
+                    start = -1;
+                    cnt = 0;
+                }
+                if (cs[q] != 0 || start >= 0) {
+                    if (start < 0) start = q;
+                    cnt += cs[q];
+                }
+            }
+            if (start >= 0 && cnt > 0) {
+                items[w] = make_int4(row, (cbase + start) << SUB_SHIFT,
+                                     min(k, (cbase + len) << SUB_SHIFT), r);
+                item_off[w] = out;
+            }
         }
-        nitem++;
+        wbase += group_last<64>(it_incl);
+        obase += group_last<64>(out_incl);
     }
-    if (!FILL) item_cnt[r] = nitem;
+    if (!FILL && l == 0) item_cnt[r] = wbase;
 }
 
 // --------------------------------------------------------- big rows: numeric
@@ -769,7 +807,7 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
         const int *rowsb = p.bin_rows + st.s[4];
         p.item_cnt = ws.alloc_n<int64_t>((size_t)p.nbig + 1);
         int64_t *tmp2 = ws.alloc_n<int64_t>(scan_tmp_elems((size_t)p.nbig + 1));
-        hipLaunchKernelGGL(k_items<false>, dim3(grid_for(p.nbig, 128)), dim3(128), 0, s, p.nbig,
+        hipLaunchKernelGGL(k_items<false>, dim3(grid_for((size_t)p.nbig * 64, 256)), dim3(256), 0, s, p.nbig,
                            rowsb, k, p.span_sb0, p.sb_off, p.subcnt, d_ic, p.item_cnt, nullptr,
                            nullptr);
         TSG_LAUNCH_CHECK();
@@ -781,7 +819,7 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
         p.nitems = h_tot[1];
         p.items = ws.alloc_n<int4>((size_t)p.nitems + 1);
         p.item_off = ws.alloc_n<int64_t>((size_t)p.nitems + 1);
-        hipLaunchKernelGGL(k_items<true>, dim3(grid_for(p.nbig, 128)), dim3(128), 0, s, p.nbig,
+        hipLaunchKernelGGL(k_items<true>, dim3(grid_for((size_t)p.nbig * 64, 256)), dim3(256), 0, s, p.nbig,
                            rowsb, k, p.span_sb0, p.sb_off, p.subcnt, d_ic, p.item_cnt, p.items,
                            p.item_off);
         TSG_LAUNCH_CHECK();

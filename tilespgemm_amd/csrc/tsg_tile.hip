@@ -106,15 +106,20 @@ __global__ void __launch_bounds__(256) k_tile_convert(
 constexpr int TC_WAVES = 4;
 constexpr int TC_WORDS = 2048;
 
-__global__ void __launch_bounds__(64 * TC_WAVES) k_tile_count(int m, int blkm,
+// With step > 1 only tile rows 0, step, 2*step, ... are counted (wave i takes
+// tile row i*step) and, when sums is given, their tile and entry counts are
+// added to sums[0] and sums[1].
+__global__ void __launch_bounds__(64 * TC_WAVES) k_tile_count(int m, int blkm, int step,
                                                               const int *__restrict__ ia,
                                                               const int *__restrict__ ja,
-                                                              int *__restrict__ tcount) {
+                                                              int *__restrict__ tcount,
+                                                              unsigned long long *__restrict__ sums) {
     __shared__ unsigned s_bits[TC_WAVES][TC_WORDS];
     unsigned *bits = s_bits[threadIdx.x >> 6];
-    const int I = blockIdx.x * TC_WAVES + (threadIdx.x >> 6);
+    const int64_t Iw = (int64_t)(blockIdx.x * TC_WAVES + (threadIdx.x >> 6)) * step;
     const int l = __lane_id();
-    if (I >= blkm) return;
+    if (Iw >= blkm) return;
+    const int I = (int)Iw;
     const int p0 = ia[I * 16], p1 = ia[min(m, I * 16 + 16)];
     int lo = INT_MAX, hi = -1;
     for (int p = p0 + l; p < p1; p += 64) {
@@ -139,7 +144,13 @@ __global__ void __launch_bounds__(64 * TC_WAVES) k_tile_count(int m, int blkm,
         wave_lds_fence();
     }
     count = wave_sum(count);
-    if (l == 0) tcount[I] = count;
+    if (l == 0) {
+        if (tcount) tcount[I] = count;
+        if (sums) {
+            atomicAdd(&sums[0], (unsigned long long)count);
+            atomicAdd(&sums[1], (unsigned long long)(p1 - p0));
+        }
+    }
 }
 
 __global__ void k_perm_scatter(int64_t nnz, const int *__restrict__ perm,
@@ -386,7 +397,7 @@ void convert(TileMat &T, const CsrIn &X, bool fill_values, Arena &ws, hipStream_
     T.tile_ptr = ws.alloc_n<int>((size_t)T.blkm + 1);
     if (T.blkm > 0) {
         hipLaunchKernelGGL(k_tile_count, dim3(grid_for(T.blkm, TC_WAVES)), dim3(64 * TC_WAVES), 0,
-                           s, X.rows, T.blkm, X.ptr, X.idx, tcount);
+                           s, X.rows, T.blkm, 1, X.ptr, X.idx, tcount, nullptr);
         TSG_LAUNCH_CHECK();
     }
     exclusive_scan_i32(tcount, T.tile_ptr, (size_t)T.blkm, tmp, s);
@@ -540,18 +551,25 @@ void tile_numeric(TilePlan &p, const CsrIn &A, const CsrIn &B, const int64_t *d_
 
 // Number of non-empty 16x16 tiles of X (the tile-count pass of the
 // conversion); used to choose between the tile path and the row path.
-int tile_count_only(const CsrIn &X, Arena &ws, hipStream_t s) {
+double tile_density(const CsrIn &X, double threshold, Arena &ws, hipStream_t s) {
     const int blkm = (X.rows + 15) / 16;
-    if (blkm == 0) return 0;
-    int *tcount = ws.alloc_n<int>((size_t)blkm + 1);
-    int *tptr = ws.alloc_n<int>((size_t)blkm + 1);
-    int64_t *tmp = ws.alloc_n<int64_t>(scan_tmp_elems((size_t)blkm + 1));
-    hipLaunchKernelGGL(k_tile_count, dim3(grid_for(blkm, TC_WAVES)), dim3(64 * TC_WAVES), 0, s,
-                       X.rows, blkm, X.ptr, X.idx, tcount);
-    TSG_LAUNCH_CHECK();
-    exclusive_scan_i32(tcount, tptr, (size_t)blkm, tmp, s);
-    int nt = 0;
-    TSG_HIP_TRY(hipMemcpyAsync(&nt, tptr + blkm, sizeof(int), hipMemcpyDeviceToHost, s));
-    TSG_HIP_TRY(hipStreamSynchronize(s));
-    return nt;
+    if (blkm == 0 || X.nnz == 0) return 0.0;
+    unsigned long long *sums = ws.alloc_n<unsigned long long>(2);
+    auto count = [&](int step) {
+        TSG_HIP_TRY(hipMemsetAsync(sums, 0, 2 * sizeof(unsigned long long), s));
+        const int nrows = (blkm + step - 1) / step;
+        hipLaunchKernelGGL(k_tile_count, dim3(grid_for(nrows, TC_WAVES)), dim3(64 * TC_WAVES), 0,
+                           s, X.rows, blkm, step, X.ptr, X.idx, nullptr, sums);
+        TSG_LAUNCH_CHECK();
+        unsigned long long h[2] = {0, 0};
+        TSG_HIP_TRY(hipMemcpyAsync(h, sums, sizeof(h), hipMemcpyDeviceToHost, s));
+        TSG_HIP_TRY(hipStreamSynchronize(s));
+        return h[0] > 0 ? (double)h[1] / (double)h[0] : 0.0;
+    };
+    // a deterministic sample of <= 4096 tile rows decides unless it lands
+    // near the threshold, where every tile row is counted
+    const int step = blkm > 4096 ? (blkm + 4095) / 4096 : 1;
+    const double est = count(step);
+    if (step == 1 || est < 0.75 * threshold || est > 1.33 * threshold) return est;
+    return count(1);
 }

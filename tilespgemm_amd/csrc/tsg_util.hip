@@ -198,34 +198,48 @@ StageTimer::~StageTimer() { clear(); }
 // to two or three (no searches, no per-row loops), which is what sets their
 // duration.
 
-// Thread t checks row pointer t (t < m) and entry t (t < nnz).  Rows are
-// sorted and duplicate-free iff every descent ja[p-1] >= ja[p] (p > 0) sits at
-// the first entry of a non-empty row, i.e. iff the number of all descents
-// equals the number of descents at row starts: cnt[0] counts the former,
-// cnt[1] the latter.  Every access stays inside ia[0..m] and ja[0..nnz)
-// whatever the contents.
+// Threads check row pointers t < m and entries t < nnz (grid-stride).  Rows
+// are sorted and duplicate-free iff every descent ja[p-1] >= ja[p] (p > 0)
+// sits at the first entry of a non-empty row, i.e. iff the number of all
+// descents equals the number of descents at row starts: out[1] counts the
+// former, out[2] the latter (summed per block, one atomic per block and
+// counter).  Every access stays inside ia[0..m] and ja[0..nnz) whatever the
+// contents.
 __global__ void __launch_bounds__(256) k_validate(int m, int n, const int *__restrict__ ia,
                                                   const int *__restrict__ ja, int64_t nnz,
                                                   int *__restrict__ out) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ int s_cnt[2][4];
     int f = 0;
     int at_start = 0, descent = 0;
-    if (t < m) {
-        const int64_t lo = ia[t], hi = ia[t + 1];
-        if (lo > hi || lo < 0 || hi > nnz) f |= 1;
-        if (t == 0 && lo != 0) f |= 1;
-        if (t == m - 1 && hi != nnz) f |= 1;
-        if (lo > 0 && lo < hi && hi <= nnz) at_start = ja[lo - 1] >= ja[lo];
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t lim = nnz > m ? nnz : m;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < lim; t += stride) {
+        if (t < m) {
+            const int64_t lo = ia[t], hi = ia[t + 1];
+            if (lo > hi || lo < 0 || hi > nnz) f |= 1;
+            if (t == 0 && lo != 0) f |= 1;
+            if (t == m - 1 && hi != nnz) f |= 1;
+            if (lo > 0 && lo < hi && hi <= nnz) at_start += ja[lo - 1] >= ja[lo];
+        }
+        if (t < nnz) {
+            const int c = ja[t];
+            if (c < 0 || c >= n) f |= 2;
+            if (t > 0) descent += ja[t - 1] >= c;
+        }
     }
-    if (t < nnz) {
-        const int c = ja[t];
-        if (c < 0 || c >= n) f |= 2;
-        if (t > 0) descent = ja[t - 1] >= c;
-    }
-    const uint64_t md = __ballot(descent), ms = __ballot(at_start);
+    descent = wave_sum(descent);
+    at_start = wave_sum(at_start);
+    const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
-        if (md) atomicAdd(&out[1], (int)__popcll(md));
-        if (ms) atomicAdd(&out[2], (int)__popcll(ms));
+        s_cnt[0][w] = descent;
+        s_cnt[1][w] = at_start;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int d = s_cnt[0][0] + s_cnt[0][1] + s_cnt[0][2] + s_cnt[0][3];
+        const int st = s_cnt[1][0] + s_cnt[1][1] + s_cnt[1][2] + s_cnt[1][3];
+        if (d) atomicAdd(&out[1], d);
+        if (st) atomicAdd(&out[2], st);
     }
     if (f) atomicOr(&out[0], f);
 }
@@ -233,7 +247,10 @@ __global__ void __launch_bounds__(256) k_validate(int m, int n, const int *__res
 int validate_csr_pair(const CsrIn &A, const CsrIn &B, Arena &ws, hipStream_t s) {
     int *d = ws.alloc_n<int>(6);
     TSG_HIP_TRY(hipMemsetAsync(d, 0, 6 * sizeof(int), s));
-    auto threads = [](const CsrIn &X) { return std::max((size_t)X.rows, (size_t)X.nnz); };
+    // a fixed grid (grid-stride inside): few per-block atomics on the counters
+    auto threads = [](const CsrIn &X) {
+        return std::min(std::max((size_t)X.rows, (size_t)X.nnz), (size_t)256 * 2048);
+    };
     if (A.rows > 0) {
         hipLaunchKernelGGL(k_validate, dim3(grid_for(threads(A), 256)), dim3(256), 0, s, A.rows,
                            A.cols, A.ptr, A.idx, A.nnz, d);
