@@ -265,6 +265,27 @@ double tsg_stage_ms(tsg_handle_t handle, int stage);
  * NULL). */
 const char *tsg_last_error(tsg_handle_t handle);
 
+/* ====================================================================
+ * Matrix Market input (host only; replaces the reference loader
+ * mmio_allinone, mmio_highlevel.h:593-759, with the same semantics).
+ * Reads a coordinate file into a freshly malloc'ed 0-based CSR: entries of a
+ * row in file order, no sorting, duplicates kept; symmetric and hermitian
+ * files mirrored off the diagonal (same value); pattern entries 1.0, integer
+ * entries their value, complex entries their real part.  *symmetric reports
+ * the mirroring.  Release the arrays with tsg_free.  Returns TSG_OK, a
+ * TSG_MTX_ERR_* code (the reference's -1/-2/-4 less 100, plus bad entries),
+ * TSG_ERR_OVERFLOW (more than 2^31-1 entries) or TSG_ERR_ALLOC.
+ * ==================================================================== */
+enum {
+    TSG_MTX_ERR_OPEN = -101,     /* file cannot be opened                  */
+    TSG_MTX_ERR_BANNER = -102,   /* not "%%MatrixMarket matrix coordinate" */
+    TSG_MTX_ERR_SIZE = -104,     /* bad size line (or non-square symmetric) */
+    TSG_MTX_ERR_ENTRIES = -105   /* short file or an index out of range    */
+};
+int tsg_read_mtx(const char *path, int *m, int *n, int64_t *nnz, int *symmetric, int **ia,
+                 int **ja, double **a);
+void tsg_free(void *p);
+
 #ifdef __cplusplus
 }
 #endif

@@ -280,3 +280,32 @@ def tile_spgemm(m, n, k, A, B, fused: bool = True):
     maskC, rowptrC, offC = tile_step2(tA, tB, cscB, tptrC, tcolC)
     colC, valC = tile_step3(tA, tB, cscB, tptrC, tcolC, offC, maskC, fused)
     return tile_to_csr(m, tptrC, tcolC, offC, maskC, rowptrC, colC, valC)
+
+
+def mtx_read(text: str):
+    """Matrix Market coordinate text -> (m, n, indptr, indices, data,
+    symmetric), restating the reference loader mmio_allinone
+    (mmio_highlevel.h:593-759): 1-based to 0-based; symmetric/hermitian
+    entries off the diagonal mirrored with the same value, the mirror placed
+    right after the entry in its row's file order (:700-725); rows keep file
+    order, no sort, duplicates kept; pattern -> 1.0, integer -> value,
+    complex -> real part (:648-672)."""
+    lines = [ln for ln in text.splitlines()]
+    banner = lines[0].split()
+    field, symmetry = banner[3].lower(), banner[4].lower()
+    body = [ln for ln in lines[1:] if ln.strip() and not ln.lstrip().startswith("%")]
+    m, n, nz = (int(x) for x in body[0].split())
+    mirror = symmetry in ("symmetric", "hermitian")
+    rows = [[] for _ in range(m)]
+    for ln in body[1:1 + nz]:
+        tok = ln.split()
+        r, c = int(tok[0]) - 1, int(tok[1]) - 1
+        v = 1.0 if field == "pattern" else float(tok[2])
+        rows[r].append((c, v))
+        if mirror and r != c:
+            rows[c].append((r, v))
+    indptr = np.zeros(m + 1, dtype=np.int32)
+    indptr[1:] = np.cumsum([len(x) for x in rows])
+    indices = np.array([c for x in rows for c, _ in x], dtype=np.int32)
+    data = np.array([v for x in rows for _, v in x], dtype=np.float64)
+    return m, n, indptr, indices, data, mirror

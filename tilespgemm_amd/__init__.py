@@ -63,6 +63,8 @@ SIGNATURES = {
                                 _vp, _vp, _vp]),
     "tsg_batch_symbolic": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
     "tsg_batch_numeric": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
+    "tsg_read_mtx": (_i, [C.c_char_p, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "tsg_free": (None, [_vp]),
     "tsg_server_create": (_i, [_vp, _i, _vp]),
     "tsg_server_submit": (_i, [_vp, _vp, _vp]),
     "tsg_server_wait": (_i, [_vp, C.c_uint64, _vp]),
@@ -163,6 +165,32 @@ class Product(C.Structure):
         return cls(A.rows, A.cols, B.cols, A.indptr.data_ptr(), A.indices.data_ptr(),
                    A.data.data_ptr(), A.nnz, B.indptr.data_ptr(), B.indices.data_ptr(),
                    B.data.data_ptr(), B.nnz)
+
+
+MTX_ERRORS = {-101: "cannot open", -102: "bad banner", -104: "bad size line",
+              -105: "short file or index out of range"}
+
+
+def read_mtx(path: str):
+    """Matrix Market file -> (m, n, indptr int32, indices int32, data float64,
+    symmetric) with the reference loader's semantics (tsg_read_mtx)."""
+    m, n, sym = C.c_int(0), C.c_int(0), C.c_int(0)
+    nnz = C.c_int64(0)
+    pia, pja, pa = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    lib = load()
+    rc = lib.tsg_read_mtx(os.fsencode(path), C.byref(m), C.byref(n), C.byref(nnz), C.byref(sym),
+                          C.byref(pia), C.byref(pja), C.byref(pa))
+    if rc != 0:
+        raise TsgError(rc, f"{path}: {MTX_ERRORS.get(rc, 'read failed')}")
+    try:
+        k = nnz.value
+        ip = np.ctypeslib.as_array(C.cast(pia, C.POINTER(C.c_int)), (m.value + 1,)).copy()
+        ix = np.ctypeslib.as_array(C.cast(pja, C.POINTER(C.c_int)), (max(k, 1),))[:k].copy()
+        dv = np.ctypeslib.as_array(C.cast(pa, C.POINTER(C.c_double)), (max(k, 1),))[:k].copy()
+    finally:
+        for ptr in (pia, pja, pa):
+            lib.tsg_free(ptr)
+    return m.value, n.value, ip, ix, dv, bool(sym.value)
 
 
 class ServerConfig(C.Structure):

@@ -16,7 +16,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtsg.so")
-SOURCES = ["tsg_util.hip", "tsg_row.hip", "tsg_tile.hip", "tsg_batch.hip", "tsg_server.hip", "tsg_api.hip"]
+CLI = os.path.join(HERE, "tsg_spgemm")
+SOURCES = ["tsg_util.hip", "tsg_row.hip", "tsg_tile.hip", "tsg_batch.hip", "tsg_server.hip",
+           "tsg_api.hip", "tsg_io.hip"]
 ARCH = os.environ.get("TSG_OFFLOAD_ARCH", "gfx950")
 
 
@@ -54,8 +56,16 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("link of libtsg.so failed")
     os.replace(tmp, LIB)
+    # command-line driver, linked against the library next to it
+    cmd = [hipcc(), *flags(), os.path.join(CSRC, "tsg_cli.hip"), "-o", CLI + ".tmp",
+           f"-L{HERE}", "-ltsg", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("build of tsg_spgemm failed")
+    os.replace(CLI + ".tmp", CLI)
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {CLI}")
     return LIB
 
 
