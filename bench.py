@@ -135,6 +135,15 @@ def bench_batch(args):
     h.set_stream(s)
     torch.cuda.synchronize()
 
+    tuned = None
+    if not args.no_autotune:
+        from tilespgemm_amd import autotune as AT
+        t0 = time.time()
+        with torch.cuda.stream(s):
+            tuned = AT.tune(h, AT.batch_run(h, pairs), reps=2, log=log)
+        log(f"[bench] autotune {time.time() - t0:.1f}s: tile_density {tuned['tile_density']}, "
+            f"row_bins {tuned['row_bins']}, {tuned['default_ms']:.2f} -> {tuned['ms']:.2f} ms")
+
     def batched_step():
         arr, ic, per, roff, tot = h.batch_symbolic(pairs)
         jc = torch.empty(max(tot, 1), dtype=torch.int32, device=dev)
@@ -191,7 +200,11 @@ def bench_batch(args):
                    "per_call_req_s": round(nreq / t_single, 2),
                    "speedup_vs_per_call": round(t_single / t_batch, 2),
                    "graph_numeric_req_s": round(nreq / t_graph, 2),
-                   "graph_numeric_gflops": round(2.0 * products / t_graph / 1e9, 2)},
+                   "graph_numeric_gflops": round(2.0 * products / t_graph / 1e9, 2),
+                   "autotune": None if tuned is None else {
+                       "tile_density": tuned["tile_density"], "row_bins": tuned["row_bins"],
+                       "default_ms": round(tuned["default_ms"], 3),
+                       "tuned_ms": round(tuned["ms"], 3), "trials": len(tuned["trials"])}},
     }
     print(json.dumps(out), flush=True)
 
@@ -363,6 +376,8 @@ def main():
     ap.add_argument("--method", default="auto", choices=["auto", "tile", "row"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch", type=int, default=512, help="products in the er_batch workload")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="er_batch: keep the default tuning parameters")
     ap.add_argument("--requests", type=int, default=2000, help="requests in the served workload")
     ap.add_argument("--edge-factor", type=int, default=4, help="R-MAT draws / n in the served mix")
     ap.add_argument("--workers", type=int, default=3, help="server workers (served workload)")

@@ -93,6 +93,20 @@ int tsg_set_method(tsg_handle_t handle, int method);
 int tsg_set_fp_mode(tsg_handle_t handle, int fp_mode);
 /* Method the last symbolic phase used (TSG_METHOD_TILE or TSG_METHOD_ROW). */
 int tsg_get_method_used(tsg_handle_t handle);
+
+/* Tuning parameters (no counterpart in the reference, where the matching
+ * thresholds are compile-time: main_spgemm-col.cu:26-31,
+ * spgemm_nsparse_kernel.h:4-14).  They change speed, never results.
+ * tile_density: TSG_METHOD_AUTO takes the tile path when A holds at least
+ * this many entries per non-empty 16x16 tile (default 6).
+ * Row bins: rows with at most bin1 (<= 64), bin2 (<= 256) and bin3 (<= 1024)
+ * intermediate products go to the 16-, 32- and 64-lane LDS-hash kernels,
+ * longer rows to the big-row path (defaults 64, 256, 1024; bin1 <= bin2 <=
+ * bin3).  tilespgemm_amd.autotune searches them for a workload. */
+int tsg_set_tile_density(tsg_handle_t handle, double entries_per_tile);
+int tsg_set_row_bins(tsg_handle_t handle, int bin1, int bin2, int bin3);
+int tsg_get_tuning(tsg_handle_t handle, double *tile_density, int *bin1, int *bin2,
+                   int *bin3);
 /* Device workspace currently held by the handle, bytes. */
 size_t tsg_workspace_bytes(tsg_handle_t handle);
 
@@ -171,6 +185,8 @@ typedef struct {
     size_t mem_budget;           /* bytes charged to admitted, unreleased
                                     requests; 0 = 64 GiB                    */
     int method;                  /* TSG_METHOD_*                            */
+    double tile_density;         /* tsg_set_tile_density, 0 = default       */
+    int bin1, bin2, bin3;        /* tsg_set_row_bins, bin3 = 0: defaults    */
 } tsg_server_config_t;
 
 typedef struct {

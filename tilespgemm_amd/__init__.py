@@ -63,6 +63,9 @@ SIGNATURES = {
                                 _vp, _vp, _vp]),
     "tsg_batch_symbolic": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
     "tsg_batch_numeric": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
+    "tsg_set_tile_density": (_i, [_vp, C.c_double]),
+    "tsg_set_row_bins": (_i, [_vp, _i, _i, _i]),
+    "tsg_get_tuning": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "tsg_read_mtx": (_i, [C.c_char_p, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "tsg_free": (None, [_vp]),
     "tsg_server_create": (_i, [_vp, _i, _vp]),
@@ -196,7 +199,8 @@ def read_mtx(path: str):
 class ServerConfig(C.Structure):
     """tsg_server_config_t (include/tsg.h)."""
     _fields_ = [("workers", C.c_int), ("max_batch", C.c_int), ("mem_budget", C.c_size_t),
-                ("method", C.c_int)]
+                ("method", C.c_int), ("tile_density", C.c_double), ("bin1", C.c_int),
+                ("bin2", C.c_int), ("bin3", C.c_int)]
 
 
 class Result(C.Structure):
@@ -215,9 +219,12 @@ class Server:
     releases it."""
 
     def __init__(self, device: int = 0, workers: int = 2, max_batch: int = 64,
-                 mem_budget: int = 32 << 30, method: int = METHOD_AUTO):
+                 mem_budget: int = 32 << 30, method: int = METHOD_AUTO, tuning=None):
         self._s = C.c_void_p()
-        cfg = ServerConfig(workers, max_batch, mem_budget, method)
+        t = tuning or {}
+        bins = t.get("row_bins", (0, 0, 0))
+        cfg = ServerConfig(workers, max_batch, mem_budget, method,
+                           float(t.get("tile_density", 0.0)), *[int(x) for x in bins])
         _check(load().tsg_server_create(C.byref(self._s), device, C.byref(cfg)))
         self._keep = {}
 
@@ -323,6 +330,21 @@ class Handle:
 
     def set_method(self, method: int):
         _check(load().tsg_set_method(self._h, method), self._h)
+
+    def set_tuning(self, tuning: dict):
+        """Apply {"tile_density": x, "row_bins": (b1, b2, b3)} (either key
+        optional; see tsg_set_tile_density / tsg_set_row_bins)."""
+        if "tile_density" in tuning:
+            _check(load().tsg_set_tile_density(self._h, float(tuning["tile_density"])), self._h)
+        if "row_bins" in tuning:
+            b1, b2, b3 = (int(x) for x in tuning["row_bins"])
+            _check(load().tsg_set_row_bins(self._h, b1, b2, b3), self._h)
+
+    def tuning(self) -> dict:
+        d = C.c_double(0)
+        b = [C.c_int(0) for _ in range(3)]
+        _check(load().tsg_get_tuning(self._h, C.byref(d), *[C.byref(x) for x in b]), self._h)
+        return {"tile_density": d.value, "row_bins": tuple(x.value for x in b)}
 
     def set_fp_mode(self, fp_mode: int):
         _check(load().tsg_set_fp_mode(self._h, fp_mode), self._h)

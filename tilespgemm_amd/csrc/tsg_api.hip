@@ -75,8 +75,8 @@ int choose_method(tsg_handle_t h, const CsrIn &A, bool sorted) {
     if (A.nnz == 0 || A.rows == 0) return TSG_METHOD_ROW;
     // Tile path when A's 16x16 tiles hold enough entries to amortize the
     // per-tile metadata (~60 bytes per tile), row path otherwise.
-    const double per_tile = tile_density(A, 6.0, h->ws, h->stream);
-    return per_tile >= 6.0 ? TSG_METHOD_TILE : TSG_METHOD_ROW;
+    const double per_tile = tile_density(A, h->tile_density, h->ws, h->stream);
+    return per_tile >= h->tile_density ? TSG_METHOD_TILE : TSG_METHOD_ROW;
 }
 
 }  // namespace
@@ -137,6 +137,29 @@ int tsg_set_stream(tsg_handle_t h, void *stream) {
 int tsg_set_method(tsg_handle_t h, int method) {
     if (!h || method < TSG_METHOD_AUTO || method > TSG_METHOD_ROW) return TSG_ERR_INVALID;
     h->method = method;
+    return TSG_OK;
+}
+
+int tsg_set_tile_density(tsg_handle_t h, double entries_per_tile) {
+    if (!h || !(entries_per_tile > 0.0)) return TSG_ERR_INVALID;
+    h->tile_density = entries_per_tile;
+    return TSG_OK;
+}
+
+int tsg_set_row_bins(tsg_handle_t h, int bin1, int bin2, int bin3) {
+    if (!h) return TSG_ERR_INVALID;
+    if (bin1 < 0 || bin1 > 64 || bin2 < bin1 || bin2 > 256 || bin3 < bin2 || bin3 > 1024)
+        return TSG_ERR_INVALID;
+    h->row_tune = RowTuning{bin1, bin2, bin3};
+    return TSG_OK;
+}
+
+int tsg_get_tuning(tsg_handle_t h, double *tile_density, int *bin1, int *bin2, int *bin3) {
+    if (!h) return TSG_ERR_INVALID;
+    if (tile_density) *tile_density = h->tile_density;
+    if (bin1) *bin1 = h->row_tune.bin1;
+    if (bin2) *bin2 = h->row_tune.bin2;
+    if (bin3) *bin3 = h->row_tune.bin3;
     return TSG_OK;
 }
 
@@ -210,7 +233,7 @@ int tsg_spgemm_symbolic(tsg_handle_t h, int m, int n, int k, const int *d_ia, co
             const bool b_is_a = d_ia == d_ib && d_ja == d_jb && m == n && n == k;
             tile_symbolic(h->tile, A, B, b_is_a, d_ic, nnz_c, h->ws, h->stream, h->tm, h->cus);
         } else {
-            row_symbolic(h->row, A, B, d_ic, nnz_c, h->ws, h->stream, h->tm, h->cus);
+            row_symbolic(h->row, A, B, d_ic, nnz_c, h->ws, h->stream, h->tm, h->cus, h->row_tune);
             const bool b_validated = B.ptr != A.ptr || B.idx != A.idx || B.cols != A.cols;
             h->row.b_strict = b_validated ? (vf & 0x400) == 0 : (vf & 0x4) == 0;
         }

@@ -98,6 +98,13 @@ struct StageTimer {
 };
 
 // ------------------------------------------------------------ row path
+// Tuning parameters of the row path (tsg_set_row_bins): rows with at most
+// bin1 / bin2 / bin3 intermediate products go to the 16-, 32- and 64-lane
+// hash kernels, the rest to the big-row path.
+struct RowTuning {
+    int bin1 = 64, bin2 = 256, bin3 = 1024;
+};
+
 struct RowPlan {
     bool valid = false;
     bool b_strict = false;           // B rows strictly ascending (no duplicates)
@@ -132,7 +139,8 @@ struct RowPlan {
 };
 
 void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic,
-                  int64_t *nnz_c, Arena &ws, hipStream_t s, StageTimer &tm, int cus);
+                  int64_t *nnz_c, Arena &ws, hipStream_t s, StageTimer &tm, int cus,
+                  const RowTuning &tune = RowTuning());
 void row_numeric(RowPlan &p, const CsrIn &A, const CsrIn &B, const int64_t *d_ic,
                  int *d_jc, double *d_c, int fused, Arena &ws, hipStream_t s,
                  StageTimer &tm, int cus);
@@ -203,6 +211,8 @@ struct tsg_handle_s {
     int method = TSG_METHOD_AUTO;
     int fp_mode = TSG_FP_FUSED;
     int method_used = TSG_METHOD_AUTO;
+    double tile_density = 6.0;   // AUTO: tile path from this many entries per tile
+    RowTuning row_tune;
     Arena ws;       // plan of the last symbolic phase
     Arena bws;      // batch layout (outlives the plan's reset)
     Arena aux;      // scratch of utility calls (transpose, product counts)

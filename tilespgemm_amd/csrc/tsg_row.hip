@@ -33,7 +33,7 @@ namespace {
 constexpr int SMALL_PROD = 1024;       // rows up to this many products: small path
 constexpr int NBINS = 5;               // 0: empty, 1..3: small, 4: big
 constexpr int BIN_BIG = 4;
-constexpr int bin_limit[NBINS] = {0, 64, 256, 1024, 0x7fffffff};
+constexpr int bin_limit[NBINS] = {0, 64, 256, SMALL_PROD, 0x7fffffff};
 
 constexpr int SUB_SHIFT = 10;          // sub-block = 1024 columns
 constexpr int BIG_PASS_SHIFT = 20;     // one symbolic pass covers 2^20 columns
@@ -55,21 +55,30 @@ constexpr int ITEM_U = 4;
 
 struct BinStarts { int s[NBINS + 1]; };
 
-__device__ __forceinline__ int bin_of(int64_t prod) {
+// Row bins by intermediate products: (0, l1] -> 16-lane groups with 128 hash
+// slots, (l1, l2] -> 32 lanes and 512 slots, (l2, l3] -> 64 lanes and 2048
+// slots, above l3 -> the big-row path.  Each small kernel is exact for any
+// row up to bin_limit[b] products (hash load <= 1/2), so any limits with
+// l1 <= 64, l2 <= 256, l3 <= 1024 (checked on the host) are valid; they are
+// tuning parameters (tsg_set_row_bins).
+struct BinLimits { int l1, l2, l3; };
+
+__device__ __forceinline__ int bin_of(int64_t prod, BinLimits lim) {
     if (prod == 0) return 0;
-    if (prod <= 64) return 1;
-    if (prod <= 256) return 2;
-    if (prod <= SMALL_PROD) return 3;
+    if (prod <= lim.l1) return 1;
+    if (prod <= lim.l2) return 2;
+    if (prod <= lim.l3) return 3;
     return BIN_BIG;
 }
 
 // ------------------------------------------------------------------ binning
-__global__ void k_bin_count(int m, const int64_t *__restrict__ rowprod, int *__restrict__ cnt) {
+__global__ void k_bin_count(int m, const int64_t *__restrict__ rowprod, BinLimits lim,
+                            int *__restrict__ cnt) {
     __shared__ int s_cnt[NBINS];
     if (threadIdx.x < NBINS) s_cnt[threadIdx.x] = 0;
     __syncthreads();
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < m) atomicAdd(&s_cnt[bin_of(rowprod[i])], 1);
+    if (i < m) atomicAdd(&s_cnt[bin_of(rowprod[i], lim)], 1);
     __syncthreads();
     if (threadIdx.x < NBINS && s_cnt[threadIdx.x]) atomicAdd(&cnt[threadIdx.x], s_cnt[threadIdx.x]);
 }
@@ -78,7 +87,8 @@ __global__ void k_bin_count(int m, const int64_t *__restrict__ rowprod, int *__r
 // gives each lane its slot among the wave's rows of that bin; per block, one
 // LDS atomic per wave and bin; per grid, one global atomic per block and bin.
 __global__ void __launch_bounds__(256) k_bin_scatter(int m, const int64_t *__restrict__ rowprod,
-                                                     BinStarts st, int *__restrict__ fill,
+                                                     BinLimits lim, BinStarts st,
+                                                     int *__restrict__ fill,
                                                      int *__restrict__ bin_rows,
                                                      int64_t *__restrict__ rownnz) {
     __shared__ int s_cnt[NBINS], s_base[NBINS];
@@ -86,7 +96,7 @@ __global__ void __launch_bounds__(256) k_bin_scatter(int m, const int64_t *__res
     __syncthreads();
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
-    const int b = i < m ? bin_of(rowprod[i]) : -1;
+    const int b = i < m ? bin_of(rowprod[i], lim) : -1;
     if (b == 0) rownnz[i] = 0;
     const uint64_t below = (uint64_t(1) << lane) - 1;
     int pos = 0;
@@ -676,8 +686,13 @@ void set_smem(K kernel, size_t bytes) {
 
 // =================================================================== host
 void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int64_t *nnz_c,
-                  Arena &ws, hipStream_t s, StageTimer &tm, int cus) {
+                  Arena &ws, hipStream_t s, StageTimer &tm, int cus, const RowTuning &tune) {
     const int m = A.rows, n = A.cols, k = B.cols;
+    TSG_CHECK(tune.bin1 >= 0 && tune.bin1 <= bin_limit[1] && tune.bin1 <= tune.bin2 &&
+                  tune.bin2 <= bin_limit[2] && tune.bin2 <= tune.bin3 &&
+                  tune.bin3 <= bin_limit[3],
+              TSG_ERR_INVALID, "row bins out of range");
+    const BinLimits lim{tune.bin1, tune.bin2, tune.bin3};
     p = RowPlan();
     p.m = m;
     p.n = n;
@@ -692,7 +707,7 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
     csr_row_products(m, A.ptr, A.idx, A.nnz, B.ptr, p.rowprod, ws, s);
     TSG_HIP_TRY(hipMemsetAsync(d_cnt, 0, sizeof(int) * (2 * NBINS + 2), s));
     if (m > 0) {
-        hipLaunchKernelGGL(k_bin_count, dim3(grid_for(m, 256)), dim3(256), 0, s, m, p.rowprod,
+        hipLaunchKernelGGL(k_bin_count, dim3(grid_for(m, 256)), dim3(256), 0, s, m, p.rowprod, lim,
                            d_cnt);
         TSG_LAUNCH_CHECK();
     }
@@ -705,7 +720,7 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
     for (int b = 0; b <= NBINS; b++) p.bin_start_h[b] = st.s[b];
     p.nbins = NBINS;
     if (m > 0) {
-        hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(m, 256)), dim3(256), 0, s, m, p.rowprod,
+        hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(m, 256)), dim3(256), 0, s, m, p.rowprod, lim,
                            st, d_cnt + NBINS, p.bin_rows, p.rownnz);
         TSG_LAUNCH_CHECK();
     }

@@ -246,6 +246,8 @@ void tsg_server_s::worker_main() {
     tsg_handle_t h = nullptr;
     if (tsg_create(&h, device) != TSG_OK) return;
     tsg_set_method(h, cfg.method);
+    if (cfg.tile_density > 0.0) tsg_set_tile_density(h, cfg.tile_density);
+    if (cfg.bin3 > 0) tsg_set_row_bins(h, cfg.bin1, cfg.bin2, cfg.bin3);
     for (;;) {
         std::vector<Req *> batch;
         {
@@ -293,6 +295,15 @@ int tsg_server_create(tsg_server_t *server, int device, const tsg_server_config_
     if (s->cfg.mem_budget == 0) s->cfg.mem_budget = size_t(64) << 30;
     int rc = tsg_create(&s->est, device);
     if (rc != TSG_OK) return rc;
+    // the estimator's handle checks the tuning the workers will apply
+    if ((s->cfg.tile_density > 0.0 &&
+         tsg_set_tile_density(s->est, s->cfg.tile_density) != TSG_OK) ||
+        (s->cfg.bin3 > 0 &&
+         tsg_set_row_bins(s->est, s->cfg.bin1, s->cfg.bin2, s->cfg.bin3) != TSG_OK) ||
+        s->cfg.tile_density < 0.0) {
+        tsg_destroy(s->est);
+        return TSG_ERR_INVALID;
+    }
     s->estimator = std::thread([p = s.get()] { p->estimator_main(); });
     for (int i = 0; i < s->cfg.workers; i++)
         s->workers.emplace_back([p = s.get()] { p->worker_main(); });
