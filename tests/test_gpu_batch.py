@@ -135,3 +135,32 @@ def test_graph_captured_numeric_replays(handle, method):
                 assert_csr_equal(got, O.csr_spgemm(m, k, X, X))
     finally:
         handle.set_stream(None)
+
+
+def test_auto_batch_tile_dense_with_unsorted_rows(handle):
+    """A tile-dense batch (27-point stencils) lays out global columns for the
+    tile path; one product with reversed rows sends the whole batch to the
+    row path, whose block-diagonal columns are then made local.  Both batches
+    equal the oracle."""
+    S = [gen.stencil27(m, seed=m) for m in (5, 6, 7)]
+    dims = [m ** 3 for m in (5, 6, 7)]
+    handle.set_method(tsg.METHOD_AUTO)
+    pairs = [(dev(n, n, X),) * 2 for n, X in zip(dims, S)]
+    res = handle.batch_spgemm(pairs)
+    assert handle.method_used == tsg.METHOD_TILE
+    for n, X, (ic, jc, c) in zip(dims, S, res):
+        assert_csr_equal((to_host(ic), to_host(jc), to_host(c)), O.csr_spgemm(n, n, X, X))
+    ip, ix, d = (x.copy() for x in S[1])
+    for i in range(dims[1]):
+        ix[ip[i]:ip[i + 1]] = ix[ip[i]:ip[i + 1]][::-1].copy()
+        d[ip[i]:ip[i + 1]] = d[ip[i]:ip[i + 1]][::-1].copy()
+    U = (ip, ix, d)
+    pairs[1] = (dev(dims[1], dims[1], U), dev(dims[1], dims[1], S[1]))
+    res = handle.batch_spgemm(pairs)
+    assert handle.method_used == tsg.METHOD_ROW
+    want = [O.csr_spgemm(dims[0], dims[0], S[0], S[0]),
+            O.ref_thmkl(dims[1], dims[1], dims[1], U, S[1], "fma") if O.ref_available()
+            else O.csr_spgemm(dims[1], dims[1], U, S[1]),
+            O.csr_spgemm(dims[2], dims[2], S[2], S[2])]
+    for w, (ic, jc, c) in zip(want, res):
+        assert_csr_equal((to_host(ic), to_host(jc), to_host(c)), w)

@@ -8,10 +8,6 @@
 #include <memory>
 #include <mutex>
 
-// tsg_tile.hip: entries per 16x16 tile of X (a sample decides when clear of
-// the threshold)
-double tile_density(const CsrIn &X, double threshold, Arena &ws, hipStream_t s);
-
 
 namespace {
 

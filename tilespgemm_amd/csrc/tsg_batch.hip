@@ -50,7 +50,11 @@ __global__ void k_cat_rows(int count, const BDesc *__restrict__ desc,
     out_ptr[r] = (int)(eoff[p] + ptr[r - roff[p]]);
 }
 
-template <bool B, bool VALUES>
+// Entries of the concatenated A (B = false) or B (B = true).  A's columns
+// index rows of the concatenated B, so they move by the product's B-row
+// offset; B's columns move by the product's column offset unless LOCAL (each
+// product's C then comes out with its own columns).
+template <bool B, bool VALUES, bool LOCAL = false>
 __global__ void k_cat_entries(int count, const BDesc *__restrict__ desc,
                               const int64_t *__restrict__ off, int64_t nnz,
                               int *__restrict__ out_idx, double *__restrict__ out_val) {
@@ -64,7 +68,7 @@ __global__ void k_cat_entries(int count, const BDesc *__restrict__ desc,
         out_val[q] = B ? desc[p].b[local] : desc[p].a[local];
     } else {
         const int *idx = B ? desc[p].jb : desc[p].ja;
-        out_idx[q] = (int)(idx[local] + coff[p]);
+        out_idx[q] = (B && LOCAL) ? idx[local] : (int)(idx[local] + coff[p]);
     }
 }
 
@@ -138,6 +142,8 @@ void batch_symbolic(tsg_handle_t h, int count, const tsg_product_t *prods, int64
     TSG_CHECK(O(OFF_ROW, count) < lim && O(OFF_BROW, count) < lim && O(OFF_BCOL, count) < lim &&
                   O(OFF_A, count) < lim && O(OFF_B, count) < lim,
               TSG_ERR_OVERFLOW, "batch exceeds int32 rows, columns or entries");
+    int max_k = 0;
+    for (int p = 0; p < count; p++) max_k = std::max(max_k, prods[p].k);
     S.count = count;
     S.b_is_a = b_is_a;
     S.rows = (int)O(OFF_ROW, count);
@@ -158,13 +164,14 @@ void batch_symbolic(tsg_handle_t h, int count, const tsg_product_t *prods, int64
     S.ia = h->bws.alloc_n<int>((size_t)S.rows + 1);
     S.ja = h->bws.alloc_n<int>((size_t)S.nnz_a + 1);
     S.a = h->bws.alloc_n<double>((size_t)S.nnz_a + 1);
+    // B keeps its own arrays except when every product is A_i * A_i: then its
+    // row pointers and values are A's, and so are its columns unless they are
+    // kept local (below)
     if (b_is_a) {
         S.ib = S.ia;
-        S.jb = S.ja;
         S.b = S.a;
     } else {
         S.ib = h->bws.alloc_n<int>((size_t)S.cols_a + 1);
-        S.jb = h->bws.alloc_n<int>((size_t)S.nnz_b + 1);
         S.b = h->bws.alloc_n<double>((size_t)S.nnz_b + 1);
     }
     int64_t *d_pc = h->bws.alloc_n<int64_t>((size_t)count + 1);
@@ -184,15 +191,44 @@ void batch_symbolic(tsg_handle_t h, int count, const tsg_product_t *prods, int64
         hipLaunchKernelGGL(k_cat_rows<true>, dim3(grid_for((size_t)S.cols_a + 1, 256)), dim3(256),
                            0, s, count, dd, S.row_off, (int64_t)S.cols_a, S.ib);
         TSG_LAUNCH_CHECK();
-        if (S.nnz_b > 0) {
-            hipLaunchKernelGGL((k_cat_entries<true, false>), dim3(grid_for((size_t)S.nnz_b, 256)),
-                               dim3(256), 0, s, count, dd, S.row_off, S.nnz_b, S.jb, nullptr);
-            TSG_LAUNCH_CHECK();
-        }
+    }
+    // Method of the whole batch, decided before B's columns are laid out: on
+    // the row path every row of A_i reaches only rows of B_i, so B keeps each
+    // product's own columns and C comes out with them (and the column range,
+    // which sizes the row path's structures, is max k_i, not sum k_i); the
+    // tile path keeps the block-diagonal columns (its B tile columns would
+    // otherwise merge across products) and C's columns are made local after.
+    // (AUTO with tile-dense A keeps AUTO: the symbolic phase still sends
+    // unsorted input to the row path, whose result is then made local too)
+    int method = h->method;
+    if (method == TSG_METHOD_AUTO) {
+        const CsrIn Acat{S.rows, S.cols_a, S.ia, S.ja, nullptr, S.nnz_a};
+        if (S.nnz_a == 0 || tile_density(Acat, h->tile_density, h->bws, s) < h->tile_density)
+            method = TSG_METHOD_ROW;
+    }
+    S.local_cols = method == TSG_METHOD_ROW;
+    if (S.local_cols) S.cols_b = max_k;
+    if (b_is_a && !S.local_cols)
+        S.jb = S.ja;
+    else
+        S.jb = h->bws.alloc_n<int>((size_t)S.nnz_b + 1);
+    if (S.nnz_b > 0 && S.jb != S.ja) {
+        if (S.local_cols)
+            hipLaunchKernelGGL((k_cat_entries<true, false, true>),
+                               dim3(grid_for((size_t)S.nnz_b, 256)), dim3(256), 0, s, count, dd,
+                               S.row_off, S.nnz_b, S.jb, nullptr);
+        else
+            hipLaunchKernelGGL((k_cat_entries<true, false, false>),
+                               dim3(grid_for((size_t)S.nnz_b, 256)), dim3(256), 0, s, count, dd,
+                               S.row_off, S.nnz_b, S.jb, nullptr);
+        TSG_LAUNCH_CHECK();
     }
     int64_t tot = 0;
+    const int saved_method = h->method;
+    h->method = method;
     int rc = tsg_spgemm_symbolic(h, S.rows, S.cols_a, S.cols_b, S.ia, S.ja, S.nnz_a, S.ib, S.jb,
                                  S.nnz_b, d_ic, &tot);
+    h->method = saved_method;
     if (rc != TSG_OK) throw TsgError{rc, h->err};
     *nnz_c_total = tot;
     if (nnz_c) {
@@ -231,7 +267,7 @@ void batch_numeric(tsg_handle_t h, int count, const tsg_product_t *prods, const 
     int rc = tsg_spgemm_numeric(h, S.rows, S.cols_a, S.cols_b, S.ia, S.ja, S.a, S.nnz_a, S.ib,
                                 S.jb, S.b, S.nnz_b, d_ic, d_jc, d_c);
     if (rc != TSG_OK) throw TsgError{rc, h->err};
-    if (S.rows > 0) {
+    if (S.rows > 0 && !S.local_cols) {
         hipLaunchKernelGGL(k_local_cols, dim3(grid_for((size_t)S.rows * 8, 256)), dim3(256), 0, s,
                            count, S.row_off, (int64_t)S.rows, d_ic, d_jc);
         TSG_LAUNCH_CHECK();

@@ -193,7 +193,8 @@ struct BatchState {
     bool valid = false;
     int count = 0;
     bool b_is_a = false;
-    int rows = 0, cols_a = 0, cols_b = 0;     // sum m, sum n, sum k
+    bool local_cols = false;                   // B (and so C) keeps each product's own columns
+    int rows = 0, cols_a = 0, cols_b = 0;     // sum m, sum n; columns of C (see local_cols)
     int64_t nnz_a = 0, nnz_b = 0;
     void *desc = nullptr;                      // device descriptors
     int64_t *row_off = nullptr;                // [count+1] C/A row offsets
@@ -236,6 +237,9 @@ void batch_rebase(tsg_handle_t h, const int64_t *d_ic, int64_t *d_out);
 // 0-7 and of B in bits 8-15: 1 bad row pointers, 2 column out of range,
 // 4 a row not strictly ascending.
 int validate_csr_pair(const CsrIn &A, const CsrIn &B, Arena &ws, hipStream_t s);
+// entries per non-empty 16x16 tile of X (tsg_tile.hip; a sample decides when
+// clear of the threshold)
+double tile_density(const CsrIn &X, double threshold, Arena &ws, hipStream_t s);
 void csr_transpose(int m, int n, const int *ia, const int *ja, const double *a,
                    int64_t nnz, int *tp, int *ti, double *tv, Arena &ws,
                    hipStream_t s, int *tpos = nullptr);
