@@ -254,6 +254,47 @@ __global__ void __launch_bounds__(RPB * G) k_small_numeric(int nrows, const int 
             kmax = max(kmax, __shfl_xor(kmax, d, G));
         }
         const int span = kmax - kmin;
+        const int nw = (span >> 5) + 1;              // bitmap words over [kmin, kmax]
+        if (nw <= SLOTS) {
+            // rank by bitmap: the free back halves of the table hold one bit per
+            // column of the row's span (the value half, SLOTS words) and the
+            // outputs before every pair of words (the key half); an entry's
+            // position is its pair's prefix plus the bits below it
+            unsigned *bmp = reinterpret_cast<unsigned *>(vals + SLOTS / 2);
+            int *pfx = keys + SLOTS / 2;
+            for (int i = l; i < nw; i += G) bmp[i] = 0u;
+            wave_lds_fence();
+            for (int i = l; i < nocc; i += G) {
+                const int kk = keys[i] - kmin;
+                atomicOr(&bmp[kk >> 5], 1u << (kk & 31));
+            }
+            wave_lds_fence();
+            const int npair = (nw + 1) >> 1;
+            int carry = 0;
+            for (int p0 = 0; p0 < npair; p0 += G) {
+                const int p = p0 + l;
+                int pc = 0;
+                if (p < npair) {
+                    pc = __popc(bmp[2 * p]);
+                    if (2 * p + 1 < nw) pc += __popc(bmp[2 * p + 1]);
+                }
+                const int incl = group_incl_scan<G>(pc);
+                if (p < npair) pfx[p] = carry + incl - pc;
+                carry += group_last<G>(incl);
+            }
+            wave_lds_fence();
+            for (int i = l; i < nocc; i += G) {
+                const int kk = keys[i] - kmin;
+                const int w = kk >> 5;
+                int rank = pfx[w >> 1] + __popc(bmp[w] & ((1u << (kk & 31)) - 1u));
+                if (w & 1) rank += __popc(bmp[w - 1]);
+                if (rank < nnz) {
+                    jc[o + rank] = keys[i];
+                    c[o + rank] = vals[i];
+                }
+            }
+            return;
+        }
         const int bits = span > 0 ? 32 - __clz(span) : 1;
         const bool in_dst = radix_sort_group<G>(keys, vals, keys + SLOTS / 2, vals + SLOTS / 2,
                                                 nocc, kmin, bits, hist);
