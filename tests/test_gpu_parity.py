@@ -309,3 +309,60 @@ def test_auto_method_choice(handle):
     got = run(handle, 2048, 2048, 2048, R, RT, tsg.METHOD_AUTO)
     assert handle.method_used == tsg.METHOD_ROW
     assert_csr_equal(got, O.csr_spgemm(2048, 2048, R, RT))
+
+
+def _scramble_rows(X, seed, dup_every=0):
+    """Reverse every row's entries (and, with dup_every, repeat every k-th
+    entry right after itself with a new value): rows no longer ascending."""
+    rng = np.random.default_rng(seed)
+    ip, ix, d = (np.asarray(x) for x in X)
+    rows, cols, vals = [], [], []
+    newp = [0]
+    for i in range(ip.size - 1):
+        c = ix[ip[i]:ip[i + 1]][::-1]
+        v = d[ip[i]:ip[i + 1]][::-1]
+        rc, rv = [], []
+        for j, (cc, vv) in enumerate(zip(c, v)):
+            rc.append(cc)
+            rv.append(vv)
+            if dup_every and j % dup_every == 0:
+                rc.append(cc)
+                rv.append(rng.uniform(-1, 1))
+        cols += rc
+        vals += rv
+        newp.append(newp[-1] + len(rc))
+    return (np.array(newp, dtype=np.int32), np.array(cols, dtype=np.int32),
+            np.array(vals, dtype=np.float64))
+
+
+@pytest.mark.parametrize("dup_every", [0, 3])
+@pytest.mark.parametrize("bins", [(64, 256, 1024), (0, 0, 0), (16, 64, 256)])
+def test_unsorted_and_duplicate_b_rows(handle, dup_every, bins):
+    """B rows out of order (and with repeated columns): the row path sorts B
+    stably, so every output accumulates in the reference's order; small rows
+    (hash and rank paths) and big rows (every row big with bins 0,0,0)."""
+    R = gen.rmat(12, 60_000, seed=8)
+    RT = gen.transpose(4096, 4096, *R)
+    U = _scramble_rows(RT, seed=dup_every, dup_every=dup_every)
+    handle.set_tuning({"row_bins": bins})
+    try:
+        got = run(handle, 4096, 4096, 4096, R, U, tsg.METHOD_AUTO)
+        assert handle.method_used == tsg.METHOD_ROW
+        want = O.ref_thmkl(4096, 4096, 4096, R, U, "fma") if O.ref_available() else \
+            O.csr_spgemm(4096, 4096, R, U)
+        assert_csr_equal(got, want)
+        # a second numeric phase with new values reuses the sorted copy's map
+        dA, dU = to_dev(4096, 4096, R), to_dev(4096, 4096, U)
+        ic, nnz = handle.symbolic(dA, dU)
+        import torch
+        jc = torch.empty(max(nnz, 1), dtype=torch.int32, device="cuda:0")
+        c = torch.empty(max(nnz, 1), dtype=torch.float64, device="cuda:0")
+        U2 = (U[0], U[1], U[2] * 2.0 + 1.0)
+        dU.data.copy_(torch.from_numpy(U2[2]))
+        handle.numeric(dA, dU, ic, jc, c)
+        want2 = O.ref_thmkl(4096, 4096, 4096, R, U2, "fma") if O.ref_available() else \
+            O.csr_spgemm(4096, 4096, R, U2)
+        assert_csr_equal((ic.cpu().numpy(), jc[:nnz].cpu().numpy(), c[:nnz].cpu().numpy()),
+                         want2)
+    finally:
+        handle.set_tuning({"row_bins": (64, 256, 1024)})

@@ -229,9 +229,32 @@ int tsg_spgemm_symbolic(tsg_handle_t h, int m, int n, int k, const int *d_ia, co
             const bool b_is_a = d_ia == d_ib && d_ja == d_jb && m == n && n == k;
             tile_symbolic(h->tile, A, B, b_is_a, d_ic, nnz_c, h->ws, h->stream, h->tm, h->cus);
         } else {
-            row_symbolic(h->row, A, B, d_ic, nnz_c, h->ws, h->stream, h->tm, h->cus, h->row_tune);
+            // The row path reads every B row in column order (spans from a
+            // row's first and last column, block pointers by search), so B
+            // rows that are not strictly ascending are replaced by a stably
+            // sorted copy: equal columns keep their storage order, so every
+            // output still accumulates its products in the reference's order.
             const bool b_validated = B.ptr != A.ptr || B.idx != A.idx || B.cols != A.cols;
-            h->row.b_strict = b_validated ? (vf & 0x400) == 0 : (vf & 0x4) == 0;
+            bool b_strict = b_validated ? (vf & 0x400) == 0 : (vf & 0x4) == 0;
+            CsrIn Bs = B;
+            int *b_idx = nullptr, *b_perm = nullptr;
+            double *b_val = nullptr;
+            if (!b_strict && nnz_b > 0) {
+                b_idx = h->ws.alloc_n<int>((size_t)nnz_b);
+                b_perm = h->ws.alloc_n<int>((size_t)nnz_b);
+                b_val = h->ws.alloc_n<double>((size_t)nnz_b);
+                h->aux.reset();
+                csr_sort_rows(n, k, d_ib, d_jb, nnz_b, b_idx, b_perm, h->ws, h->aux, h->stream);
+                Bs.idx = b_idx;
+                // duplicates remain after sorting: strict only if none
+                b_strict = (validate_csr_pair(Bs, Bs, h->aux, h->stream) & 0x4) == 0;
+            }
+            row_symbolic(h->row, A, Bs, d_ic, nnz_c, h->ws, h->stream, h->tm, h->cus, h->row_tune);
+            h->row.b_strict = b_strict;
+            h->row.b_canon = b_idx != nullptr;
+            h->row.b_idx = b_idx;
+            h->row.b_perm = b_perm;
+            h->row.b_val = b_val;
         }
     });
 }
@@ -255,7 +278,13 @@ int tsg_spgemm_numeric(tsg_handle_t h, int m, int n, int k, const int *d_ia, con
             tile_numeric(h->tile, A, B, d_ic, d_jc, d_c, fused, h->ws, h->stream, h->tm, h->cus);
         } else {
             TSG_CHECK(h->row.valid, TSG_ERR_STATE, "numeric phase without a symbolic phase");
-            row_numeric(h->row, A, B, d_ic, d_jc, d_c, fused, h->ws, h->stream, h->tm, h->cus);
+            CsrIn Bs = B;
+            if (h->row.b_canon) {
+                gather_values(nnz_b, h->row.b_perm, d_b, h->row.b_val, h->stream);
+                Bs.idx = h->row.b_idx;
+                Bs.val = h->row.b_val;
+            }
+            row_numeric(h->row, A, Bs, d_ic, d_jc, d_c, fused, h->ws, h->stream, h->tm, h->cus);
         }
     });
 }

@@ -136,6 +136,12 @@ struct RowPlan {
     bool use_rank = false;
     unsigned *rowbits = nullptr;     // [nsb_total * 32] words
     int64_t *item_off = nullptr;     // output offset (within C) per item
+    // B with rows sorted, when the input's were not (set after row_symbolic):
+    // the plan ran on (B.ptr, b_idx); numeric phases gather values by b_perm
+    bool b_canon = false;
+    int *b_idx = nullptr;
+    int *b_perm = nullptr;
+    double *b_val = nullptr;
 };
 
 void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic,
@@ -243,5 +249,10 @@ double tile_density(const CsrIn &X, double threshold, Arena &ws, hipStream_t s);
 void csr_transpose(int m, int n, const int *ia, const int *ja, const double *a,
                    int64_t nnz, int *tp, int *ti, double *tv, Arena &ws,
                    hipStream_t s, int *tpos = nullptr);
+// Row-sorted copy of X's columns (stable: equal columns keep their order) and
+// the original position of every sorted entry; temporaries from tmp.
+void csr_sort_rows(int m, int n, const int *ia, const int *ja, int64_t nnz, int *sorted_ja,
+                   int *perm, Arena &keep, Arena &tmp, hipStream_t s);
+void gather_values(int64_t n, const int *perm, const double *src, double *dst, hipStream_t s);
 void csr_row_products(int m, const int *ia, const int *ja, int64_t nnz, const int *ib,
                       int64_t *rowprod, Arena &ws, hipStream_t s);

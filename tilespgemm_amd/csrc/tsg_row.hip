@@ -116,6 +116,31 @@ __global__ void __launch_bounds__(256) k_bin_scatter(int m, const int64_t *__res
     if (b > 0) bin_rows[st.s[b] + s_base[b] + pos] = i;
 }
 
+// Column span [lo, hi] of a row's products, from the first and last column
+// of each B row its A entries name (lo > hi for a row without products); G
+// lanes of a group.
+template <int G>
+__device__ __forceinline__ void row_span(int pa, int pe, const int *__restrict__ ja,
+                                         const int *__restrict__ ib,
+                                         const int *__restrict__ jb, int &lo, int &hi) {
+    const int l = __lane_id() & (G - 1);
+    lo = 0x7fffffff;
+    hi = -1;
+    for (int p = pa + l; p < pe; p += G) {
+        const int t = ja[p];
+        const int b0 = ib[t], b1 = ib[t + 1];
+        if (b0 < b1) {
+            lo = min(lo, jb[b0]);
+            hi = max(hi, jb[b1 - 1]);
+        }
+    }
+#pragma unroll
+    for (int d = G / 2; d > 0; d >>= 1) {
+        lo = min(lo, __shfl_xor(lo, d, G));
+        hi = max(hi, __shfl_xor(hi, d, G));
+    }
+}
+
 // ------------------------------------------------------ small rows: symbolic
 template <int G, int SLOTS>
 __global__ void __launch_bounds__(256) k_small_symbolic(int nrows, const int *__restrict__ rows,
@@ -131,11 +156,36 @@ __global__ void __launch_bounds__(256) k_small_symbolic(int nrows, const int *__
     const int l = threadIdx.x & (G - 1);
     int *keys = s_keys + grp * SLOTS;
     const int r = blockIdx.x * RPB + grp;
-    for (int i = l; i < SLOTS; i += G) keys[i] = -1;
     s_mark[threadIdx.x] = -1;
     wave_lds_fence();
     if (r >= nrows) return;
     const int row = rows[r];
+    int lo, hi;
+    row_span<G>(ia[row], ia[row + 1], ja, ib, jb, lo, hi);
+    const int nw = hi >= lo ? ((hi - lo) >> 5) + 1 : 0;
+    if (nw <= SLOTS) {
+        // the row's columns fit a bitmap over its span in the table's space:
+        // mark them and count the bits (no hashing)
+        unsigned *bmp = reinterpret_cast<unsigned *>(keys);
+        for (int i = l; i < nw; i += G) bmp[i] = 0u;
+        wave_lds_fence();
+        for_products<G, SMALL_U, false, false>(ia[row], ia[row + 1], 0, G, ja, nullptr, ib, jb,
+                                               nullptr, nullptr, 0, 0, 0, 0, s_mark + grp * G,
+                                               [&](bool act, int col, double, double, bool) {
+                                                   if (act) {
+                                                       const int cc = col - lo;
+                                                       atomicOr(&bmp[cc >> 5], 1u << (cc & 31));
+                                                   }
+                                               });
+        wave_lds_fence();
+        int cnt = 0;
+        for (int i = l; i < nw; i += G) cnt += __popc(bmp[i]);
+        cnt = group_sum<G>(cnt);
+        if (l == 0) rownnz[row] = cnt;
+        return;
+    }
+    for (int i = l; i < SLOTS; i += G) keys[i] = -1;
+    wave_lds_fence();
     int created_cnt = 0;
     for_products<G, SMALL_U, false, false>(ia[row], ia[row + 1], 0, G, ja, nullptr, ib, jb,
                                            nullptr, nullptr, 0, 0, 0, 0, s_mark + grp * G,
@@ -188,12 +238,80 @@ __global__ void __launch_bounds__(RPB * G) k_small_numeric(int nrows, const int 
     unsigned *hist = tags;      // after the accumulation
     const int r = blockIdx.x * RPB + grp;
     __shared__ int s_mark[RPB * G];
-    for (int i = l; i < SLOTS; i += G) { keys[i] = -1; vals[i] = 0.0; }
     for (int i = l; i < NT; i += G) tags[i] = 0xffffffffu;
     s_mark[threadIdx.x] = -1;
     wave_lds_fence();
     if (r >= nrows) return;
     const int row = rows[r];
+    {
+        int lo, hi;
+        row_span<G>(ia[row], ia[row + 1], ja, ib, jb, lo, hi);
+        const int nw = hi >= lo ? ((hi - lo) >> 5) + 1 : 0;
+        if (nw <= SLOTS) {
+            // Rank path: the row's span fits a bitmap in the table's space.
+            // Mark the columns, prefix-count the words, then accumulate every
+            // product at its column's rank (outputs come out sorted: no hash,
+            // compaction or sort).  Key space: SLOTS bitmap words; value space:
+            // the words' prefix counts (first half, as ints) and the outputs
+            // (second half, <= SLOTS/2 of them).
+            unsigned *bmp = reinterpret_cast<unsigned *>(keys);
+            int *pfx = reinterpret_cast<int *>(vals);
+            double *acc = vals + SLOTS / 2;
+            const int64_t o = ic[row];
+            const int nnz = (int)(ic[row + 1] - o);
+            for (int i = l; i < nw; i += G) bmp[i] = 0u;
+            wave_lds_fence();
+            for_products<G, SMALL_U, false, false>(
+                ia[row], ia[row + 1], 0, G, ja, nullptr, ib, jb, nullptr, nullptr, 0, 0, 0, 0,
+                s_mark + grp * G, [&](bool act, int col, double, double, bool) {
+                    if (act) {
+                        const int cc = col - lo;
+                        atomicOr(&bmp[cc >> 5], 1u << (cc & 31));
+                    }
+                });
+            wave_lds_fence();
+            int carry = 0;
+            for (int w0 = 0; w0 < nw; w0 += G) {
+                const int w = w0 + l;
+                const int pc = w < nw ? __popc(bmp[w]) : 0;
+                const int incl = group_incl_scan<G>(pc);
+                if (w < nw) pfx[w] = carry + incl - pc;
+                carry += group_last<G>(incl);
+            }
+            for (int i = l; i < nnz; i += G) acc[i] = 0.0;
+            wave_lds_fence();
+            for_products<G, SMALL_U, false, true>(
+                ia[row], ia[row + 1], 0, G, ja, a, ib, jb, b, nullptr, 0, 0, 0, 0,
+                s_mark + grp * G, [&](bool act, int col, double av, double bv, bool single) {
+                    int rank = 0;
+                    if (act) {
+                        const int cc = col - lo;
+                        const int w = cc >> 5;
+                        rank = pfx[w] + __popc(bmp[w] & ((1u << (cc & 31)) - 1u));
+                    }
+                    ordered_accumulate<FUSED, NT>(act && rank < nnz, rank, av, bv, acc, tags,
+                                                  (unsigned)l, single && b_strict);
+                });
+            wave_lds_fence();
+            for (int i = l; i < nnz; i += G) c[o + i] = acc[i];
+            for (int w0 = 0; w0 < nw; w0 += G) {
+                const int w = w0 + l;
+                if (w < nw) {
+                    unsigned word = bmp[w];
+                    int pos = pfx[w];
+                    while (word) {
+                        const int bit = __ffs(word) - 1;
+                        word &= word - 1;
+                        if (pos < nnz) jc[o + pos] = lo + (w << 5) + bit;
+                        pos++;
+                    }
+                }
+            }
+            return;
+        }
+    }
+    for (int i = l; i < SLOTS; i += G) { keys[i] = -1; vals[i] = 0.0; }
+    wave_lds_fence();
     for_products<G, SMALL_U, false, true>(ia[row], ia[row + 1], 0, G, ja, a, ib, jb, b, nullptr,
                                           0, 0, 0, 0, s_mark + grp * G,
                                           [&](bool act, int col, double av, double bv, bool single) {

@@ -382,3 +382,44 @@ void csr_transpose(int m, int n, const int *ia, const int *ja, const double *a, 
                        pos_out, rowidx, a, ti, tv, tpos);
     TSG_LAUNCH_CHECK();
 }
+
+// ================================================== stable row sort of B
+// Row-sorted copy of a CSR matrix whose rows may be unsorted or hold
+// duplicates: two stable transposes (each a stable radix sort of columns by
+// position) give every row ascending columns with equal columns in their
+// original order; perm[q] is the original position of sorted entry q.  The
+// row pointers are unchanged.
+__global__ void k_compose_pos(int64_t n, const int *__restrict__ outer,
+                              const int *__restrict__ inner, int *__restrict__ perm) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) perm[q] = inner[outer[q]];
+}
+
+__global__ void k_gather_vals(int64_t n, const int *__restrict__ perm,
+                              const double *__restrict__ src, double *__restrict__ dst) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) dst[q] = src[perm[q]];
+}
+
+void csr_sort_rows(int m, int n, const int *ia, const int *ja, int64_t nnz, int *sorted_ja,
+                   int *perm, Arena &keep, Arena &tmp, hipStream_t s) {
+    if (nnz <= 0) return;
+    int *tp = tmp.alloc_n<int>((size_t)n + 1);
+    int *ti = tmp.alloc_n<int>((size_t)nnz);
+    int *pos1 = tmp.alloc_n<int>((size_t)nnz);
+    int *tp2 = tmp.alloc_n<int>((size_t)m + 1);
+    int *pos2 = tmp.alloc_n<int>((size_t)nnz);
+    csr_transpose(m, n, ia, ja, nullptr, nnz, tp, ti, nullptr, tmp, s, pos1);
+    csr_transpose(n, m, tp, ti, nullptr, nnz, tp2, sorted_ja, nullptr, tmp, s, pos2);
+    hipLaunchKernelGGL(k_compose_pos, dim3(grid_for((size_t)nnz, 256)), dim3(256), 0, s, nnz,
+                       pos2, pos1, perm);
+    TSG_LAUNCH_CHECK();
+    (void)keep;
+}
+
+void gather_values(int64_t n, const int *perm, const double *src, double *dst, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_gather_vals, dim3(grid_for((size_t)n, 256)), dim3(256), 0, s, n, perm,
+                       src, dst);
+    TSG_LAUNCH_CHECK();
+}
