@@ -14,7 +14,7 @@ from tests.util import assert_csr_equal
 
 
 def test_descent_finds_separable_minimum():
-    target = {"bin1": 32, "bin2": 128, "bin3": 512, "tile_density": 9.0}
+    target = {"bin1": 32, "bin2": 128, "bin3": 512, "tile_density": 64.0}
 
     def cost(t):
         b1, b2, b3 = t["row_bins"]
@@ -23,7 +23,7 @@ def test_descent_finds_separable_minimum():
                 + abs(np.log2(t["tile_density"] / target["tile_density"])))
 
     best, ms, trials = AT.coordinate_descent(cost)
-    assert best == {"tile_density": 9.0, "row_bins": (32, 128, 512)}
+    assert best == {"tile_density": 64.0, "row_bins": (32, 128, 512)}
     assert ms == pytest.approx(1.0)
     # every measured setting is valid and measured once
     keys = [repr(sorted(t.items())) for t, _ in trials]
@@ -33,7 +33,7 @@ def test_descent_finds_separable_minimum():
 
 def test_descent_keeps_default_without_gain():
     best, ms, trials = AT.coordinate_descent(lambda t: 5.0)
-    assert best == {"tile_density": 6.0, "row_bins": (64, 256, 1024)}
+    assert best == {"tile_density": 32.0, "row_bins": (64, 256, 1024)}
     assert ms == 5.0
 
 
@@ -97,5 +97,5 @@ def test_invalid_tuning_rejected():
             h.set_tuning({"row_bins": bins})
     with pytest.raises(tsg.TsgError):
         h.set_tuning({"tile_density": 0.0})
-    assert h.tuning() == {"tile_density": 6.0, "row_bins": (64, 256, 1024)}
+    assert h.tuning() == {"tile_density": 32.0, "row_bins": (64, 256, 1024)}
     h.close()

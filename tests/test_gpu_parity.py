@@ -296,13 +296,28 @@ def test_reference_host_entry(name):
     assert_csr_equal((ic64, jc64, c64), (g["ic"], g["jc"], g["c_fma"]))
 
 
+def _banded(n, bw, seed=0):
+    rng = np.random.default_rng(seed)
+    rows = np.repeat(np.arange(n), 2 * bw + 1)
+    cols = rows + np.tile(np.arange(-bw, bw + 1), n)
+    ok = (cols >= 0) & (cols < n)
+    return gen.from_coo(n, rows[ok], cols[ok], rng.standard_normal(int(ok.sum())))
+
+
 def test_auto_method_choice(handle):
-    """AUTO takes the tile path for tile-dense A (stencils) and the row path
-    for hypersparse A (R-MAT), with identical results either way."""
+    """AUTO takes the tile path for tile-dense A (a band of half-width 16:
+    ~100 entries per 16x16 tile) and the row path for a 5-point Laplacian
+    (~18 per tile) and hypersparse A (R-MAT), with identical results either
+    way."""
+    handle.set_tuning({"tile_density": 32.0})
+    D = _banded(3000, 16, seed=2)
+    got = run(handle, 3000, 3000, 3000, D, D, tsg.METHOD_AUTO, same=True)
+    assert handle.method_used == tsg.METHOD_TILE
+    assert_csr_equal(got, O.csr_spgemm(3000, 3000, D, D))
     L = gen.laplacian2d(48)
     n = 48 * 48
     got = run(handle, n, n, n, L, L, tsg.METHOD_AUTO, same=True)
-    assert handle.method_used == tsg.METHOD_TILE
+    assert handle.method_used == tsg.METHOD_ROW
     assert_csr_equal(got, O.csr_spgemm(n, n, L, L))
     R = gen.rmat(11, 20_000, seed=4)
     RT = gen.transpose(2048, 2048, *R)

@@ -29,7 +29,7 @@ from typing import Callable, Dict, List, Optional
 
 import numpy as np
 
-DEFAULT = {"tile_density": 6.0, "row_bins": (64, 256, 1024)}
+DEFAULT = {"tile_density": 32.0, "row_bins": (64, 256, 1024)}
 
 # candidate values of each coordinate (validity: b1 <= b2 <= b3, b1 <= 64,
 # b2 <= 256, b3 <= 1024)
@@ -37,7 +37,7 @@ SPACE = {
     "bin3": (256, 512, 1024),
     "bin2": (64, 128, 256),
     "bin1": (16, 32, 64),
-    "tile_density": (3.0, 4.5, 6.0, 9.0, 1e9),
+    "tile_density": (8.0, 16.0, 32.0, 64.0, 1e9),
 }
 
 

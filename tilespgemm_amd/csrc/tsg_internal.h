@@ -218,7 +218,7 @@ struct tsg_handle_s {
     int method = TSG_METHOD_AUTO;
     int fp_mode = TSG_FP_FUSED;
     int method_used = TSG_METHOD_AUTO;
-    double tile_density = 6.0;   // AUTO: tile path from this many entries per tile
+    double tile_density = 32.0;  // AUTO: tile path from this many entries per tile
     RowTuning row_tune;
     Arena ws;       // plan of the last symbolic phase
     Arena bws;      // batch layout (outlives the plan's reset)
