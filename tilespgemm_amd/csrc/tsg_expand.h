@@ -18,6 +18,73 @@ __device__ __forceinline__ int brange_lo(const int *__restrict__ bptr, int strid
     return lo;
 }
 
+// Products of one group of G A entries: lane l's A entry contributes the B
+// positions [lo, lo + cnt) with value a.  Batches of G products are handed to
+// f in product order (see for_products); bofs/bstep select every bstep-th
+// batch starting at bofs.  s_mark: the group's G ints of LDS, all -1 on entry
+// and on return.
+template <int G, int U, bool VALUES, class F>
+__device__ __forceinline__ void expand_group(int lo, int cnt, double a,
+                                             const int *__restrict__ jb,
+                                             const double *__restrict__ bval, int *s_mark,
+                                             F &&f, int bofs = 0, int bstep = 1) {
+    static_assert(G >= 16, "segment lookup scans 16, 32 or 64 lanes");
+    const int l = __lane_id() & (G - 1);
+    const int gbase = __lane_id() & ~(G - 1);
+    const unsigned long long gmask = G == 64 ? ~0ull : ((1ull << G) - 1);
+    const int incl = group_incl_scan<G>(cnt);
+    const int total = group_last<G>(incl);
+    const int start = incl - cnt;
+    // position in B of product q of source lane s: off_s + q
+    const int off = lo - start;
+    for (int base = bofs * G; base < total; base += G * U * bstep) {
+        int e[U], sl[U];
+        bool act[U];
+        double as[U];
+        // batches of this chunk holding products: the rest are skipped
+        // (uniform over the group; with G == 64, over the wave)
+        const int nb = (total - base + G * bstep - 1) / (G * bstep);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            act[u] = false;
+            as[u] = 0.0;
+            e[u] = 0;
+            sl[u] = 0;
+            if (u < nb) {
+                const int qb = base + u * G * bstep;
+                const int q = qb + l;
+                act[u] = q < total;
+                wave_lds_fence();
+                if (cnt > 0 && start < qb + G && incl > qb)
+                    s_mark[start > qb ? start - qb : 0] = l;
+                wave_lds_fence();
+                const int v = s_mark[l];
+                s_mark[l] = -1;
+                const int s = dpp_incl_max<G>(v);
+                as[u] = VALUES ? __shfl(a, s, G) : 0.0;
+                e[u] = __shfl(off, s, G) + q;
+                sl[u] = s;
+            }
+        }
+        int col[U];
+        double bv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            col[u] = act[u] ? jb[e[u]] : 0;
+            bv[u] = (VALUES && act[u]) ? bval[e[u]] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (base + u * G * bstep < total) {
+                const int s0 = group_first<G>(sl[u]);
+                const unsigned long long diff = __ballot(act[u] && sl[u] != s0);
+                const bool single = ((diff >> gbase) & gmask) == 0;
+                f(act[u], col[u], as[u], bv[u], single);
+            }
+        }
+    }
+}
+
 // Visits the products of the A entries [pa, pe) of one row in A-row order,
 // G lanes at a time, optionally restricted to B columns [c0, c1).  The
 // A entries are taken in groups of G starting at pa + gofs, stride gstride.
@@ -47,8 +114,6 @@ __device__ __forceinline__ void for_products(int pa, int pe, int gofs, int gstri
                                              int bstep = 1) {
     static_assert(G >= 16, "segment lookup scans 16, 32 or 64 lanes");
     const int l = __lane_id() & (G - 1);
-    const int gbase = __lane_id() & ~(G - 1);
-    const unsigned long long gmask = G == 64 ? ~0ull : ((1ull << G) - 1);
     // Software pipeline over groups of A entries: the A entry (t, a) of group
     // g+2 and the B range of group g+1 are loaded while group g's products
     // are processed.
@@ -87,58 +152,7 @@ __device__ __forceinline__ void for_products(int pa, int pe, int gofs, int gstri
         double a2;
         load_range(t1, lo_n, hi_n);                 // next group's B ranges
         load_t(g + 2 * gstride, t2, a2);            // group after next
-        const int cnt = hi - lo;
-        const int incl = group_incl_scan<G>(cnt);
-        const int total = group_last<G>(incl);
-        const int start = incl - cnt;
-        // position in B of product q of source lane s: off_s + q
-        const int off = lo - start;
-        for (int base = bofs * G; base < total; base += G * U * bstep) {
-            int e[U], sl[U];
-            bool act[U];
-            double as[U];
-            // batches of this chunk holding products: the rest are skipped
-            // (uniform over the group; with G == 64, over the wave)
-            const int nb = (total - base + G * bstep - 1) / (G * bstep);
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                act[u] = false;
-                as[u] = 0.0;
-                e[u] = 0;
-                sl[u] = 0;
-                if (u < nb) {
-                    const int qb = base + u * G * bstep;
-                    const int q = qb + l;
-                    act[u] = q < total;
-                    wave_lds_fence();
-                    if (cnt > 0 && start < qb + G && incl > qb)
-                        s_mark[start > qb ? start - qb : 0] = l;
-                    wave_lds_fence();
-                    const int v = s_mark[l];
-                    s_mark[l] = -1;
-                    const int s = dpp_incl_max<G>(v);
-                    as[u] = VALUES ? __shfl(a, s, G) : 0.0;
-                    e[u] = __shfl(off, s, G) + q;
-                    sl[u] = s;
-                }
-            }
-            int col[U];
-            double bv[U];
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                col[u] = act[u] ? jb[e[u]] : 0;
-                bv[u] = (VALUES && act[u]) ? bval[e[u]] : 0.0;
-            }
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                if (base + u * G * bstep < total) {
-                    const int s0 = group_first<G>(sl[u]);
-                    const unsigned long long diff = __ballot(act[u] && sl[u] != s0);
-                    const bool single = ((diff >> gbase) & gmask) == 0;
-                    f(act[u], col[u], as[u], bv[u], single);
-                }
-            }
-        }
+        expand_group<G, U, VALUES>(lo, hi - lo, a, jb, bval, s_mark, f, bofs, bstep);
         lo = lo_n;
         hi = hi_n;
         t1 = t2;

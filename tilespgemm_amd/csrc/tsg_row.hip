@@ -40,14 +40,12 @@ constexpr int BIG_PASS_SHIFT = 20;     // one symbolic pass covers 2^20 columns
 constexpr int BIG_T = 1024;            // threads of the big symbolic kernel
 constexpr int ITEM_CAP = 1024;         // outputs per item
 constexpr int ITEM_SLOTS = 2048;       // hash slots per item (load <= 1/2)
-constexpr int ITEM_WBITS_SHIFT = 15;   // item width <= 32768 columns
+constexpr int ITEM_WBITS_SHIFT = 14;   // item width <= 16384 columns
 constexpr int ITEM_WORDS = (1 << ITEM_WBITS_SHIFT) / 32;
 // items are whole sub-blocks, and one sub-block can hold 1 << SUB_SHIFT outputs
 static_assert(ITEM_CAP >= (1 << SUB_SHIFT), "an item must hold a full sub-block");
 constexpr int ITEM_TAGS = 256;
 constexpr int ITEM_WAVES = 1;          // waves per block (one item per wave)
-constexpr int ITEM_BLOCKS_PER_CU = 6;  // LDS-bound residency (~25 KB per wave)
-constexpr int RANK_BLOCKS_PER_CU = 9;  // rank-indexed items: 17 KB per wave
 constexpr int RANK_U = 8;
 constexpr int SMALL_U = 2;             // product batches in flight per lane
 constexpr int BIG_SYM_U = 8;
@@ -732,6 +730,96 @@ struct RankSmem {
     int mark[64];                 // segment marks of the product expansion
 };
 
+// Rank table of one item: the item's bitmap words (<= ITEM_WORDS, every
+// load issued before the first is used, bits at or past c1 cleared) become
+// {word, outputs before it} pairs in S.words; the item's first cnt values
+// are zeroed.  Returns cnt, the item's output count.
+__device__ __forceinline__ int item_rank_table(RankSmem &S, const unsigned *__restrict__ src,
+                                               int c0, int c1) {
+    constexpr int WCH = ITEM_WORDS / 64;
+    const int l = __lane_id();
+    const int nwords = (c1 - c0 + 31) >> 5;
+    unsigned wv[WCH];
+#pragma unroll
+    for (int j = 0; j < WCH; j++) {
+        const int w = j * 64 + l;
+        wv[j] = w < nwords ? src[w] : 0u;
+    }
+    // the 64-word chunks are scanned independently
+    int incl[WCH], tot[WCH];
+#pragma unroll
+    for (int j = 0; j < WCH; j++) {
+        const int w = j * 64 + l;
+        const int lastbits = (c1 - c0) - (w << 5);
+        if (w < nwords && lastbits < 32) wv[j] &= (1u << lastbits) - 1u;
+        incl[j] = wave_incl_scan(__popc(wv[j]));
+        tot[j] = group_last<64>(incl[j]);
+    }
+    int cnt = 0;
+#pragma unroll
+    for (int j = 0; j < WCH; j++) {
+        const int w = j * 64 + l;
+        if (w < nwords)
+            S.words[w] = make_uint2(wv[j], (unsigned)(cnt + incl[j] - __popc(wv[j])));
+        cnt += tot[j];
+    }
+    for (int i = l; i < cnt; i += 64) S.vals[i] = 0.0;
+    wave_lds_fence();
+    return cnt;
+}
+
+// Writes an item's cnt values and their columns to C at offset o: values
+// straight from LDS, columns expanded from the rank table into LDS (over the
+// values, once written) and then stored coalesced.
+__device__ __forceinline__ void item_emit(RankSmem &S, int cnt, int c0, int c1, int64_t o,
+                                          int *__restrict__ jc, double *__restrict__ c) {
+    constexpr int WCH = ITEM_WORDS / 64;
+    const int l = __lane_id();
+    const int nwords = (c1 - c0 + 31) >> 5;
+    wave_lds_fence();
+    for (int i = l; i < cnt; i += 64) c[o + i] = S.vals[i];
+    wave_lds_fence();
+    int *cols = reinterpret_cast<int *>(S.vals);
+#pragma unroll
+    for (int j = 0; j < WCH; j++) {
+        const int w = j * 64 + l;
+        if (w < nwords) {
+            const uint2 wd = S.words[w];
+            int pos = (int)wd.y;
+            unsigned word = wd.x;
+            while (word) {
+                const int bit = __ffs(word) - 1;
+                word &= word - 1;
+                cols[pos++] = c0 + (w << 5) + bit;
+            }
+        }
+    }
+    wave_lds_fence();
+    for (int i = l; i < cnt; i += 64) jc[o + i] = cols[i];
+    wave_lds_fence();
+}
+
+// Accumulates one batch of products into the item's rank-ordered values:
+// a batch from one A entry (distinct columns when B's rows are strictly
+// ascending) adds directly, any other batch by tag rounds in lane order.
+template <bool FUSED>
+__device__ __forceinline__ void item_accumulate(RankSmem &S, int c0, bool act, int col,
+                                                double av, double bv, bool single, int b_strict) {
+    int rank = -1;
+    if (act) {
+        const int cc = col - c0;
+        const uint2 wd = S.words[cc >> 5];
+        rank = (int)wd.y + __popc(wd.x & ((1u << (cc & 31)) - 1u));
+    }
+    if (single && b_strict) {
+        if (rank >= 0) S.vals[rank] = acc_step<FUSED>(S.vals[rank], av, bv);
+        wave_lds_fence();
+        return;
+    }
+    ordered_accumulate<FUSED, ITEM_TAGS>(rank >= 0, rank, av, bv, S.vals, S.tags,
+                                         (unsigned)__lane_id(), false);
+}
+
 template <bool FUSED>
 __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric_rank(
     int64_t nitems, const int4 *__restrict__ items, const int64_t *__restrict__ item_off,
@@ -747,84 +835,44 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric_rank(
     S.mark[l] = -1;
     const int64_t wave_id = (int64_t)blockIdx.x * ITEM_WAVES + (threadIdx.x >> 6);
     const int64_t nwaves = (int64_t)gridDim.x * ITEM_WAVES;
-    constexpr int WCH = ITEM_WORDS / 64;
-    for (int64_t it = wave_id; it < nitems; it += nwaves) {
-        const int4 item = items[it];
-        const int row = item.x, big = item.w;
-        const unsigned *src = rowbits + sb_off[big] * (1 << (SUB_SHIFT - 5)) +
-                              ((item.y >> 5) - (span_sb0[big] << (SUB_SHIFT - 5)));
-        const int c0 = item.y, c1 = item.z;
-        const int nwords = (c1 - c0 + 31) >> 5;
-        // the item's bitmap words (<= ITEM_WORDS): every load is issued before
-        // the first is used
-        unsigned wv[WCH];
-#pragma unroll
-        for (int j = 0; j < WCH; j++) {
-            const int w = j * 64 + l;
-            wv[j] = w < nwords ? src[w] : 0u;
-        }
-        // rank table: the 64-word chunks are scanned independently
-        int incl[WCH], tot[WCH];
-#pragma unroll
-        for (int j = 0; j < WCH; j++) {
-            const int w = j * 64 + l;
-            const int lastbits = (c1 - c0) - (w << 5);
-            if (w < nwords && lastbits < 32) wv[j] &= (1u << lastbits) - 1u;
-            incl[j] = wave_incl_scan(__popc(wv[j]));
-            tot[j] = group_last<64>(incl[j]);
-        }
-        int cnt = 0;
-#pragma unroll
-        for (int j = 0; j < WCH; j++) {
-            const int w = j * 64 + l;
-            if (w < nwords)
-                S.words[w] = make_uint2(wv[j], (unsigned)(cnt + incl[j] - __popc(wv[j])));
-            cnt += tot[j];
-        }
-        for (int i = l; i < cnt; i += 64) S.vals[i] = 0.0;
-        wave_lds_fence();
-        for_products<64, RANK_U, true, true>(
-            ia[row], ia[row + 1], 0, 64, ja, a, ib, jb, b, bptr, bstride, bs, c0, c1, S.mark,
-            [&](bool act, int col, double av, double bv, bool single) {
-                int rank = -1;
-                if (act) {
-                    const int cc = col - c0;
-                    const uint2 wd = S.words[cc >> 5];
-                    rank = (int)wd.y + __popc(wd.x & ((1u << (cc & 31)) - 1u));
-                }
-                if (single && b_strict) {
-                    if (rank >= 0) S.vals[rank] = acc_step<FUSED>(S.vals[rank], av, bv);
-                    wave_lds_fence();
-                    return;
-                }
-                ordered_accumulate<FUSED, ITEM_TAGS>(rank >= 0, rank, av, bv, S.vals, S.tags,
-                                                     (unsigned)l, false);
-            });
-        wave_lds_fence();
-        const int64_t o = item_off[it];
-        for (int i = l; i < cnt; i += 64) c[o + i] = S.vals[i];
-        // columns: expanded from the words into LDS (over the values, which are
-        // written), then stored coalesced
-        wave_lds_fence();
-        int *cols = reinterpret_cast<int *>(S.vals);
-#pragma unroll
-        for (int j = 0; j < WCH; j++) {
-            const int w = j * 64 + l;
-            if (w < nwords) {
-                const uint2 wd = S.words[w];
-                int pos = (int)wd.y;
-                unsigned word = wd.x;
-                while (word) {
-                    const int bit = __ffs(word) - 1;
-                    word &= word - 1;
-                    cols[pos++] = c0 + (w << 5) + bit;
-                }
-            }
-        }
-        wave_lds_fence();
-        for (int i = l; i < cnt; i += 64) jc[o + i] = cols[i];
-        wave_lds_fence();
+    // the next item's descriptor is loaded when an item starts and its row
+    // bounds and bitmap address while the item's outputs are stored, so an
+    // item's first product loads need no chain of dependent loads
+    int4 item = make_int4(0, 0, 0, 0);
+    int pa = 0, pe = 0;
+    const unsigned *src = rowbits;
+    auto locate = [&]() {
+        pa = ia[item.x];
+        pe = ia[item.x + 1];
+        src = rowbits + sb_off[item.w] * (1 << (SUB_SHIFT - 5)) +
+              ((item.y >> 5) - (span_sb0[item.w] << (SUB_SHIFT - 5)));
+    };
+    if (wave_id < nitems) {
+        item = items[wave_id];
+        locate();
     }
+    for (int64_t it = wave_id; it < nitems; it += nwaves) {
+        const int c0 = item.y, c1 = item.z, ipa = pa, ipe = pe;
+        const unsigned *isrc = src;
+        const int64_t o = item_off[it];
+        const int64_t next = it + nwaves;
+        if (next < nitems) item = items[next];
+        const int cnt = item_rank_table(S, isrc, c0, c1);
+        for_products<64, RANK_U, true, true>(
+            ipa, ipe, 0, 64, ja, a, ib, jb, b, bptr, bstride, bs, c0, c1, S.mark,
+            [&](bool act, int col, double av, double bv, bool single) {
+                item_accumulate<FUSED>(S, c0, act, col, av, bv, single, b_strict);
+            });
+        if (next < nitems) locate();
+        item_emit(S, cnt, c0, c1, o, jc, c);
+    }
+}
+
+// One block per item (the hardware dispatcher hands the next item to
+// whichever CU frees a slot first, so no CU idles behind a long tail); the
+// item kernels keep a grid-stride loop for item counts beyond the grid limit.
+static unsigned item_grid(int64_t nitems) {
+    return (unsigned)std::min<int64_t>((nitems + ITEM_WAVES - 1) / ITEM_WAVES, int64_t(1) << 30);
 }
 
 // Raises a kernel's dynamic LDS limit once per process (later calls, e.g.
@@ -1034,8 +1082,7 @@ static void row_numeric_t(RowPlan &p, const CsrIn &A, const CsrIn &B, const int6
         constexpr size_t sm = sizeof(RankSmem) * ITEM_WAVES;
         auto kern = k_big_numeric_rank<FUSED>;
         set_smem(kern, sm);
-        const unsigned grid = (unsigned)std::min<int64_t>(
-            (p.nitems + ITEM_WAVES - 1) / ITEM_WAVES, (int64_t)cus * RANK_BLOCKS_PER_CU);
+        const unsigned grid = item_grid(p.nitems);
         hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * ITEM_WAVES), sm, s, p.nitems, p.items,
                            p.item_off, p.rowbits, p.span_sb0, p.sb_off, A.ptr, A.idx, A.val, B.ptr,
                            B.idx, B.val, p.bptr, p.nblk + 1, p.blk_shift, d_jc, d_c,
@@ -1047,8 +1094,7 @@ static void row_numeric_t(RowPlan &p, const CsrIn &A, const CsrIn &B, const int6
         constexpr size_t sm = sizeof(ItemSmem) * ITEM_WAVES;
         auto kern = k_big_numeric<FUSED>;
         set_smem(kern, sm);
-        const unsigned grid = (unsigned)std::min<int64_t>(
-            (p.nitems + ITEM_WAVES - 1) / ITEM_WAVES, (int64_t)cus * ITEM_BLOCKS_PER_CU);
+        const unsigned grid = item_grid(p.nitems);
         hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * ITEM_WAVES), sm, s, p.nitems, p.items,
                            p.item_off, A.ptr, A.idx, A.val, B.ptr, B.idx, B.val, p.bptr,
                            p.nblk + 1, p.blk_shift, d_jc, d_c, (int)p.b_strict);
