@@ -161,87 +161,6 @@ __device__ __forceinline__ void for_products(int pa, int pe, int gofs, int gstri
     }
 }
 
-// Marks the B columns of the products of A entries [pa, pe) (restricted to
-// [c0, c1)) in an LDS bitmap.  Product order does not matter here, so each
-// lane takes U consecutive products of a window of 64*U: same-word bits of a
-// lane are OR-ed in registers and the lane issues one atomic per word run,
-// which keeps runs of close columns (a dense B row) from serializing on one
-// LDS address.  Waves share every group of A entries by taking every
-// bstep-th window, starting at bofs.
-// s_seg is a per-wave LDS scratch of 128 ints (segment ends and offsets are
-// read from it inside the divergent segment walk, where lane shuffles could
-// name inactive lanes).
-template <int U>
-__device__ __forceinline__ void mark_products(int pa, int pe, const int *__restrict__ ja,
-                                              const int *__restrict__ ib,
-                                              const int *__restrict__ jb, int c0, int c1,
-                                              unsigned *bits, int bofs, int bstep,
-                                              int *s_seg) {
-    const int l = __lane_id();
-    for (int g = pa; g < pe; g += 64) {
-        const int p = g + l;
-        int lo = 0, hi = 0;
-        if (p < pe) {
-            const int t = ja[p];
-            lo = ib[t];
-            hi = ib[t + 1];
-        }
-        const int cnt = hi - lo;
-        const int incl = wave_incl_scan(cnt);
-        const int total = group_last<64>(incl);
-        const int off = lo - (incl - cnt);
-        wave_lds_fence();
-        s_seg[l] = incl;
-        s_seg[64 + l] = off;
-        wave_lds_fence();
-        for (int base = bofs * 64 * U; base < total; base += 64 * U * bstep) {
-            const int q0 = base + l * U;
-            int s = 0;
-#pragma unroll
-            for (int step = 32; step > 0; step >>= 1) {
-                int probe = __shfl(incl, s + step - 1, 64);
-                if (probe <= q0) s += step;
-            }
-            // walk U consecutive products starting in segment s
-            int col[U];
-            int sidx = s < 63 ? s : 63;
-            int send = s_seg[sidx];
-            int soff = s_seg[64 + sidx];
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                const int q = q0 + u;
-                col[u] = -1;
-                if (q < total) {
-                    while (q >= send && sidx < 63) {   // next non-empty segment
-                        sidx++;
-                        send = s_seg[sidx];
-                        soff = s_seg[64 + sidx];
-                    }
-                    col[u] = soff + q;                  // B position, loaded below
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < U; u++) col[u] = col[u] >= 0 ? jb[col[u]] : -1;
-            int w = -1;
-            unsigned acc = 0u;
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                const int c = col[u];
-                if (c >= c0 && c < c1) {
-                    const int cc = c - c0;
-                    if ((cc >> 5) != w) {
-                        if (w >= 0) atomicOr(&bits[w], acc);
-                        w = cc >> 5;
-                        acc = 0u;
-                    }
-                    acc |= 1u << (cc & 31);
-                }
-            }
-            if (w >= 0) atomicOr(&bits[w], acc);
-        }
-    }
-}
-
 // Insert col into an LDS linear-probing table; returns the slot and whether
 // this call created the key.  At most SLOTS probes: a full table (which the
 // symbolic phase's sizing rules out) yields -1 instead of spinning.

@@ -476,7 +476,7 @@ __global__ void __launch_bounds__(BIG_T) k_big_symbolic(int nbig, const int *__r
     extern __shared__ __align__(16) unsigned char smem[];
     unsigned *bits = reinterpret_cast<unsigned *>(smem);
     __shared__ int s_total[BIG_T / 64];
-    __shared__ int s_seg[BIG_T / 64][128];
+    __shared__ int s_mark[BIG_T / 64][64];
     const int wave = threadIdx.x >> 6;
     constexpr int NW = BIG_T / 64;
     for (int r = blockIdx.x; r < nbig; r += gridDim.x) {
@@ -493,10 +493,24 @@ __global__ void __launch_bounds__(BIG_T) k_big_symbolic(int nbig, const int *__r
             for (int i = threadIdx.x; i < words; i += BIG_T) bits[i] = 0u;
             __syncthreads();
             // every wave walks every group of A entries and takes every NW-th
-            // window of its products: the row's work splits evenly over the
-            // block however few A entries the row has
-            mark_products<BIG_SYM_U>(pa, pe, ja, ib, jb, pass0, pass1, bits, wave, NW,
-                                     s_seg[wave]);
+            // batch of 64 products (lane-contiguous, so a gather instruction
+            // reads few lines): the row's work splits evenly over the block
+            // however few A entries the row has
+            {
+                int *mark = s_mark[wave];
+                mark[threadIdx.x & 63] = -1;
+                wave_lds_fence();
+                for_products<64, BIG_SYM_U, false, false>(
+                    pa, pe, 0, 64, ja, nullptr, ib, jb, nullptr, nullptr, 0, 0, pass0, pass1,
+                    mark,
+                    [&](bool act, int col, double, double, bool) {
+                        if (act && col >= pass0 && col < pass1) {
+                            const int cc = col - pass0;
+                            atomicOr(&bits[cc >> 5], 1u << (cc & 31));
+                        }
+                    },
+                    wave, NW);
+            }
             __syncthreads();
             const int64_t sbase = so + ((pass0 - span0) >> SUB_SHIFT);
             if (rowbits) {
