@@ -106,6 +106,26 @@ def test_row_path_big_rows_and_items(handle):
     assert_csr_equal(got, want)
 
 
+def test_big_row_items_in_column_order(handle):
+    """Items run in first-column order (forced for any item count) give the
+    oracle's bits: wide rows with many items, and R-MAT A*A^T with
+    duplicate-free B (single-entry batches) and ordered accumulation."""
+    assert handle.item_order() == 1 << 20
+    handle.set_item_order(0)
+    try:
+        m, n, k, A, B = _wide_case()
+        got = run(handle, m, n, k, A, B, tsg.METHOD_ROW)
+        assert_csr_equal(got, O.csr_spgemm(m, k, A, B, fused=True))
+        R = gen.rmat(12, 60_000, seed=3)
+        RT = gen.transpose(4096, 4096, *R)
+        got = run(handle, 4096, 4096, 4096, R, RT, tsg.METHOD_ROW)
+        assert_csr_equal(got, O.csr_spgemm(4096, 4096, R, RT))
+    finally:
+        handle.set_item_order(1 << 20)
+    with pytest.raises(tsg.TsgError):
+        handle.set_item_order(-1)
+
+
 def test_row_path_more_than_2p20_columns(handle):
     """k > 2^20: the big-row symbolic bitmap runs in several column passes."""
     rng = np.random.default_rng(4)

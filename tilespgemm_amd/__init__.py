@@ -66,6 +66,8 @@ SIGNATURES = {
     "tsg_set_tile_density": (_i, [_vp, C.c_double]),
     "tsg_set_row_bins": (_i, [_vp, _i, _i, _i]),
     "tsg_get_tuning": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "tsg_set_item_order": (_i, [_vp, _i64]),
+    "tsg_get_item_order": (_i, [_vp, _vp]),
     "tsg_read_mtx": (_i, [C.c_char_p, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "tsg_free": (None, [_vp]),
     "tsg_server_create": (_i, [_vp, _i, _vp]),
@@ -345,6 +347,16 @@ class Handle:
         b = [C.c_int(0) for _ in range(3)]
         _check(load().tsg_get_tuning(self._h, C.byref(d), *[C.byref(x) for x in b]), self._h)
         return {"tile_density": d.value, "row_bins": tuple(x.value for x in b)}
+
+    def set_item_order(self, min_items: int):
+        """Big-row items run in first-column order from min_items items on
+        (tsg_set_item_order)."""
+        _check(load().tsg_set_item_order(self._h, int(min_items)), self._h)
+
+    def item_order(self) -> int:
+        v = C.c_int64(0)
+        _check(load().tsg_get_item_order(self._h, C.byref(v)), self._h)
+        return v.value
 
     def set_fp_mode(self, fp_mode: int):
         _check(load().tsg_set_fp_mode(self._h, fp_mode), self._h)

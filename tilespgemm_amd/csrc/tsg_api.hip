@@ -146,7 +146,21 @@ int tsg_set_row_bins(tsg_handle_t h, int bin1, int bin2, int bin3) {
     if (!h) return TSG_ERR_INVALID;
     if (bin1 < 0 || bin1 > 64 || bin2 < bin1 || bin2 > 256 || bin3 < bin2 || bin3 > 1024)
         return TSG_ERR_INVALID;
-    h->row_tune = RowTuning{bin1, bin2, bin3};
+    h->row_tune.bin1 = bin1;
+    h->row_tune.bin2 = bin2;
+    h->row_tune.bin3 = bin3;
+    return TSG_OK;
+}
+
+int tsg_set_item_order(tsg_handle_t h, int64_t min_items) {
+    if (!h || min_items < 0) return TSG_ERR_INVALID;
+    h->row_tune.item_sort_min = min_items;
+    return TSG_OK;
+}
+
+int tsg_get_item_order(tsg_handle_t h, int64_t *min_items) {
+    if (!h || !min_items) return TSG_ERR_INVALID;
+    *min_items = h->row_tune.item_sort_min;
     return TSG_OK;
 }
 

@@ -103,6 +103,7 @@ struct StageTimer {
 // hash kernels, the rest to the big-row path.
 struct RowTuning {
     int bin1 = 64, bin2 = 256, bin3 = 1024;
+    int64_t item_sort_min = int64_t(1) << 20;  // big-row items column-ordered from this many
 };
 
 struct RowPlan {
@@ -131,7 +132,7 @@ struct RowPlan {
     // items
     int64_t nitems = 0;
     int64_t *item_cnt = nullptr;     // [nbig+1] items per big row -> offsets
-    int4 *items = nullptr;           // {row, c0, c1, big-row index}
+    int4 *items = nullptr;           // {row, c0, c1, big-row index}, by c0
     // per big row span bitmaps kept from the symbolic phase (rank mode)
     bool use_rank = false;
     unsigned *rowbits = nullptr;     // [nsb_total * 32] words
@@ -254,5 +255,7 @@ void csr_transpose(int m, int n, const int *ia, const int *ja, const double *a,
 void csr_sort_rows(int m, int n, const int *ia, const int *ja, int64_t nnz, int *sorted_ja,
                    int *perm, Arena &keep, Arena &tmp, hipStream_t s);
 void gather_values(int64_t n, const int *perm, const double *src, double *dst, hipStream_t s);
+void sort_order_i32(const int *keys, int64_t n, int end_bit, int *order, Arena &ws,
+                    hipStream_t s);
 void csr_row_products(int m, const int *ia, const int *ja, int64_t nnz, const int *ib,
                       int64_t *rowprod, Arena &ws, hipStream_t s);

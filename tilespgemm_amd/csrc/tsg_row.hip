@@ -651,6 +651,30 @@ __global__ void __launch_bounds__(256) k_items(int nbig, const int *__restrict__
     if (!FILL && l == 0) item_cnt[r] = wbase;
 }
 
+// Items in order of their first column: keys, then the gather of the items
+// and their output offsets into the sorted order.  The numeric phase takes
+// items in this order, so the items in flight at any time gather B entries
+// of one narrow column band, which the L2 caches keep, instead of bands
+// spread over all of B.
+__global__ void __launch_bounds__(256) k_item_keys(int64_t n, const int4 *__restrict__ items,
+                                                   int *__restrict__ keys) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) keys[i] = items[i].y >> SUB_SHIFT;
+}
+
+__global__ void __launch_bounds__(256) k_gather_items(int64_t n, const int *__restrict__ order,
+                                                      const int4 *__restrict__ items,
+                                                      const int64_t *__restrict__ off,
+                                                      int4 *__restrict__ items_out,
+                                                      int64_t *__restrict__ off_out) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) {
+        const int i = order[j];
+        items_out[j] = items[i];
+        off_out[j] = off[i];
+    }
+}
+
 // --------------------------------------------------------- big rows: numeric
 // One wave per item (grid-stride over items).  LDS per wave: a key hash with
 // a 16-bit id per slot, values compacted by insertion id (only the item's
@@ -1059,6 +1083,24 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
                            rowsb, k, p.span_sb0, p.sb_off, p.subcnt, d_ic, p.item_cnt, p.items,
                            p.item_off);
         TSG_LAUNCH_CHECK();
+        if (p.nitems > 1 && p.nitems >= tune.item_sort_min && p.nitems < (int64_t(1) << 31)) {
+            const size_t ni = (size_t)p.nitems;
+            int *keys = ws.alloc_n<int>(ni);
+            int *order = ws.alloc_n<int>(ni);
+            hipLaunchKernelGGL(k_item_keys, dim3(grid_for(ni, 256)), dim3(256), 0, s, p.nitems,
+                               p.items, keys);
+            TSG_LAUNCH_CHECK();
+            int end_bit = 1;
+            while (end_bit < 31 && (int64_t(1) << end_bit) <= ((int64_t)k >> SUB_SHIFT)) end_bit++;
+            sort_order_i32(keys, p.nitems, end_bit, order, ws, s);
+            int4 *items_s = ws.alloc_n<int4>(ni + 1);
+            int64_t *off_s = ws.alloc_n<int64_t>(ni + 1);
+            hipLaunchKernelGGL(k_gather_items, dim3(grid_for(ni, 256)), dim3(256), 0, s,
+                               p.nitems, order, p.items, p.item_off, items_s, off_s);
+            TSG_LAUNCH_CHECK();
+            p.items = items_s;
+            p.item_off = off_s;
+        }
         tm.end();
     } else {
         TSG_HIP_TRY(hipMemcpyAsync(&h_tot[0], d_ic + m, sizeof(int64_t), hipMemcpyDeviceToHost, s));

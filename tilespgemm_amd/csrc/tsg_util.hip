@@ -383,6 +383,25 @@ void csr_transpose(int m, int n, const int *ia, const int *ja, const double *a, 
     TSG_LAUNCH_CHECK();
 }
 
+// ====================================================== stable key order
+// order[j] = index of the j-th smallest of keys[0..n) (each in [0, 2^end_bit)),
+// equal keys in index order.
+void sort_order_i32(const int *keys, int64_t n, int end_bit, int *order, Arena &ws,
+                    hipStream_t s) {
+    if (n <= 0) return;
+    TSG_CHECK(n < ((int64_t)1 << 31), TSG_ERR_OVERFLOW, "sort_order_i32: more than 2^31 keys");
+    int *iota = ws.alloc_n<int>((size_t)n);
+    int *keys_out = ws.alloc_n<int>((size_t)n);
+    hipLaunchKernelGGL(k_iota, dim3(grid_for((size_t)n, 256)), dim3(256), 0, s, n, iota);
+    TSG_LAUNCH_CHECK();
+    size_t tbytes = 0;
+    TSG_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, keys, keys_out, iota, order,
+                                                   (int)n, 0, end_bit, s));
+    void *tstore = ws.alloc(tbytes);
+    TSG_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tstore, tbytes, keys, keys_out, iota, order,
+                                                   (int)n, 0, end_bit, s));
+}
+
 // ================================================== stable row sort of B
 // Row-sorted copy of a CSR matrix whose rows may be unsorted or hold
 // duplicates: two stable transposes (each a stable radix sort of columns by
