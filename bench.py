@@ -403,6 +403,9 @@ def main():
     ap.add_argument("--workload", default="rmat20_aat")
     ap.add_argument("--method", default="auto", choices=["auto", "tile", "row"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--item-order", type=int, default=None,
+                    help="tsg_set_item_order threshold (big-row items run in first-column "
+                         "order from this many items; default: the library's)")
     ap.add_argument("--batch", type=int, default=512, help="products in the er_batch workload")
     ap.add_argument("--no-autotune", action="store_true",
                     help="er_batch: keep the default tuning parameters")
@@ -436,6 +439,8 @@ def main():
     dA = tsg.DeviceCsr.from_host(m, n, *A, device=dev)
     method = {"auto": tsg.METHOD_AUTO, "tile": tsg.METHOD_TILE, "row": tsg.METHOD_ROW}[args.method]
     h = tsg.Handle(0, method=method, timing=True)
+    if args.item_order is not None:
+        h.set_item_order(args.item_order)
     if W["aat"]:
         dB = h.transpose(dA)
         k = m
