@@ -14,12 +14,15 @@
 //     bucket table), so the parallel sum equals the sequential one.
 //   * Small rows (<= SMALL_PROD products): one lane group per row, LDS hash
 //     table, output sorted by an in-LDS bitonic sort.
-//   * Big rows: the symbolic phase marks the row's columns in an LDS bitmap
-//     covering up to 2^20 columns and records the count of every 1024-column
-//     sub-block; the numeric phase splits the row into items of <= ITEM_CAP
-//     outputs and <= ITEM_WBITS columns, each accumulated by one wave in an
-//     LDS hash and emitted in column order through an LDS bitmap.  A B block
-//     pointer table gives each B row's sub-range of an item without a search.
+//   * Big rows: the symbolic phase cuts each row's column span into bands of
+//     65536 columns, marks the row's columns of a band in an LDS bitmap and
+//     records the count of every 1024-column sub-block (and, when memory
+//     allows, the span's bitmap); the numeric phase splits the row into items
+//     of <= ITEM_CAP outputs and <= 16384 columns, each accumulated by one
+//     wave into a rank-ordered LDS array (rank from the stored bitmap) and
+//     written in column order.  A B block pointer table gives each B row's
+//     sub-range of a band or item without a search; bands and items run in
+//     order of their first column, so those in flight share a band of B.
 #include "tsg_internal.h"
 #include "tsg_device.h"
 #include "tsg_expand.h"
@@ -36,8 +39,9 @@ constexpr int BIN_BIG = 4;
 constexpr int bin_limit[NBINS] = {0, 64, 256, SMALL_PROD, 0x7fffffff};
 
 constexpr int SUB_SHIFT = 10;          // sub-block = 1024 columns
-constexpr int BIG_PASS_SHIFT = 20;     // one symbolic pass covers 2^20 columns
-constexpr int BIG_T = 1024;            // threads of the big symbolic kernel
+constexpr int BAND_SHIFT = 6;          // symbolic band = 64 sub-blocks (65536 columns)
+constexpr int BAND_T = 256;            // threads per band unit
+constexpr int BAND_WORDS = (1 << (BAND_SHIFT + SUB_SHIFT)) / 32;
 constexpr int ITEM_CAP = 1024;         // outputs per item
 constexpr int ITEM_SLOTS = 2048;       // hash slots per item (load <= 1/2)
 constexpr int ITEM_WBITS_SHIFT = 14;   // item width <= 16384 columns
@@ -95,7 +99,7 @@ __global__ void __launch_bounds__(256) k_bin_scatter(int m, const int64_t *__res
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const int b = i < m ? bin_of(rowprod[i], lim) : -1;
-    if (b == 0) rownnz[i] = 0;
+    if (b == 0 || b == BIN_BIG) rownnz[i] = 0;
     const uint64_t below = (uint64_t(1) << lane) - 1;
     int pos = 0;
     for (int q = 1; q < NBINS; q++) {
@@ -459,91 +463,104 @@ __global__ void __launch_bounds__(256) k_big_span(int nbig, const int *__restric
 }
 
 // -------------------------------------------------------- big rows: symbolic
-// One block per big row (grid-stride).  LDS bitmap over the row's column
-// span, up to 2^20 columns per pass; writes the count of every 1024-column
-// sub-block of the span (subcnt, at the row's span offset) and, when rowbits
-// is given, the span's bitmap.
-__global__ void __launch_bounds__(BIG_T) k_big_symbolic(int nbig, const int *__restrict__ rows,
-                                                        int k, const int *__restrict__ ia,
-                                                        const int *__restrict__ ja,
-                                                        const int *__restrict__ ib,
-                                                        const int *__restrict__ jb,
-                                                        const int *__restrict__ span_sb0,
-                                                        const int64_t *__restrict__ sb_off,
-                                                        uint16_t *__restrict__ subcnt,
-                                                        int64_t *__restrict__ rownnz,
-                                                        unsigned *__restrict__ rowbits) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    unsigned *bits = reinterpret_cast<unsigned *>(smem);
-    __shared__ int s_total[BIG_T / 64];
-    __shared__ int s_mark[BIG_T / 64][64];
-    const int wave = threadIdx.x >> 6;
-    constexpr int NW = BIG_T / 64;
-    for (int r = blockIdx.x; r < nbig; r += gridDim.x) {
+// A big row's span is cut into bands of 64 sub-blocks; a band unit {r, u}
+// covers span sub-blocks [64u, 64u + 64) of big row r.  Units are run in
+// order of their first column (like the numeric items), so the units in
+// flight read B columns of one narrow band, which the L2 caches keep.
+__global__ void __launch_bounds__(256) k_band_count(int nbig, const int64_t *__restrict__ nsb,
+                                                    int64_t *__restrict__ cnt) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nbig) cnt[r] = (nsb[r] + (1 << BAND_SHIFT) - 1) >> BAND_SHIFT;
+}
+
+__global__ void __launch_bounds__(256) k_band_fill(int nbig, const int *__restrict__ span_sb0,
+                                                   const int64_t *__restrict__ off,
+                                                   int2 *__restrict__ units,
+                                                   int *__restrict__ keys) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nbig) return;
+    const int64_t o = off[r], n = off[r + 1] - o;
+    for (int u = 0; u < n; u++) {
+        units[o + u] = make_int2(r, u);
+        keys[o + u] = span_sb0[r] + (u << BAND_SHIFT);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_gather_units(int64_t n, const int *__restrict__ order,
+                                                      const int2 *__restrict__ units,
+                                                      int2 *__restrict__ out) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) out[j] = units[order[j]];
+}
+
+// One block per band unit: LDS bitmap over the band, the row's products in
+// the band (B rows restricted through the block pointers) marked by the 4
+// waves taking every 4th batch of 64 products; then the band's bitmap words
+// (when rowbits is given), its sub-block counts and its share of the row's
+// output count (rownnz of big rows is zero on entry).
+__global__ void __launch_bounds__(BAND_T) k_big_symbolic(
+    int64_t nunits, const int2 *__restrict__ units, const int *__restrict__ rows, int k,
+    const int *__restrict__ ia, const int *__restrict__ ja, const int *__restrict__ ib,
+    const int *__restrict__ jb, const int *__restrict__ bptr, int bstride, int bs,
+    const int *__restrict__ span_sb0, const int64_t *__restrict__ sb_off,
+    uint16_t *__restrict__ subcnt, int64_t *__restrict__ rownnz, unsigned *__restrict__ rowbits) {
+    __shared__ unsigned bits[BAND_WORDS];
+    __shared__ int s_mark[BAND_T / 64][64];
+    __shared__ int s_total[BAND_T / 64];
+    constexpr int NW = BAND_T / 64;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    s_mark[wave][lane] = -1;
+    for (int64_t ui = blockIdx.x; ui < nunits; ui += gridDim.x) {
+        const int2 un = units[ui];
+        const int r = un.x, u = un.y;
         const int row = rows[r];
-        const int pa = ia[row], pe = ia[row + 1];
-        const int64_t so = sb_off[r];
-        const int nsb_row = (int)(sb_off[r + 1] - so);
-        const int span0 = span_sb0[r] << SUB_SHIFT;
-        const int span1 = min(k, (span_sb0[r] + nsb_row) << SUB_SHIFT);
-        int row_total = 0;
-        for (int pass0 = span0; pass0 < span1; pass0 += (1 << BIG_PASS_SHIFT)) {
-            const int pass1 = min(span1, pass0 + (1 << BIG_PASS_SHIFT));
-            const int words = (pass1 - pass0 + 31) >> 5;
-            for (int i = threadIdx.x; i < words; i += BIG_T) bits[i] = 0u;
-            __syncthreads();
-            // every wave walks every group of A entries and takes every NW-th
-            // batch of 64 products (lane-contiguous, so a gather instruction
-            // reads few lines): the row's work splits evenly over the block
-            // however few A entries the row has
-            {
-                int *mark = s_mark[wave];
-                mark[threadIdx.x & 63] = -1;
-                wave_lds_fence();
-                for_products<64, BIG_SYM_U, false, false>(
-                    pa, pe, 0, 64, ja, nullptr, ib, jb, nullptr, nullptr, 0, 0, pass0, pass1,
-                    mark,
-                    [&](bool act, int col, double, double, bool) {
-                        if (act && col >= pass0 && col < pass1) {
-                            const int cc = col - pass0;
-                            atomicOr(&bits[cc >> 5], 1u << (cc & 31));
-                        }
-                    },
-                    wave, NW);
-            }
-            __syncthreads();
-            const int64_t sbase = so + ((pass0 - span0) >> SUB_SHIFT);
-            if (rowbits) {
-                unsigned *dst = rowbits + sbase * (1 << (SUB_SHIFT - 5));
-                for (int i = threadIdx.x; i < words; i += BIG_T) dst[i] = bits[i];
-            }
-            const int nsb = (pass1 - pass0 + (1 << SUB_SHIFT) - 1) >> SUB_SHIFT;
-            // sub-block counts: a sub-block's 32 words are one half-wave, read
-            // consecutively (no bank conflicts) and summed across the half
-            static_assert(SUB_SHIFT - 5 == 5, "a sub-block is 32 bitmap words");
-            int part = 0;
-            const int lane = threadIdx.x & 63;
-            for (int w0 = wave * 64; w0 < (nsb << 5); w0 += BIG_T) {
-                const int w = w0 + lane;
-                int pc = w < words ? __popc(bits[w]) : 0;
-#pragma unroll
-                for (int d = 16; d > 0; d >>= 1) pc += __shfl_xor(pc, d, 64);
-                if ((lane & 31) == 0 && (w >> 5) < nsb) {
-                    subcnt[sbase + (w >> 5)] = (uint16_t)pc;
-                    part += pc;
+        const int64_t so = sb_off[r] + ((int64_t)u << BAND_SHIFT);
+        const int nsb = (int)min((int64_t)1 << BAND_SHIFT, sb_off[r + 1] - so);
+        const int sb0 = span_sb0[r] + (u << BAND_SHIFT);
+        const int c0 = sb0 << SUB_SHIFT;
+        const int c1 = (int)min((int64_t)k, (int64_t)(sb0 + nsb) << SUB_SHIFT);
+        const int words = (c1 - c0 + 31) >> 5;
+        for (int i = threadIdx.x; i < words; i += BAND_T) bits[i] = 0u;
+        __syncthreads();
+        for_products<64, BIG_SYM_U, true, false>(
+            ia[row], ia[row + 1], 0, 64, ja, nullptr, ib, jb, nullptr, bptr, bstride, bs, c0,
+            c1, s_mark[wave],
+            [&](bool act, int col, double, double, bool) {
+                if (act) {
+                    const int cc = col - c0;
+                    atomicOr(&bits[cc >> 5], 1u << (cc & 31));
                 }
-            }
-            part = wave_sum(part);
-            if ((threadIdx.x & 63) == 0) s_total[wave] = part;
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                int t = 0;
-                for (int i = 0; i < NW; i++) t += s_total[i];
-                row_total += t;
-            }
-            __syncthreads();
+            },
+            wave, NW);
+        __syncthreads();
+        if (rowbits) {
+            unsigned *dst = rowbits + so * (1 << (SUB_SHIFT - 5));
+            for (int i = threadIdx.x; i < words; i += BAND_T) dst[i] = bits[i];
         }
-        if (threadIdx.x == 0) rownnz[row] = row_total;
+        // sub-block counts: a sub-block's 32 words are one half-wave, read
+        // consecutively (no bank conflicts) and summed across the half
+        static_assert(SUB_SHIFT - 5 == 5, "a sub-block is 32 bitmap words");
+        int part = 0;
+        for (int w0 = wave * 64; w0 < (nsb << 5); w0 += BAND_T) {
+            const int w = w0 + lane;
+            int pc = w < words ? __popc(bits[w]) : 0;
+#pragma unroll
+            for (int d = 16; d > 0; d >>= 1) pc += __shfl_xor(pc, d, 64);
+            if ((lane & 31) == 0 && (w >> 5) < nsb) {
+                subcnt[so + (w >> 5)] = (uint16_t)pc;
+                part += pc;
+            }
+        }
+        part = wave_sum(part);
+        if (lane == 0) s_total[wave] = part;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int t = 0;
+            for (int i = 0; i < NW; i++) t += s_total[i];
+            if (t) atomicAdd(reinterpret_cast<unsigned long long *>(rownnz + row),
+                             (unsigned long long)t);
+        }
+        __syncthreads();
     }
 }
 
@@ -1002,43 +1019,9 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
     p.nbig = st.s[5] - st.s[4];
     p.sub_shift = SUB_SHIFT;
     if (p.nbig > 0) {
-        tm.begin("row.big_symbolic");
         const int *rowsb = p.bin_rows + st.s[4];
-        // column span of every big row, in sub-blocks, and span offsets
-        p.span_sb0 = ws.alloc_n<int>((size_t)p.nbig + 1);
-        p.sb_off = ws.alloc_n<int64_t>((size_t)p.nbig + 1);
-        int64_t *tmp_sb = ws.alloc_n<int64_t>(scan_tmp_elems((size_t)p.nbig + 1));
-        hipLaunchKernelGGL(k_big_span, dim3(grid_for((size_t)p.nbig * 64, 256)), dim3(256), 0, s,
-                           p.nbig, rowsb, A.ptr, A.idx, B.ptr, B.idx, p.span_sb0, p.sb_off);
-        TSG_LAUNCH_CHECK();
-        exclusive_scan_i64(p.sb_off, p.sb_off, (size_t)p.nbig, tmp_sb, s);
-        int64_t nsb_total = 0;
-        TSG_HIP_TRY(hipMemcpyAsync(&nsb_total, p.sb_off + p.nbig, sizeof(int64_t),
-                                   hipMemcpyDeviceToHost, s));
-        TSG_HIP_TRY(hipStreamSynchronize(s));
-        p.nsb_total = nsb_total;
-        p.subcnt = ws.alloc_n<uint16_t>((size_t)nsb_total + 1);
-        // keep every big row's span bitmap for the rank-indexed numeric phase
-        // when it fits a budget of device memory; else the numeric phase uses
-        // per-item hash tables
-        const size_t bits_bytes = (size_t)nsb_total * ((1u << SUB_SHIFT) / 8);
-        size_t free_b = 0, total_b = 0;
-        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) { (void)hipGetLastError(); free_b = 0; }
-        const size_t budget = std::min<size_t>(size_t(48) << 30, free_b / 10 * 4);
-        p.use_rank = bits_bytes <= budget;
-        p.rowbits = p.use_rank ? ws.alloc_n<unsigned>(bits_bytes / 4 + 1) : nullptr;
-        const int span_max = std::min(k, 1 << BIG_PASS_SHIFT);
-        const size_t smem = sizeof(unsigned) * (size_t)((span_max + 31) / 32);
-        set_smem(k_big_symbolic, smem);
-        const unsigned grid = (unsigned)std::min(p.nbig, cus * 1);
-        hipLaunchKernelGGL(k_big_symbolic, dim3(grid), dim3(BIG_T), smem, s, p.nbig, rowsb, k,
-                           A.ptr, A.idx, B.ptr, B.idx, p.span_sb0, p.sb_off, p.subcnt, p.rownnz,
-                           p.rowbits);
-        TSG_LAUNCH_CHECK();
-        tm.end();
-
-        // B block pointers at the sub-block width (item boundaries then need
-        // no search) while the table stays <= 6 GiB, coarser beyond.
+        // B block pointers at the sub-block width (item and band boundaries
+        // then need no search) while the table stays <= 6 GiB, coarser beyond
         tm.begin("row.bptr");
         int bs = SUB_SHIFT;
         while (bs < 30) {
@@ -1054,6 +1037,63 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
                                B.ptr, B.idx, p.nblk, bs, p.bptr);
             TSG_LAUNCH_CHECK();
         }
+        tm.end();
+
+        tm.begin("row.big_symbolic");
+        // column span of every big row, in sub-blocks, span offsets and the
+        // band units over the spans
+        p.span_sb0 = ws.alloc_n<int>((size_t)p.nbig + 1);
+        p.sb_off = ws.alloc_n<int64_t>((size_t)p.nbig + 1);
+        int64_t *band_off = ws.alloc_n<int64_t>((size_t)p.nbig + 1);
+        int64_t *tmp_sb = ws.alloc_n<int64_t>(scan_tmp_elems((size_t)p.nbig + 1));
+        hipLaunchKernelGGL(k_big_span, dim3(grid_for((size_t)p.nbig * 64, 256)), dim3(256), 0, s,
+                           p.nbig, rowsb, A.ptr, A.idx, B.ptr, B.idx, p.span_sb0, p.sb_off);
+        TSG_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_band_count, dim3(grid_for((size_t)p.nbig, 256)), dim3(256), 0, s,
+                           p.nbig, p.sb_off, band_off);
+        TSG_LAUNCH_CHECK();
+        exclusive_scan_i64(p.sb_off, p.sb_off, (size_t)p.nbig, tmp_sb, s);
+        exclusive_scan_i64(band_off, band_off, (size_t)p.nbig, tmp_sb, s);
+        int64_t tot2[2] = {0, 0};
+        TSG_HIP_TRY(hipMemcpyAsync(&tot2[0], p.sb_off + p.nbig, sizeof(int64_t),
+                                   hipMemcpyDeviceToHost, s));
+        TSG_HIP_TRY(hipMemcpyAsync(&tot2[1], band_off + p.nbig, sizeof(int64_t),
+                                   hipMemcpyDeviceToHost, s));
+        TSG_HIP_TRY(hipStreamSynchronize(s));
+        const int64_t nsb_total = tot2[0], nunits = tot2[1];
+        p.nsb_total = nsb_total;
+        p.subcnt = ws.alloc_n<uint16_t>((size_t)nsb_total + 1);
+        // keep every big row's span bitmap for the rank-indexed numeric phase
+        // when it fits a budget of device memory; else the numeric phase uses
+        // per-item hash tables
+        const size_t bits_bytes = (size_t)nsb_total * ((1u << SUB_SHIFT) / 8);
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) { (void)hipGetLastError(); free_b = 0; }
+        const size_t budget = std::min<size_t>(size_t(48) << 30, free_b / 10 * 4);
+        p.use_rank = bits_bytes <= budget;
+        p.rowbits = p.use_rank ? ws.alloc_n<unsigned>(bits_bytes / 4 + 1) : nullptr;
+        int2 *units = ws.alloc_n<int2>((size_t)nunits + 1);
+        int *ukeys = ws.alloc_n<int>((size_t)nunits + 1);
+        hipLaunchKernelGGL(k_band_fill, dim3(grid_for((size_t)p.nbig, 256)), dim3(256), 0, s,
+                           p.nbig, p.span_sb0, band_off, units, ukeys);
+        TSG_LAUNCH_CHECK();
+        if (nunits > 1 && nunits >= tune.item_sort_min && nunits < (int64_t(1) << 31)) {
+            int *order = ws.alloc_n<int>((size_t)nunits);
+            int end_bit = 1;
+            while (end_bit < 31 && (int64_t(1) << end_bit) <= ((int64_t)k >> SUB_SHIFT)) end_bit++;
+            sort_order_i32(ukeys, nunits, end_bit, order, ws, s);
+            int2 *sorted = ws.alloc_n<int2>((size_t)nunits + 1);
+            hipLaunchKernelGGL(k_gather_units, dim3(grid_for((size_t)nunits, 256)), dim3(256), 0,
+                               s, nunits, order, units, sorted);
+            TSG_LAUNCH_CHECK();
+            units = sorted;
+        }
+        hipLaunchKernelGGL(k_big_symbolic,
+                           dim3((unsigned)std::min<int64_t>(nunits, int64_t(1) << 30)),
+                           dim3(BAND_T), 0, s, nunits, units, rowsb, k, A.ptr, A.idx, B.ptr,
+                           B.idx, p.bptr, p.nblk + 1, p.blk_shift, p.span_sb0, p.sb_off,
+                           p.subcnt, p.rownnz, p.rowbits);
+        TSG_LAUNCH_CHECK();
         tm.end();
     }
 
