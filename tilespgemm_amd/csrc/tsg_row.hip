@@ -152,7 +152,7 @@ __global__ void __launch_bounds__(256) k_small_symbolic(int nrows, const int *__
                                                         const int *__restrict__ jb,
                                                         int64_t *__restrict__ rownnz) {
     constexpr int RPB = 256 / G;
-    __shared__ int s_keys[RPB * SLOTS];
+    __shared__ __align__(16) int s_keys[RPB * SLOTS];
     __shared__ int s_mark[256];
     const int grp = threadIdx.x / G;
     const int l = threadIdx.x & (G - 1);
@@ -169,7 +169,9 @@ __global__ void __launch_bounds__(256) k_small_symbolic(int nrows, const int *__
         // the row's columns fit a bitmap over its span in the table's space:
         // mark them and count the bits (no hashing)
         unsigned *bmp = reinterpret_cast<unsigned *>(keys);
-        for (int i = l; i < nw; i += G) bmp[i] = 0u;
+        uint4 *bmp4 = reinterpret_cast<uint4 *>(keys);
+        const int nw4 = (nw + 3) >> 2;
+        for (int i = l; i < nw4; i += G) bmp4[i] = make_uint4(0u, 0u, 0u, 0u);
         wave_lds_fence();
         for_products<G, SMALL_U, false, false>(ia[row], ia[row + 1], 0, G, ja, nullptr, ib, jb,
                                                nullptr, nullptr, 0, 0, 0, 0, s_mark + grp * G,
@@ -181,7 +183,10 @@ __global__ void __launch_bounds__(256) k_small_symbolic(int nrows, const int *__
                                                });
         wave_lds_fence();
         int cnt = 0;
-        for (int i = l; i < nw; i += G) cnt += __popc(bmp[i]);
+        for (int i = l; i < nw4; i += G) {
+            const uint4 wv = bmp4[i];
+            cnt += __popc(wv.x) + __popc(wv.y) + __popc(wv.z) + __popc(wv.w);
+        }
         cnt = group_sum<G>(cnt);
         if (l == 0) rownnz[row] = cnt;
         return;
@@ -256,12 +261,17 @@ __global__ void __launch_bounds__(RPB * G) k_small_numeric(int nrows, const int 
             // compaction or sort).  Key space: SLOTS bitmap words; value space:
             // the words' prefix counts (first half, as ints) and the outputs
             // (second half, <= SLOTS/2 of them).
+            // The bitmap loops take four words per lane (16-byte LDS accesses;
+            // SLOTS is a multiple of 4, so the rounded-up word count fits).
             unsigned *bmp = reinterpret_cast<unsigned *>(keys);
+            uint4 *bmp4 = reinterpret_cast<uint4 *>(keys);
             int *pfx = reinterpret_cast<int *>(vals);
+            int4 *pfx4 = reinterpret_cast<int4 *>(vals);
             double *acc = vals + SLOTS / 2;
             const int64_t o = ic[row];
             const int nnz = (int)(ic[row + 1] - o);
-            for (int i = l; i < nw; i += G) bmp[i] = 0u;
+            const int nw4 = (nw + 3) >> 2;
+            for (int i = l; i < nw4; i += G) bmp4[i] = make_uint4(0u, 0u, 0u, 0u);
             wave_lds_fence();
             for_products<G, SMALL_U, false, false>(
                 ia[row], ia[row + 1], 0, G, ja, nullptr, ib, jb, nullptr, nullptr, 0, 0, 0, 0,
@@ -273,11 +283,16 @@ __global__ void __launch_bounds__(RPB * G) k_small_numeric(int nrows, const int 
                 });
             wave_lds_fence();
             int carry = 0;
-            for (int w0 = 0; w0 < nw; w0 += G) {
-                const int w = w0 + l;
-                const int pc = w < nw ? __popc(bmp[w]) : 0;
+            for (int q0 = 0; q0 < nw4; q0 += G) {
+                const int q = q0 + l;
+                const uint4 wv = q < nw4 ? bmp4[q] : make_uint4(0u, 0u, 0u, 0u);
+                const int p0 = __popc(wv.x), p1 = __popc(wv.y), p2 = __popc(wv.z);
+                const int pc = p0 + p1 + p2 + __popc(wv.w);
                 const int incl = group_incl_scan<G>(pc);
-                if (w < nw) pfx[w] = carry + incl - pc;
+                if (q < nw4) {
+                    const int b0 = carry + incl - pc;
+                    pfx4[q] = make_int4(b0, b0 + p0, b0 + p0 + p1, b0 + p0 + p1 + p2);
+                }
                 carry += group_last<G>(incl);
             }
             for (int i = l; i < nnz; i += G) acc[i] = 0.0;
@@ -296,15 +311,20 @@ __global__ void __launch_bounds__(RPB * G) k_small_numeric(int nrows, const int 
                 });
             wave_lds_fence();
             for (int i = l; i < nnz; i += G) c[o + i] = acc[i];
-            for (int w0 = 0; w0 < nw; w0 += G) {
-                const int w = w0 + l;
-                if (w < nw) {
-                    unsigned word = bmp[w];
-                    int pos = pfx[w];
+            for (int q = l; q < nw4; q += G) {
+                const uint4 wv = bmp4[q];
+                const int4 pv = pfx4[q];
+                const unsigned ws[4] = {wv.x, wv.y, wv.z, wv.w};
+                const int ps[4] = {pv.x, pv.y, pv.z, pv.w};
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    unsigned word = ws[j];
+                    int pos = ps[j];
+                    const int cb = lo + ((4 * q + j) << 5);
                     while (word) {
                         const int bit = __ffs(word) - 1;
                         word &= word - 1;
-                        if (pos < nnz) jc[o + pos] = lo + (w << 5) + bit;
+                        if (pos < nnz) jc[o + pos] = cb + bit;
                         pos++;
                     }
                 }
