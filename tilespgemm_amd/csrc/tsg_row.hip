@@ -40,7 +40,7 @@ constexpr int bin_limit[NBINS] = {0, 64, 256, SMALL_PROD, 0x7fffffff};
 
 constexpr int SUB_SHIFT = 10;          // sub-block = 1024 columns
 constexpr int BAND_SHIFT = 6;          // symbolic band = 64 sub-blocks (65536 columns)
-constexpr int BAND_T = 256;            // threads per band unit
+constexpr int BAND_T = 128;            // threads per band unit
 constexpr int BAND_WORDS = (1 << (BAND_SHIFT + SUB_SHIFT)) / 32;
 constexpr int ITEM_CAP = 1024;         // outputs per item
 constexpr int ITEM_SLOTS = 2048;       // hash slots per item (load <= 1/2)
@@ -52,7 +52,7 @@ constexpr int ITEM_TAGS = 256;
 constexpr int ITEM_WAVES = 1;          // waves per block (one item per wave)
 constexpr int RANK_U = 8;
 constexpr int SMALL_U = 2;             // product batches in flight per lane
-constexpr int BIG_SYM_U = 8;
+constexpr int BIG_SYM_U = 4;
 constexpr int ITEM_U = 4;
 
 struct BinStarts { int s[NBINS + 1]; };
@@ -514,8 +514,8 @@ __global__ void __launch_bounds__(256) k_gather_units(int64_t n, const int *__re
 }
 
 // One block per band unit: LDS bitmap over the band, the row's products in
-// the band (B rows restricted through the block pointers) marked by the 4
-// waves taking every 4th batch of 64 products; then the band's bitmap words
+// the band (B rows restricted through the block pointers) marked by the 2
+// waves taking every other batch of 64 products; then the band's bitmap words
 // (when rowbits is given), its sub-block counts and its share of the row's
 // output count (rownnz of big rows is zero on entry).
 __global__ void __launch_bounds__(BAND_T) k_big_symbolic(
