@@ -805,21 +805,28 @@ struct RankSmem {
     int mark[64];                 // segment marks of the product expansion
 };
 
-// Rank table of one item: the item's bitmap words (<= ITEM_WORDS, every
-// load issued before the first is used, bits at or past c1 cleared) become
-// {word, outputs before it} pairs in S.words; the item's first cnt values
-// are zeroed.  Returns cnt, the item's output count.
-__device__ __forceinline__ int item_rank_table(RankSmem &S, const unsigned *__restrict__ src,
-                                               int c0, int c1) {
-    constexpr int WCH = ITEM_WORDS / 64;
+// Rank table of one item, in two parts so that other loads can be issued
+// between them: item_words loads the item's bitmap words (<= ITEM_WORDS,
+// every load issued before the first is used); item_rank_build clears bits
+// at or past c1, stores {word, outputs before it} pairs in S.words, zeroes
+// the item's first cnt values and returns cnt, the item's output count.
+constexpr int ITEM_WCH = ITEM_WORDS / 64;
+__device__ __forceinline__ void item_words(const unsigned *__restrict__ src, int c0, int c1,
+                                           unsigned (&wv)[ITEM_WCH]) {
     const int l = __lane_id();
     const int nwords = (c1 - c0 + 31) >> 5;
-    unsigned wv[WCH];
 #pragma unroll
-    for (int j = 0; j < WCH; j++) {
+    for (int j = 0; j < ITEM_WCH; j++) {
         const int w = j * 64 + l;
         wv[j] = w < nwords ? src[w] : 0u;
     }
+}
+
+__device__ __forceinline__ int item_rank_build(RankSmem &S, unsigned (&wv)[ITEM_WCH], int c0,
+                                               int c1) {
+    constexpr int WCH = ITEM_WCH;
+    const int l = __lane_id();
+    const int nwords = (c1 - c0 + 31) >> 5;
     // the 64-word chunks are scanned independently
     int incl[WCH], tot[WCH];
 #pragma unroll
@@ -932,12 +939,51 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric_rank(
         const int64_t o = item_off[it];
         const int64_t next = it + nwaves;
         if (next < nitems) item = items[next];
-        const int cnt = item_rank_table(S, isrc, c0, c1);
-        for_products<64, RANK_U, true, true>(
-            ipa, ipe, 0, 64, ja, a, ib, jb, b, bptr, bstride, bs, c0, c1, S.mark,
-            [&](bool act, int col, double av, double bv, bool single) {
-                item_accumulate<FUSED>(S, c0, act, col, av, bv, single, b_strict);
-            });
+        // the product pipeline of for_products, with the rank table built
+        // after the first groups' A entries and B ranges are issued, so the
+        // bitmap load and that chain of dependent loads overlap
+        auto load_t = [&](int g, int &t, double &av) {
+            const int p = g + l;
+            t = -1;
+            av = 0.0;
+            if (g < ipe && p < ipe) {
+                t = ja[p];
+                av = a[p];
+            }
+        };
+        auto load_range = [&](int t, int &lo, int &hi) {
+            lo = 0;
+            hi = 0;
+            if (t >= 0) {
+                lo = brange_lo(bptr, bstride, bs, jb, t, c0);
+                hi = brange_lo(bptr, bstride, bs, jb, t, c1);
+            }
+        };
+        int t0, t1;
+        double a0, a1;
+        load_t(ipa, t0, a0);
+        load_t(ipa + 64, t1, a1);
+        unsigned wv[ITEM_WCH];
+        item_words(isrc, c0, c1, wv);
+        int lo, hi;
+        load_range(t0, lo, hi);
+        const int cnt = item_rank_build(S, wv, c0, c1);
+        auto f = [&](bool act, int col, double av, double bv, bool single) {
+            item_accumulate<FUSED>(S, c0, act, col, av, bv, single, b_strict);
+        };
+        for (int g = ipa; g < ipe; g += 64) {
+            const double ag = a0;
+            int lo_n, hi_n, t2;
+            double a2;
+            load_range(t1, lo_n, hi_n);
+            load_t(g + 128, t2, a2);
+            expand_group<64, RANK_U, true>(lo, hi - lo, ag, jb, b, S.mark, f);
+            lo = lo_n;
+            hi = hi_n;
+            t1 = t2;
+            a0 = a1;
+            a1 = a2;
+        }
         if (next < nitems) locate();
         item_emit(S, cnt, c0, c1, o, jc, c);
     }
