@@ -808,8 +808,10 @@ struct RankSmem {
 // Rank table of one item, in two parts so that other loads can be issued
 // between them: item_words loads the item's bitmap words (<= ITEM_WORDS,
 // every load issued before the first is used); item_rank_build clears bits
-// at or past c1, stores {word, outputs before it} pairs in S.words, zeroes
-// the item's first cnt values and returns cnt, the item's output count.
+// at or past c1, stores {word, outputs before it} pairs in S.words, writes
+// the item's columns to jc at offset o (expanded into S.vals, then stored
+// coalesced; the stores drain while the products are processed), zeroes the
+// item's first cnt values and returns cnt, the item's output count.
 constexpr int ITEM_WCH = ITEM_WORDS / 64;
 __device__ __forceinline__ void item_words(const unsigned *__restrict__ src, int c0, int c1,
                                            unsigned (&wv)[ITEM_WCH]) {
@@ -823,7 +825,7 @@ __device__ __forceinline__ void item_words(const unsigned *__restrict__ src, int
 }
 
 __device__ __forceinline__ int item_rank_build(RankSmem &S, unsigned (&wv)[ITEM_WCH], int c0,
-                                               int c1) {
+                                               int c1, int64_t o, int *__restrict__ jc) {
     constexpr int WCH = ITEM_WCH;
     const int l = __lane_id();
     const int nwords = (c1 - c0 + 31) >> 5;
@@ -838,46 +840,35 @@ __device__ __forceinline__ int item_rank_build(RankSmem &S, unsigned (&wv)[ITEM_
         tot[j] = group_last<64>(incl[j]);
     }
     int cnt = 0;
+    int *cols = reinterpret_cast<int *>(S.vals);
 #pragma unroll
     for (int j = 0; j < WCH; j++) {
         const int w = j * 64 + l;
-        if (w < nwords)
-            S.words[w] = make_uint2(wv[j], (unsigned)(cnt + incl[j] - __popc(wv[j])));
+        const int base = cnt + incl[j] - __popc(wv[j]);
+        if (w < nwords) S.words[w] = make_uint2(wv[j], (unsigned)base);
+        unsigned word = wv[j];
+        int pos = base;
+        while (word) {
+            const int bit = __ffs(word) - 1;
+            word &= word - 1;
+            cols[pos++] = c0 + (w << 5) + bit;
+        }
         cnt += tot[j];
     }
+    wave_lds_fence();
+    for (int i = l; i < cnt; i += 64) jc[o + i] = cols[i];
+    wave_lds_fence();
     for (int i = l; i < cnt; i += 64) S.vals[i] = 0.0;
     wave_lds_fence();
     return cnt;
 }
 
-// Writes an item's cnt values and their columns to C at offset o: values
-// straight from LDS, columns expanded from the rank table into LDS (over the
-// values, once written) and then stored coalesced.
-__device__ __forceinline__ void item_emit(RankSmem &S, int cnt, int c0, int c1, int64_t o,
-                                          int *__restrict__ jc, double *__restrict__ c) {
-    constexpr int WCH = ITEM_WORDS / 64;
+// Writes an item's cnt values to C at offset o, straight from LDS.
+__device__ __forceinline__ void item_store_values(RankSmem &S, int cnt, int64_t o,
+                                                  double *__restrict__ c) {
     const int l = __lane_id();
-    const int nwords = (c1 - c0 + 31) >> 5;
     wave_lds_fence();
     for (int i = l; i < cnt; i += 64) c[o + i] = S.vals[i];
-    wave_lds_fence();
-    int *cols = reinterpret_cast<int *>(S.vals);
-#pragma unroll
-    for (int j = 0; j < WCH; j++) {
-        const int w = j * 64 + l;
-        if (w < nwords) {
-            const uint2 wd = S.words[w];
-            int pos = (int)wd.y;
-            unsigned word = wd.x;
-            while (word) {
-                const int bit = __ffs(word) - 1;
-                word &= word - 1;
-                cols[pos++] = c0 + (w << 5) + bit;
-            }
-        }
-    }
-    wave_lds_fence();
-    for (int i = l; i < cnt; i += 64) jc[o + i] = cols[i];
     wave_lds_fence();
 }
 
@@ -967,7 +958,7 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric_rank(
         item_words(isrc, c0, c1, wv);
         int lo, hi;
         load_range(t0, lo, hi);
-        const int cnt = item_rank_build(S, wv, c0, c1);
+        const int cnt = item_rank_build(S, wv, c0, c1, o, jc);
         auto f = [&](bool act, int col, double av, double bv, bool single) {
             item_accumulate<FUSED>(S, c0, act, col, av, bv, single, b_strict);
         };
@@ -985,7 +976,7 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric_rank(
             a1 = a2;
         }
         if (next < nitems) locate();
-        item_emit(S, cnt, c0, c1, o, jc, c);
+        item_store_values(S, cnt, o, c);
     }
 }
 
