@@ -403,6 +403,8 @@ def main():
     ap.add_argument("--workload", default="rmat20_aat")
     ap.add_argument("--method", default="auto", choices=["auto", "tile", "row"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--row-bins", default=None,
+                    help="row-path bins b1,b2,b3 (tsg_set_row_bins; default: the library's)")
     ap.add_argument("--item-order", type=int, default=None,
                     help="tsg_set_item_order threshold (big-row items run in first-column "
                          "order from this many items; default: the library's)")
@@ -441,6 +443,8 @@ def main():
     h = tsg.Handle(0, method=method, timing=True)
     if args.item_order is not None:
         h.set_item_order(args.item_order)
+    if args.row_bins is not None:
+        h.set_tuning({"row_bins": tuple(int(x) for x in args.row_bins.split(","))})
     if W["aat"]:
         dB = h.transpose(dA)
         k = m
