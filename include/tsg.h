@@ -118,6 +118,13 @@ int tsg_get_tuning(tsg_handle_t handle, double *tile_density, int *bin1, int *bi
  * than it saves.  Default 2^20 (R-MAT n=2^20 A*A^T: 30M items, numeric phase
  * 6% faster).  0 always sorts.  Changes speed only. */
 int tsg_set_item_order(tsg_handle_t handle, int64_t min_items);
+/* Memory for big-row span bitmaps (no counterpart in the reference): the
+ * symbolic phase keeps every big row's column bitmap for the rank-indexed
+ * numeric kernel when they take at most this many bytes; beyond that the
+ * numeric phase builds each item's columns in an LDS hash table instead
+ * (same results, slower: R-MAT n=2^20 A*A^T needs 47 GB of bitmaps).
+ * Negative (the default): min(48 GiB, 40% of free device memory). */
+int tsg_set_bitmap_budget(tsg_handle_t handle, int64_t bytes);
 int tsg_get_item_order(tsg_handle_t handle, int64_t *min_items);
 /* Device workspace currently held by the handle, bytes. */
 size_t tsg_workspace_bytes(tsg_handle_t handle);

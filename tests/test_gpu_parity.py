@@ -126,6 +126,31 @@ def test_big_row_items_in_column_order(handle):
         handle.set_item_order(-1)
 
 
+@pytest.mark.parametrize("fp", [tsg.FP_FUSED, tsg.FP_SEPARATE])
+def test_big_rows_hash_fallback(handle, fp):
+    """No memory for span bitmaps: the big-row numeric phase keeps each
+    item's columns in an LDS hash table and emits them in column order;
+    results stay the oracle's bits (wide rows, R-MAT A*A^T, every row big,
+    B rows unsorted with repeated columns)."""
+    fused = fp == tsg.FP_FUSED
+    handle.set_bitmap_budget(0)
+    try:
+        m, n, k, A, B = _wide_case()
+        got = run(handle, m, n, k, A, B, tsg.METHOD_ROW, fp=fp)
+        assert_csr_equal(got, O.csr_spgemm(m, k, A, B, fused=fused))
+        R = gen.rmat(12, 60_000, seed=3)
+        RT = gen.transpose(4096, 4096, *R)
+        handle.set_tuning({"row_bins": (0, 0, 0)})
+        got = run(handle, 4096, 4096, 4096, R, RT, tsg.METHOD_ROW, fp=fp)
+        assert_csr_equal(got, O.csr_spgemm(4096, 4096, R, RT, fused=fused))
+        U = _scramble_rows(RT, seed=1, dup_every=3)
+        got = run(handle, 4096, 4096, 4096, R, U, tsg.METHOD_ROW, fp=fp)
+        assert_csr_equal(got, O.csr_spgemm(4096, 4096, R, U, fused=fused))
+    finally:
+        handle.set_bitmap_budget(-1)
+        handle.set_tuning({"row_bins": (64, 256, 1024)})
+
+
 def test_row_path_more_than_2p20_columns(handle):
     """k > 2^20: the big-row symbolic bitmap runs in several column passes."""
     rng = np.random.default_rng(4)

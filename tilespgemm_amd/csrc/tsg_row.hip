@@ -1126,7 +1126,9 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
         const size_t bits_bytes = (size_t)nsb_total * ((1u << SUB_SHIFT) / 8);
         size_t free_b = 0, total_b = 0;
         if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) { (void)hipGetLastError(); free_b = 0; }
-        const size_t budget = std::min<size_t>(size_t(48) << 30, free_b / 10 * 4);
+        const size_t budget = tune.bitmap_budget >= 0
+                                  ? (size_t)tune.bitmap_budget
+                                  : std::min<size_t>(size_t(48) << 30, free_b / 10 * 4);
         p.use_rank = bits_bytes <= budget;
         p.rowbits = p.use_rank ? ws.alloc_n<unsigned>(bits_bytes / 4 + 1) : nullptr;
         int2 *units = ws.alloc_n<int2>((size_t)nunits + 1);
