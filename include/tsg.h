@@ -125,6 +125,12 @@ int tsg_set_item_order(tsg_handle_t handle, int64_t min_items);
  * (same results, slower: R-MAT n=2^20 A*A^T needs 47 GB of bitmaps).
  * Negative (the default): min(48 GiB, 40% of free device memory). */
 int tsg_set_bitmap_budget(tsg_handle_t handle, int64_t bytes);
+/* Memory for the B block-pointer table of the big-row path (no counterpart
+ * in the reference): one position per B row and 1024-column block while the
+ * table takes at most this many bytes, else blocks of 2048, 4096, ...
+ * columns (up to 2^30) with a binary search inside a block.  Negative (the
+ * default): 6 GiB.  Changes speed only. */
+int tsg_set_block_pointer_budget(tsg_handle_t handle, int64_t bytes);
 int tsg_get_item_order(tsg_handle_t handle, int64_t *min_items);
 /* Device workspace currently held by the handle, bytes. */
 size_t tsg_workspace_bytes(tsg_handle_t handle);

@@ -151,6 +151,31 @@ def test_big_rows_hash_fallback(handle, fp):
         handle.set_tuning({"row_bins": (64, 256, 1024)})
 
 
+@pytest.mark.parametrize("cap", [0, 4096 * 40])
+def test_coarse_block_pointers(handle, cap):
+    """A block-pointer table capped below one entry per 1024 columns: blocks
+    widen (cap 0: one block of 2^30 columns) and every band and item range
+    comes from a binary search inside a block; same bits as the oracle, also
+    with the hash fallback and with the work sorted by column."""
+    handle.set_block_pointer_budget(cap)
+    try:
+        m, n, k, A, B = _wide_case()
+        want = O.csr_spgemm(m, k, A, B, fused=True)
+        assert_csr_equal(run(handle, m, n, k, A, B, tsg.METHOD_ROW), want)
+        handle.set_item_order(0)
+        assert_csr_equal(run(handle, m, n, k, A, B, tsg.METHOD_ROW), want)
+        handle.set_bitmap_budget(0)
+        assert_csr_equal(run(handle, m, n, k, A, B, tsg.METHOD_ROW), want)
+        R = gen.rmat(12, 60_000, seed=3)
+        RT = gen.transpose(4096, 4096, *R)
+        got = run(handle, 4096, 4096, 4096, R, RT, tsg.METHOD_ROW)
+        assert_csr_equal(got, O.csr_spgemm(4096, 4096, R, RT))
+    finally:
+        handle.set_block_pointer_budget(-1)
+        handle.set_bitmap_budget(-1)
+        handle.set_item_order(1 << 20)
+
+
 def test_row_path_more_than_2p20_columns(handle):
     """k > 2^20: the big-row symbolic bitmap runs in several column passes."""
     rng = np.random.default_rng(4)

@@ -69,6 +69,7 @@ SIGNATURES = {
     "tsg_set_item_order": (_i, [_vp, _i64]),
     "tsg_get_item_order": (_i, [_vp, _vp]),
     "tsg_set_bitmap_budget": (_i, [_vp, _i64]),
+    "tsg_set_block_pointer_budget": (_i, [_vp, _i64]),
     "tsg_read_mtx": (_i, [C.c_char_p, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "tsg_free": (None, [_vp]),
     "tsg_server_create": (_i, [_vp, _i, _vp]),
@@ -358,6 +359,11 @@ class Handle:
         """Bytes of big-row span bitmaps kept for the rank-indexed numeric
         kernel (tsg_set_bitmap_budget; negative: automatic)."""
         _check(load().tsg_set_bitmap_budget(self._h, int(nbytes)), self._h)
+
+    def set_block_pointer_budget(self, nbytes: int):
+        """Bytes of the big-row B block-pointer table before blocks widen
+        (tsg_set_block_pointer_budget; negative: automatic)."""
+        _check(load().tsg_set_block_pointer_budget(self._h, int(nbytes)), self._h)
 
     def item_order(self) -> int:
         v = C.c_int64(0)

@@ -164,6 +164,12 @@ int tsg_set_bitmap_budget(tsg_handle_t h, int64_t bytes) {
     return TSG_OK;
 }
 
+int tsg_set_block_pointer_budget(tsg_handle_t h, int64_t bytes) {
+    if (!h) return TSG_ERR_INVALID;
+    h->row_tune.bptr_budget = bytes < 0 ? -1 : bytes;
+    return TSG_OK;
+}
+
 int tsg_get_item_order(tsg_handle_t h, int64_t *min_items) {
     if (!h || !min_items) return TSG_ERR_INVALID;
     *min_items = h->row_tune.item_sort_min;

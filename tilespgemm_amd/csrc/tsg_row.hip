@@ -1080,10 +1080,12 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
         // B block pointers at the sub-block width (item and band boundaries
         // then need no search) while the table stays <= 6 GiB, coarser beyond
         tm.begin("row.bptr");
+        const size_t bptr_cap = tune.bptr_budget >= 0 ? (size_t)tune.bptr_budget
+                                                      : (size_t(6) << 30);
         int bs = SUB_SHIFT;
         while (bs < 30) {
             size_t nblk = ((size_t)k + (1u << bs) - 1) >> bs;
-            if ((size_t)n * (nblk + 1) * sizeof(int) <= (size_t(6) << 30)) break;
+            if ((size_t)n * (nblk + 1) * sizeof(int) <= bptr_cap) break;
             bs++;
         }
         p.blk_shift = bs;
