@@ -176,6 +176,24 @@ def test_coarse_block_pointers(handle, cap):
         handle.set_item_order(1 << 20)
 
 
+@pytest.mark.parametrize("n,deg,bins", [(40000, 12, (64, 256, 1024)),
+                                          (80000, 20, (64, 256, 1024)),
+                                          (40000, 12, (16, 256, 256))])
+def test_small_rows_wide_spans(handle, n, deg, bins):
+    """Small rows (<= 1024 products) whose columns spread over more than the
+    LDS bitmap covers (16K columns for the 32-lane kernel, 64K for the
+    64-lane one): hash accumulation, then a radix sort of > 64 outputs."""
+    A = gen.erdos_renyi(n, deg, seed=5)
+    handle.set_tuning({"row_bins": bins})
+    try:
+        got = run(handle, n, n, n, A, A, tsg.METHOD_ROW, same=True)
+        assert_csr_equal(got, O.csr_spgemm(n, n, A, A))
+        got = run(handle, n, n, n, A, A, tsg.METHOD_ROW, fp=tsg.FP_SEPARATE, same=True)
+        assert_csr_equal(got, O.csr_spgemm(n, n, A, A, fused=False))
+    finally:
+        handle.set_tuning({"row_bins": (64, 256, 1024)})
+
+
 def test_row_path_more_than_2p20_columns(handle):
     """k > 2^20: the big-row symbolic bitmap runs in several column passes."""
     rng = np.random.default_rng(4)
