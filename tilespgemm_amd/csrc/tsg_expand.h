@@ -6,13 +6,16 @@
 
 // ------------------------------------------------- flat product expansion
 // Range of B row t restricted to columns >= c, through the block pointer
-// table (entries per row: nblk+1; entry b = first position with col >= b<<bs).
+// table: nblk+1 blocks of `stride` (= rows of B) entries, block-major; entry
+// b*stride + t = first position of row t with col >= b<<bs.  Work that runs
+// in column order looks up a few blocks at a time, whose entries for all
+// rows are contiguous and stay in L2.
 __device__ __forceinline__ int brange_lo(const int *__restrict__ bptr, int stride, int bs,
                                          const int *__restrict__ jb, int t, int c) {
     const int b = c >> bs;
-    int lo = bptr[(size_t)t * stride + b];
+    int lo = bptr[(size_t)b * stride + t];
     if (c & ((1 << bs) - 1)) {
-        int hi = bptr[(size_t)t * stride + b + 1];
+        int hi = bptr[(size_t)(b + 1) * stride + t];
         lo = lower_bound_i32(jb, lo, hi, c);
     }
     return lo;

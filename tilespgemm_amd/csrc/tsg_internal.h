@@ -130,7 +130,7 @@ struct RowPlan {
     // B block-pointer table for big-row items
     int blk_shift = 0;               // log2 block width (columns)
     int nblk = 0;                    // blocks per B row (+1 stored)
-    int *bptr = nullptr;             // [(n) * (nblk+1)]
+    int *bptr = nullptr;             // [(nblk+1) * n], block-major
     // items
     int64_t nitems = 0;
     int64_t *item_cnt = nullptr;     // [nbig+1] items per big row -> offsets

@@ -585,17 +585,34 @@ __global__ void __launch_bounds__(BAND_T) k_big_symbolic(
 }
 
 // ------------------------------------------------------ B block pointers
-// bptr[t*(nblk+1) + b] = first position of B row t with column >= b << bs.
+// bptr[b*n + t] = first position of B row t with column >= b << bs (block
+// major).  A block of 4 waves takes 64 B rows; for each range of 128 blocks
+// the waves fill an LDS tile [block][row], a lane per block searching inside
+// its row, then write it out as 64 consecutive rows per block (256-byte runs).
+constexpr int BPT_ROWS = 64, BPT_BLK = 128;
 __global__ void __launch_bounds__(256) k_bptr(int n, const int *__restrict__ ib,
                                               const int *__restrict__ jb, int nblk, int bs,
                                               int *__restrict__ bptr) {
-    const int t = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int l = threadIdx.x & 63;
-    if (t >= n) return;
-    const int lo = ib[t], hi = ib[t + 1];
-    for (int b = l; b <= nblk; b += 64) {
-        int pos = b == nblk ? hi : lower_bound_i32(jb, lo, hi, b << bs);
-        bptr[(size_t)t * (nblk + 1) + b] = pos;
+    __shared__ int tile[BPT_BLK][BPT_ROWS + 1];
+    const int t0 = blockIdx.x * BPT_ROWS;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int b0 = 0; b0 <= nblk; b0 += BPT_BLK) {
+        for (int r = wave; r < BPT_ROWS; r += 4) {
+            const int t = t0 + r;
+            if (t >= n) break;
+            const int lo = ib[t], hi = ib[t + 1];
+            for (int bb = lane; bb < BPT_BLK; bb += 64) {
+                const int b = b0 + bb;
+                if (b <= nblk) tile[bb][r] = b == nblk ? hi : lower_bound_i32(jb, lo, hi, b << bs);
+            }
+        }
+        __syncthreads();
+        for (int bb = wave; bb < BPT_BLK; bb += 4) {
+            const int b = b0 + bb;
+            const int t = t0 + lane;
+            if (b <= nblk && t < n) bptr[(size_t)b * n + t] = tile[bb][lane];
+        }
+        __syncthreads();
     }
 }
 
@@ -1092,7 +1109,7 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
         p.nblk = (int)(((size_t)k + (1u << bs) - 1) >> bs);
         p.bptr = ws.alloc_n<int>((size_t)n * (p.nblk + 1));
         if (n > 0) {
-            hipLaunchKernelGGL(k_bptr, dim3(grid_for((size_t)n * 64, 256)), dim3(256), 0, s, n,
+            hipLaunchKernelGGL(k_bptr, dim3(grid_for((size_t)n, BPT_ROWS)), dim3(256), 0, s, n,
                                B.ptr, B.idx, p.nblk, bs, p.bptr);
             TSG_LAUNCH_CHECK();
         }
@@ -1152,7 +1169,7 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
         hipLaunchKernelGGL(k_big_symbolic,
                            dim3((unsigned)std::min<int64_t>(nunits, int64_t(1) << 30)),
                            dim3(BAND_T), 0, s, nunits, units, rowsb, k, A.ptr, A.idx, B.ptr,
-                           B.idx, p.bptr, p.nblk + 1, p.blk_shift, p.span_sb0, p.sb_off,
+                           B.idx, p.bptr, n, p.blk_shift, p.span_sb0, p.sb_off,
                            p.subcnt, p.rownnz, p.rowbits);
         TSG_LAUNCH_CHECK();
         tm.end();
@@ -1242,7 +1259,7 @@ static void row_numeric_t(RowPlan &p, const CsrIn &A, const CsrIn &B, const int6
         const unsigned grid = item_grid(p.nitems);
         hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * ITEM_WAVES), sm, s, p.nitems, p.items,
                            p.item_off, p.rowbits, p.span_sb0, p.sb_off, A.ptr, A.idx, A.val, B.ptr,
-                           B.idx, B.val, p.bptr, p.nblk + 1, p.blk_shift, d_jc, d_c,
+                           B.idx, B.val, p.bptr, p.n, p.blk_shift, d_jc, d_c,
                            (int)p.b_strict);
         TSG_LAUNCH_CHECK();
         tm.end();
@@ -1254,7 +1271,7 @@ static void row_numeric_t(RowPlan &p, const CsrIn &A, const CsrIn &B, const int6
         const unsigned grid = item_grid(p.nitems);
         hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * ITEM_WAVES), sm, s, p.nitems, p.items,
                            p.item_off, A.ptr, A.idx, A.val, B.ptr, B.idx, B.val, p.bptr,
-                           p.nblk + 1, p.blk_shift, d_jc, d_c, (int)p.b_strict);
+                           p.n, p.blk_shift, d_jc, d_c, (int)p.b_strict);
         TSG_LAUNCH_CHECK();
         tm.end();
     }
