@@ -135,6 +135,7 @@ struct RowPlan {
     int64_t nitems = 0;
     int64_t *item_cnt = nullptr;     // [nbig+1] items per big row -> offsets
     int4 *items = nullptr;           // {row, c0, c1, big-row index}, by c0
+    bool narrow_items = false;       // items of <= 8192 columns (else <= 16384)
     // per big row span bitmaps kept from the symbolic phase (rank mode)
     bool use_rank = false;
     unsigned *rowbits = nullptr;     // [nsb_total * 32] words

@@ -618,14 +618,14 @@ __global__ void __launch_bounds__(256) k_bptr(int n, const int *__restrict__ ib,
 
 // ------------------------------------------------------------- item plan
 // Split of each big row into items: the row's span is cut into windows of
-// ITEM_WBITS columns (32 sub-blocks); inside a window, runs of sub-blocks
-// with at most ITEM_CAP outputs, leading empty sub-blocks skipped.  One wave
-// per big row, one lane per window (64 windows per pass): a lane loads its
-// window's 32 counts at once and splits them in registers, and wave scans
-// place the items and their outputs.  FILL=false counts items per row;
-// FILL=true (item_cnt = exclusive item offsets) writes {row, c0, c1,
+// WSB sub-blocks (at most ITEM_WBITS columns); inside a window, runs of
+// sub-blocks with at most ITEM_CAP outputs, leading empty sub-blocks skipped.
+// One wave per big row, one lane per window (64 windows per pass): a lane
+// loads its window's counts at once and splits them in registers, and wave
+// scans place the items and their outputs.  FILL=false counts items per
+// row; FILL=true (item_cnt = exclusive item offsets) writes {row, c0, c1,
 // big-row index} and each item's first output position.
-template <bool FILL>
+template <bool FILL, int WSB>
 __global__ void __launch_bounds__(256) k_items(int nbig, const int *__restrict__ rows, int k,
                                                const int *__restrict__ span_sb0,
                                                const int64_t *__restrict__ sb_off,
@@ -634,7 +634,7 @@ __global__ void __launch_bounds__(256) k_items(int nbig, const int *__restrict__
                                                int64_t *__restrict__ item_cnt,
                                                int4 *__restrict__ items,
                                                int64_t *__restrict__ item_off) {
-    constexpr int WSB = 1 << (ITEM_WBITS_SHIFT - SUB_SHIFT);
+    static_assert(WSB <= (1 << (ITEM_WBITS_SHIFT - SUB_SHIFT)), "item wider than its LDS");
     const int r = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int l = threadIdx.x & 63;
     if (r >= nbig) return;
@@ -815,8 +815,11 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric(
 // The rank of column j inside the item is prefix[w] + popc(bits[w] below j)
 // -- one LDS read of a {bits, prefix} pair -- and values accumulate in a
 // dense rank-ordered array that is written out coalesced.
+// WORDS: bitmap words of the widest item (256 or 512: items of 8192 or
+// 16384 columns).
+template <int WORDS>
 struct RankSmem {
-    uint2 words[ITEM_WORDS];      // {bitmap word, outputs before it}
+    uint2 words[WORDS];           // {bitmap word, outputs before it}
     double vals[ITEM_CAP];
     unsigned tags[ITEM_TAGS];     // bucketed by rank
     int mark[64];                 // segment marks of the product expansion
@@ -829,21 +832,21 @@ struct RankSmem {
 // the item's columns to jc at offset o (expanded into S.vals, then stored
 // coalesced; the stores drain while the products are processed), zeroes the
 // item's first cnt values and returns cnt, the item's output count.
-constexpr int ITEM_WCH = ITEM_WORDS / 64;
+template <int WCH>
 __device__ __forceinline__ void item_words(const unsigned *__restrict__ src, int c0, int c1,
-                                           unsigned (&wv)[ITEM_WCH]) {
+                                           unsigned (&wv)[WCH]) {
     const int l = __lane_id();
     const int nwords = (c1 - c0 + 31) >> 5;
 #pragma unroll
-    for (int j = 0; j < ITEM_WCH; j++) {
+    for (int j = 0; j < WCH; j++) {
         const int w = j * 64 + l;
         wv[j] = w < nwords ? src[w] : 0u;
     }
 }
 
-__device__ __forceinline__ int item_rank_build(RankSmem &S, unsigned (&wv)[ITEM_WCH], int c0,
+template <int WORDS, int WCH = WORDS / 64>
+__device__ __forceinline__ int item_rank_build(RankSmem<WORDS> &S, unsigned (&wv)[WCH], int c0,
                                                int c1, int64_t o, int *__restrict__ jc) {
-    constexpr int WCH = ITEM_WCH;
     const int l = __lane_id();
     const int nwords = (c1 - c0 + 31) >> 5;
     // the 64-word chunks are scanned independently
@@ -881,7 +884,8 @@ __device__ __forceinline__ int item_rank_build(RankSmem &S, unsigned (&wv)[ITEM_
 }
 
 // Writes an item's cnt values to C at offset o, straight from LDS.
-__device__ __forceinline__ void item_store_values(RankSmem &S, int cnt, int64_t o,
+template <int WORDS>
+__device__ __forceinline__ void item_store_values(RankSmem<WORDS> &S, int cnt, int64_t o,
                                                   double *__restrict__ c) {
     const int l = __lane_id();
     wave_lds_fence();
@@ -892,8 +896,8 @@ __device__ __forceinline__ void item_store_values(RankSmem &S, int cnt, int64_t 
 // Accumulates one batch of products into the item's rank-ordered values:
 // a batch from one A entry (distinct columns when B's rows are strictly
 // ascending) adds directly, any other batch by tag rounds in lane order.
-template <bool FUSED>
-__device__ __forceinline__ void item_accumulate(RankSmem &S, int c0, bool act, int col,
+template <bool FUSED, int WORDS>
+__device__ __forceinline__ void item_accumulate(RankSmem<WORDS> &S, int c0, bool act, int col,
                                                 double av, double bv, bool single, int b_strict) {
     int rank = -1;
     if (act) {
@@ -910,7 +914,7 @@ __device__ __forceinline__ void item_accumulate(RankSmem &S, int c0, bool act, i
                                          (unsigned)__lane_id(), false);
 }
 
-template <bool FUSED>
+template <bool FUSED, int WORDS>
 __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric_rank(
     int64_t nitems, const int4 *__restrict__ items, const int64_t *__restrict__ item_off,
     const unsigned *__restrict__ rowbits, const int *__restrict__ span_sb0,
@@ -919,7 +923,7 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric_rank(
     const int *__restrict__ bptr, int bstride, int bs, int *__restrict__ jc,
     double *__restrict__ c, int b_strict) {
     extern __shared__ __align__(16) unsigned char smem[];
-    RankSmem &S = reinterpret_cast<RankSmem *>(smem)[threadIdx.x >> 6];
+    RankSmem<WORDS> &S = reinterpret_cast<RankSmem<WORDS> *>(smem)[threadIdx.x >> 6];
     const int l = __lane_id();
     for (int i = l; i < ITEM_TAGS; i += 64) S.tags[i] = 0xffffffffu;
     S.mark[l] = -1;
@@ -971,13 +975,13 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric_rank(
         double a0, a1;
         load_t(ipa, t0, a0);
         load_t(ipa + 64, t1, a1);
-        unsigned wv[ITEM_WCH];
+        unsigned wv[WORDS / 64];
         item_words(isrc, c0, c1, wv);
         int lo, hi;
         load_range(t0, lo, hi);
         const int cnt = item_rank_build(S, wv, c0, c1, o, jc);
         auto f = [&](bool act, int col, double av, double bv, bool single) {
-            item_accumulate<FUSED>(S, c0, act, col, av, bv, single, b_strict);
+            item_accumulate<FUSED, WORDS>(S, c0, act, col, av, bv, single, b_strict);
         };
         for (int g = ipa; g < ipe; g += 64) {
             const double ag = a0;
@@ -1185,10 +1189,22 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
         const int *rowsb = p.bin_rows + st.s[4];
         p.item_cnt = ws.alloc_n<int64_t>((size_t)p.nbig + 1);
         int64_t *tmp2 = ws.alloc_n<int64_t>(scan_tmp_elems((size_t)p.nbig + 1));
-        hipLaunchKernelGGL(k_items<false>, dim3(grid_for((size_t)p.nbig * 64, 256)), dim3(256), 0, s, p.nbig,
-                           rowsb, k, p.span_sb0, p.sb_off, p.subcnt, d_ic, p.item_cnt, nullptr,
-                           nullptr);
-        TSG_LAUNCH_CHECK();
+        // Narrow (8-sub-block) windows when the spans are wide enough that
+        // the items will be run in column order: the items in flight then
+        // read a band of B half as wide, which the L2 holds better; wide
+        // windows otherwise (fewer items, fewer (A entry, item) pairs).
+        const bool narrow = p.nsb_total / 16 >= tune.item_sort_min;
+        p.narrow_items = narrow;
+        auto items_kernel = [&](auto kern, int64_t *cnt, int4 *it, int64_t *off) {
+            hipLaunchKernelGGL(kern, dim3(grid_for((size_t)p.nbig * 64, 256)), dim3(256), 0, s,
+                               p.nbig, rowsb, k, p.span_sb0, p.sb_off, p.subcnt, d_ic, cnt, it,
+                               off);
+            TSG_LAUNCH_CHECK();
+        };
+        if (narrow)
+            items_kernel(k_items<false, 8>, p.item_cnt, nullptr, nullptr);
+        else
+            items_kernel(k_items<false, 16>, p.item_cnt, nullptr, nullptr);
         exclusive_scan_i64(p.item_cnt, p.item_cnt, (size_t)p.nbig, tmp2, s);
         TSG_HIP_TRY(hipMemcpyAsync(&h_tot[1], p.item_cnt + p.nbig, sizeof(int64_t),
                                    hipMemcpyDeviceToHost, s));
@@ -1197,10 +1213,10 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
         p.nitems = h_tot[1];
         p.items = ws.alloc_n<int4>((size_t)p.nitems + 1);
         p.item_off = ws.alloc_n<int64_t>((size_t)p.nitems + 1);
-        hipLaunchKernelGGL(k_items<true>, dim3(grid_for((size_t)p.nbig * 64, 256)), dim3(256), 0, s, p.nbig,
-                           rowsb, k, p.span_sb0, p.sb_off, p.subcnt, d_ic, p.item_cnt, p.items,
-                           p.item_off);
-        TSG_LAUNCH_CHECK();
+        if (narrow)
+            items_kernel(k_items<true, 8>, p.item_cnt, p.items, p.item_off);
+        else
+            items_kernel(k_items<true, 16>, p.item_cnt, p.items, p.item_off);
         if (p.nitems > 1 && p.nitems >= tune.item_sort_min && p.nitems < (int64_t(1) << 31)) {
             const size_t ni = (size_t)p.nitems;
             int *keys = ws.alloc_n<int>(ni);
@@ -1253,15 +1269,21 @@ static void row_numeric_t(RowPlan &p, const CsrIn &A, const CsrIn &B, const int6
     tm.end();
     if (p.nitems > 0 && p.use_rank) {
         tm.begin("row.big_numeric");
-        constexpr size_t sm = sizeof(RankSmem) * ITEM_WAVES;
-        auto kern = k_big_numeric_rank<FUSED>;
-        set_smem(kern, sm);
-        const unsigned grid = item_grid(p.nitems);
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * ITEM_WAVES), sm, s, p.nitems, p.items,
-                           p.item_off, p.rowbits, p.span_sb0, p.sb_off, A.ptr, A.idx, A.val, B.ptr,
-                           B.idx, B.val, p.bptr, p.n, p.blk_shift, d_jc, d_c,
-                           (int)p.b_strict);
-        TSG_LAUNCH_CHECK();
+        auto launch_rank = [&](auto kern, size_t sm) {
+            set_smem(kern, sm);
+            const unsigned grid = item_grid(p.nitems);
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * ITEM_WAVES), sm, s, p.nitems, p.items,
+                               p.item_off, p.rowbits, p.span_sb0, p.sb_off, A.ptr, A.idx, A.val,
+                               B.ptr, B.idx, B.val, p.bptr, p.n, p.blk_shift, d_jc, d_c,
+                               (int)p.b_strict);
+            TSG_LAUNCH_CHECK();
+        };
+        if (p.narrow_items)
+            launch_rank(k_big_numeric_rank<FUSED, ITEM_WORDS / 2>,
+                        sizeof(RankSmem<ITEM_WORDS / 2>) * ITEM_WAVES);
+        else
+            launch_rank(k_big_numeric_rank<FUSED, ITEM_WORDS>,
+                        sizeof(RankSmem<ITEM_WORDS>) * ITEM_WAVES);
         tm.end();
     } else if (p.nitems > 0) {
         tm.begin("row.big_numeric");
