@@ -110,13 +110,15 @@ int tsg_set_tile_density(tsg_handle_t handle, double entries_per_tile);
 int tsg_set_row_bins(tsg_handle_t handle, int bin1, int bin2, int bin3);
 int tsg_get_tuning(tsg_handle_t handle, double *tile_density, int *bin1, int *bin2,
                    int *bin3);
-/* Big-row work order (no counterpart in the reference): when the row path
- * splits its big rows into at least min_items numeric items (or symbolic
- * band units), they are run in order of their first column (a radix sort in
- * the symbolic phase), so the units in flight together read B entries of one
- * narrow column band and find them in L2; below that the sort costs more
- * than it saves.  Default 2^20 (R-MAT n=2^20 A*A^T: 30M items, numeric phase
- * 6% faster).  0 always sorts.  Changes speed only. */
+/* Big-row work order (no counterpart in the reference): when the column
+ * spans of the row path's big rows add up to at least min_items windows of
+ * 16384 columns, the symbolic band units and the numeric items are run in
+ * order of their first column (a radix sort in the symbolic phase), so the
+ * units in flight together read B entries of one narrow column band and
+ * find them in L2; the B block pointers are then laid out block-major and
+ * the items cut to 8192 columns.  Below that the sort costs more than it
+ * saves.  Default 2^20 (R-MAT n=2^20 A*A^T: 23M windows).  0
+ * always sorts.  Changes speed only. */
 int tsg_set_item_order(tsg_handle_t handle, int64_t min_items);
 /* Memory for big-row span bitmaps (no counterpart in the reference): the
  * symbolic phase keeps every big row's column bitmap for the rank-indexed

@@ -5,17 +5,27 @@
 #include "tsg_device.h"
 
 // ------------------------------------------------- flat product expansion
-// Range of B row t restricted to columns >= c, through the block pointer
-// table: nblk+1 blocks of `stride` (= rows of B) entries, block-major; entry
-// b*stride + t = first position of row t with col >= b<<bs.  Work that runs
-// in column order looks up a few blocks at a time, whose entries for all
-// rows are contiguous and stay in L2.
-__device__ __forceinline__ int brange_lo(const int *__restrict__ bptr, int stride, int bs,
-                                         const int *__restrict__ jb, int t, int c) {
-    const int b = c >> bs;
-    int lo = bptr[(size_t)b * stride + t];
-    if (c & ((1 << bs) - 1)) {
-        int hi = bptr[(size_t)(b + 1) * stride + t];
+// B block pointers: entry (row t, block b) = first position of B row t with
+// column >= b << bs, at p[t*tstride + b*bstride].  Row-major (tstride =
+// blocks+1, bstride = 1) keeps a row's entries together; block-major
+// (tstride = 1, bstride = rows of B) keeps a block's entries for all rows
+// together, which suits work that runs in column order: it looks up a few
+// blocks at a time, whose entries stay in L2.
+struct BlockPtrs {
+    const int *p = nullptr;
+    int64_t tstride = 0, bstride = 0;
+    int bs = 0;
+};
+
+// Range of B row t restricted to columns >= c, through the block pointers
+// (a binary search inside the block when c is not a block boundary).
+__device__ __forceinline__ int brange_lo(const BlockPtrs &bp, const int *__restrict__ jb, int t,
+                                         int c) {
+    const int b = c >> bp.bs;
+    const int64_t e = (int64_t)t * bp.tstride + (int64_t)b * bp.bstride;
+    int lo = bp.p[e];
+    if (c & ((1 << bp.bs) - 1)) {
+        int hi = bp.p[e + bp.bstride];
         lo = lower_bound_i32(jb, lo, hi, c);
     }
     return lo;
@@ -112,8 +122,8 @@ __device__ __forceinline__ void for_products(int pa, int pe, int gofs, int gstri
                                              const int *__restrict__ ib,
                                              const int *__restrict__ jb,
                                              const double *__restrict__ bval,
-                                             const int *__restrict__ bptr, int bstride, int bs,
-                                             int c0, int c1, int *s_mark, F &&f, int bofs = 0,
+                                             BlockPtrs bp, int c0, int c1, int *s_mark,
+                                             F &&f, int bofs = 0,
                                              int bstep = 1) {
     static_assert(G >= 16, "segment lookup scans 16, 32 or 64 lanes");
     const int l = __lane_id() & (G - 1);
@@ -134,8 +144,8 @@ __device__ __forceinline__ void for_products(int pa, int pe, int gofs, int gstri
         hi = 0;
         if (t >= 0) {
             if (RANGED) {
-                lo = brange_lo(bptr, bstride, bs, jb, t, c0);
-                hi = brange_lo(bptr, bstride, bs, jb, t, c1);
+                lo = brange_lo(bp, jb, t, c0);
+                hi = brange_lo(bp, jb, t, c1);
             } else {
                 lo = ib[t];
                 hi = ib[t + 1];

@@ -130,7 +130,8 @@ struct RowPlan {
     // B block-pointer table for big-row items
     int blk_shift = 0;               // log2 block width (columns)
     int nblk = 0;                    // blocks per B row (+1 stored)
-    int *bptr = nullptr;             // [(nblk+1) * n], block-major
+    int *bptr = nullptr;             // [(nblk+1) * n]: block-major with column_order
+    bool column_order = false;       // big-row bands and items run in column order
     // items
     int64_t nitems = 0;
     int64_t *item_cnt = nullptr;     // [nbig+1] items per big row -> offsets

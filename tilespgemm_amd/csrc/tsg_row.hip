@@ -174,7 +174,7 @@ __global__ void __launch_bounds__(256) k_small_symbolic(int nrows, const int *__
         for (int i = l; i < nw4; i += G) bmp4[i] = make_uint4(0u, 0u, 0u, 0u);
         wave_lds_fence();
         for_products<G, SMALL_U, false, false>(ia[row], ia[row + 1], 0, G, ja, nullptr, ib, jb,
-                                               nullptr, nullptr, 0, 0, 0, 0, s_mark + grp * G,
+                                               nullptr, BlockPtrs(), 0, 0, s_mark + grp * G,
                                                [&](bool act, int col, double, double, bool) {
                                                    if (act) {
                                                        const int cc = col - lo;
@@ -195,7 +195,7 @@ __global__ void __launch_bounds__(256) k_small_symbolic(int nrows, const int *__
     wave_lds_fence();
     int created_cnt = 0;
     for_products<G, SMALL_U, false, false>(ia[row], ia[row + 1], 0, G, ja, nullptr, ib, jb,
-                                           nullptr, nullptr, 0, 0, 0, 0, s_mark + grp * G,
+                                           nullptr, BlockPtrs(), 0, 0, s_mark + grp * G,
                                            [&](bool act, int col, double, double, bool) {
                                       if (act) {
                                           bool created;
@@ -274,7 +274,7 @@ __global__ void __launch_bounds__(RPB * G) k_small_numeric(int nrows, const int 
             for (int i = l; i < nw4; i += G) bmp4[i] = make_uint4(0u, 0u, 0u, 0u);
             wave_lds_fence();
             for_products<G, SMALL_U, false, false>(
-                ia[row], ia[row + 1], 0, G, ja, nullptr, ib, jb, nullptr, nullptr, 0, 0, 0, 0,
+                ia[row], ia[row + 1], 0, G, ja, nullptr, ib, jb, nullptr, BlockPtrs(), 0, 0,
                 s_mark + grp * G, [&](bool act, int col, double, double, bool) {
                     if (act) {
                         const int cc = col - lo;
@@ -298,7 +298,7 @@ __global__ void __launch_bounds__(RPB * G) k_small_numeric(int nrows, const int 
             for (int i = l; i < nnz; i += G) acc[i] = 0.0;
             wave_lds_fence();
             for_products<G, SMALL_U, false, true>(
-                ia[row], ia[row + 1], 0, G, ja, a, ib, jb, b, nullptr, 0, 0, 0, 0,
+                ia[row], ia[row + 1], 0, G, ja, a, ib, jb, b, BlockPtrs(), 0, 0,
                 s_mark + grp * G, [&](bool act, int col, double av, double bv, bool single) {
                     int rank = 0;
                     if (act) {
@@ -334,8 +334,8 @@ __global__ void __launch_bounds__(RPB * G) k_small_numeric(int nrows, const int 
     }
     for (int i = l; i < SLOTS; i += G) { keys[i] = -1; vals[i] = 0.0; }
     wave_lds_fence();
-    for_products<G, SMALL_U, false, true>(ia[row], ia[row + 1], 0, G, ja, a, ib, jb, b, nullptr,
-                                          0, 0, 0, 0, s_mark + grp * G,
+    for_products<G, SMALL_U, false, true>(ia[row], ia[row + 1], 0, G, ja, a, ib, jb, b,
+                                          BlockPtrs(), 0, 0, s_mark + grp * G,
                                           [&](bool act, int col, double av, double bv, bool single) {
                                      int slot = 0;
                                      if (act) {
@@ -521,9 +521,8 @@ __global__ void __launch_bounds__(256) k_gather_units(int64_t n, const int *__re
 __global__ void __launch_bounds__(BAND_T) k_big_symbolic(
     int64_t nunits, const int2 *__restrict__ units, const int *__restrict__ rows, int k,
     const int *__restrict__ ia, const int *__restrict__ ja, const int *__restrict__ ib,
-    const int *__restrict__ jb, const int *__restrict__ bptr, int bstride, int bs,
-    const int *__restrict__ span_sb0, const int64_t *__restrict__ sb_off,
-    uint16_t *__restrict__ subcnt, int64_t *__restrict__ rownnz, unsigned *__restrict__ rowbits) {
+    const int *__restrict__ jb, BlockPtrs bp, const int *__restrict__ span_sb0,
+    const int64_t *__restrict__ sb_off, uint16_t *__restrict__ subcnt, int64_t *__restrict__ rownnz, unsigned *__restrict__ rowbits) {
     __shared__ unsigned bits[BAND_WORDS];
     __shared__ int s_mark[BAND_T / 64][64];
     __shared__ int s_total[BAND_T / 64];
@@ -543,8 +542,7 @@ __global__ void __launch_bounds__(BAND_T) k_big_symbolic(
         for (int i = threadIdx.x; i < words; i += BAND_T) bits[i] = 0u;
         __syncthreads();
         for_products<64, BIG_SYM_U, true, false>(
-            ia[row], ia[row + 1], 0, 64, ja, nullptr, ib, jb, nullptr, bptr, bstride, bs, c0,
-            c1, s_mark[wave],
+            ia[row], ia[row + 1], 0, 64, ja, nullptr, ib, jb, nullptr, bp, c0, c1, s_mark[wave],
             [&](bool act, int col, double, double, bool) {
                 if (act) {
                     const int cc = col - c0;
@@ -585,8 +583,23 @@ __global__ void __launch_bounds__(BAND_T) k_big_symbolic(
 }
 
 // ------------------------------------------------------ B block pointers
-// bptr[b*n + t] = first position of B row t with column >= b << bs (block
-// major).  A block of 4 waves takes 64 B rows; for each range of 128 blocks
+// Row-major table: bptr[t*(nblk+1) + b] = first position of B row t with
+// column >= b << bs; one wave per B row, a lane per block.
+__global__ void __launch_bounds__(256) k_bptr_rows(int n, const int *__restrict__ ib,
+                                                   const int *__restrict__ jb, int nblk, int bs,
+                                                   int *__restrict__ bptr) {
+    const int t = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int l = threadIdx.x & 63;
+    if (t >= n) return;
+    const int lo = ib[t], hi = ib[t + 1];
+    for (int b = l; b <= nblk; b += 64) {
+        const int pos = b == nblk ? hi : lower_bound_i32(jb, lo, hi, b << bs);
+        bptr[(size_t)t * (nblk + 1) + b] = pos;
+    }
+}
+
+// Block-major table: bptr[b*n + t] = first position of B row t with column
+// >= b << bs.  A block of 4 waves takes 64 B rows; for each range of 128 blocks
 // the waves fill an LDS tile [block][row], a lane per block searching inside
 // its row, then write it out as 64 consecutive rows per block (256-byte runs).
 constexpr int BPT_ROWS = 64, BPT_BLK = 128;
@@ -749,7 +762,7 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric(
     int64_t nitems, const int4 *__restrict__ items, const int64_t *__restrict__ item_off,
     const int *__restrict__ ia, const int *__restrict__ ja, const double *__restrict__ a,
     const int *__restrict__ ib, const int *__restrict__ jb, const double *__restrict__ b,
-    const int *__restrict__ bptr, int bstride, int bs, int *__restrict__ jc,
+    BlockPtrs bp, int *__restrict__ jc,
     double *__restrict__ c, int b_strict) {
     extern __shared__ __align__(16) unsigned char smem[];
     ItemSmem &S = reinterpret_cast<ItemSmem *>(smem)[threadIdx.x >> 6];
@@ -767,7 +780,7 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric(
         if (l == 0) S.nid = 0;
         wave_lds_fence();
         for_products<64, ITEM_U, true, true>(
-            ia[row], ia[row + 1], 0, 64, ja, a, ib, jb, b, bptr, bstride, bs, c0, c1, S.mark,
+            ia[row], ia[row + 1], 0, 64, ja, a, ib, jb, b, bp, c0, c1, S.mark,
             [&](bool act, int col, double av, double bv, bool single) {
                 int slot = -1;
                 if (act) {
@@ -920,7 +933,7 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric_rank(
     const unsigned *__restrict__ rowbits, const int *__restrict__ span_sb0,
     const int64_t *__restrict__ sb_off, const int *__restrict__ ia, const int *__restrict__ ja, const double *__restrict__ a,
     const int *__restrict__ ib, const int *__restrict__ jb, const double *__restrict__ b,
-    const int *__restrict__ bptr, int bstride, int bs, int *__restrict__ jc,
+    BlockPtrs bp, int *__restrict__ jc,
     double *__restrict__ c, int b_strict) {
     extern __shared__ __align__(16) unsigned char smem[];
     RankSmem<WORDS> &S = reinterpret_cast<RankSmem<WORDS> *>(smem)[threadIdx.x >> 6];
@@ -967,8 +980,8 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric_rank(
             lo = 0;
             hi = 0;
             if (t >= 0) {
-                lo = brange_lo(bptr, bstride, bs, jb, t, c0);
-                hi = brange_lo(bptr, bstride, bs, jb, t, c1);
+                lo = brange_lo(bp, jb, t, c0);
+                hi = brange_lo(bp, jb, t, c1);
             }
         };
         int t0, t1;
@@ -999,6 +1012,21 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric_rank(
         if (next < nitems) locate();
         item_store_values(S, cnt, o, c);
     }
+}
+
+// The plan's block pointers in the layout they were built in.
+static BlockPtrs block_ptrs(const RowPlan &p) {
+    BlockPtrs bp;
+    bp.p = p.bptr;
+    bp.bs = p.blk_shift;
+    if (p.column_order) {
+        bp.tstride = 1;
+        bp.bstride = p.n;
+    } else {
+        bp.tstride = p.nblk + 1;
+        bp.bstride = 1;
+    }
+    return bp;
 }
 
 // One block per item (the hardware dispatcher hands the next item to
@@ -1098,27 +1126,6 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
     p.sub_shift = SUB_SHIFT;
     if (p.nbig > 0) {
         const int *rowsb = p.bin_rows + st.s[4];
-        // B block pointers at the sub-block width (item and band boundaries
-        // then need no search) while the table stays <= 6 GiB, coarser beyond
-        tm.begin("row.bptr");
-        const size_t bptr_cap = tune.bptr_budget >= 0 ? (size_t)tune.bptr_budget
-                                                      : (size_t(6) << 30);
-        int bs = SUB_SHIFT;
-        while (bs < 30) {
-            size_t nblk = ((size_t)k + (1u << bs) - 1) >> bs;
-            if ((size_t)n * (nblk + 1) * sizeof(int) <= bptr_cap) break;
-            bs++;
-        }
-        p.blk_shift = bs;
-        p.nblk = (int)(((size_t)k + (1u << bs) - 1) >> bs);
-        p.bptr = ws.alloc_n<int>((size_t)n * (p.nblk + 1));
-        if (n > 0) {
-            hipLaunchKernelGGL(k_bptr, dim3(grid_for((size_t)n, BPT_ROWS)), dim3(256), 0, s, n,
-                               B.ptr, B.idx, p.nblk, bs, p.bptr);
-            TSG_LAUNCH_CHECK();
-        }
-        tm.end();
-
         tm.begin("row.big_symbolic");
         // column span of every big row, in sub-blocks, span offsets and the
         // band units over the spans
@@ -1154,12 +1161,45 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
                                   : std::min<size_t>(size_t(48) << 30, free_b / 10 * 4);
         p.use_rank = bits_bytes <= budget;
         p.rowbits = p.use_rank ? ws.alloc_n<unsigned>(bits_bytes / 4 + 1) : nullptr;
+        // Spans this wide (at least item_sort_min windows of 16 sub-blocks)
+        // make enough bands and items for them to run in column order, which
+        // a radix sort arranges; the block pointers are then laid out block
+        // major and the items are 8 sub-blocks wide.
+        p.column_order = nsb_total / 16 >= tune.item_sort_min;
+        tm.end();
+
+        // B block pointers at the sub-block width (item and band boundaries
+        // then need no search) while the table stays <= 6 GiB, coarser beyond
+        tm.begin("row.bptr");
+        const size_t bptr_cap = tune.bptr_budget >= 0 ? (size_t)tune.bptr_budget
+                                                      : (size_t(6) << 30);
+        int bs = SUB_SHIFT;
+        while (bs < 30) {
+            size_t nblk = ((size_t)k + (1u << bs) - 1) >> bs;
+            if ((size_t)n * (nblk + 1) * sizeof(int) <= bptr_cap) break;
+            bs++;
+        }
+        p.blk_shift = bs;
+        p.nblk = (int)(((size_t)k + (1u << bs) - 1) >> bs);
+        p.bptr = ws.alloc_n<int>((size_t)n * (p.nblk + 1));
+        if (n > 0 && p.column_order) {
+            hipLaunchKernelGGL(k_bptr, dim3(grid_for((size_t)n, BPT_ROWS)), dim3(256), 0, s, n,
+                               B.ptr, B.idx, p.nblk, bs, p.bptr);
+            TSG_LAUNCH_CHECK();
+        } else if (n > 0) {
+            hipLaunchKernelGGL(k_bptr_rows, dim3(grid_for((size_t)n * 64, 256)), dim3(256), 0,
+                               s, n, B.ptr, B.idx, p.nblk, bs, p.bptr);
+            TSG_LAUNCH_CHECK();
+        }
+        tm.end();
+
+        tm.begin("row.big_symbolic");
         int2 *units = ws.alloc_n<int2>((size_t)nunits + 1);
         int *ukeys = ws.alloc_n<int>((size_t)nunits + 1);
         hipLaunchKernelGGL(k_band_fill, dim3(grid_for((size_t)p.nbig, 256)), dim3(256), 0, s,
                            p.nbig, p.span_sb0, band_off, units, ukeys);
         TSG_LAUNCH_CHECK();
-        if (nunits > 1 && nunits >= tune.item_sort_min && nunits < (int64_t(1) << 31)) {
+        if (p.column_order && nunits > 1 && nunits < (int64_t(1) << 31)) {
             int *order = ws.alloc_n<int>((size_t)nunits);
             int end_bit = 1;
             while (end_bit < 31 && (int64_t(1) << end_bit) <= ((int64_t)k >> SUB_SHIFT)) end_bit++;
@@ -1173,8 +1213,8 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
         hipLaunchKernelGGL(k_big_symbolic,
                            dim3((unsigned)std::min<int64_t>(nunits, int64_t(1) << 30)),
                            dim3(BAND_T), 0, s, nunits, units, rowsb, k, A.ptr, A.idx, B.ptr,
-                           B.idx, p.bptr, n, p.blk_shift, p.span_sb0, p.sb_off,
-                           p.subcnt, p.rownnz, p.rowbits);
+                           B.idx, block_ptrs(p), p.span_sb0, p.sb_off, p.subcnt, p.rownnz,
+                           p.rowbits);
         TSG_LAUNCH_CHECK();
         tm.end();
     }
@@ -1193,7 +1233,7 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
         // the items will be run in column order: the items in flight then
         // read a band of B half as wide, which the L2 holds better; wide
         // windows otherwise (fewer items, fewer (A entry, item) pairs).
-        const bool narrow = p.nsb_total / 16 >= tune.item_sort_min;
+        const bool narrow = p.column_order;
         p.narrow_items = narrow;
         auto items_kernel = [&](auto kern, int64_t *cnt, int4 *it, int64_t *off) {
             hipLaunchKernelGGL(kern, dim3(grid_for((size_t)p.nbig * 64, 256)), dim3(256), 0, s,
@@ -1217,7 +1257,7 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
             items_kernel(k_items<true, 8>, p.item_cnt, p.items, p.item_off);
         else
             items_kernel(k_items<true, 16>, p.item_cnt, p.items, p.item_off);
-        if (p.nitems > 1 && p.nitems >= tune.item_sort_min && p.nitems < (int64_t(1) << 31)) {
+        if (p.column_order && p.nitems > 1 && p.nitems < (int64_t(1) << 31)) {
             const size_t ni = (size_t)p.nitems;
             int *keys = ws.alloc_n<int>(ni);
             int *order = ws.alloc_n<int>(ni);
@@ -1274,7 +1314,7 @@ static void row_numeric_t(RowPlan &p, const CsrIn &A, const CsrIn &B, const int6
             const unsigned grid = item_grid(p.nitems);
             hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * ITEM_WAVES), sm, s, p.nitems, p.items,
                                p.item_off, p.rowbits, p.span_sb0, p.sb_off, A.ptr, A.idx, A.val,
-                               B.ptr, B.idx, B.val, p.bptr, p.n, p.blk_shift, d_jc, d_c,
+                               B.ptr, B.idx, B.val, block_ptrs(p), d_jc, d_c,
                                (int)p.b_strict);
             TSG_LAUNCH_CHECK();
         };
@@ -1292,8 +1332,8 @@ static void row_numeric_t(RowPlan &p, const CsrIn &A, const CsrIn &B, const int6
         set_smem(kern, sm);
         const unsigned grid = item_grid(p.nitems);
         hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * ITEM_WAVES), sm, s, p.nitems, p.items,
-                           p.item_off, A.ptr, A.idx, A.val, B.ptr, B.idx, B.val, p.bptr,
-                           p.n, p.blk_shift, d_jc, d_c, (int)p.b_strict);
+                           p.item_off, A.ptr, A.idx, A.val, B.ptr, B.idx, B.val, block_ptrs(p),
+                           d_jc, d_c, (int)p.b_strict);
         TSG_LAUNCH_CHECK();
         tm.end();
     }

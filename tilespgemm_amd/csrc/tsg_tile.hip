@@ -195,7 +195,7 @@ __global__ void __launch_bounds__(64 * S1_WAVES) k_step1(
         for (int i = l; i < nwords; i += 64) bits[i] = 0u;
         wave_lds_fence();
         for_products<64, 4, false, false>(pa, pe, 0, 64, tcolA, nullptr, tptrB, tcolB, nullptr,
-                                          nullptr, 0, 0, 0, 0, s_mark[threadIdx.x >> 6],
+                                          BlockPtrs(), 0, 0, s_mark[threadIdx.x >> 6],
                                           [&](bool act, int J, double, double, bool) {
                                            if (act && J >= w0 && J < w1) {
                                                const int c = J - w0;
