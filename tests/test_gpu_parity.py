@@ -300,6 +300,17 @@ def test_invalid_input_is_rejected(handle):
     badp[0][10] = badp[0][11] + 5             # non-monotone row pointers
     with pytest.raises(tsg.TsgError):
         run(handle, 50, 50, 50, badp, A, tsg.METHOD_AUTO)
+    # far out of range: AUTO's tile-density sample runs beside the checks and
+    # must stay in bounds whatever the arrays hold
+    for row, v in [(5, 10**9), (7, -5)]:
+        badq = (A[0].copy(), A[1], A[2])
+        badq[0][row] = v
+        with pytest.raises(tsg.TsgError):
+            run(handle, 50, 50, 50, badq, A, tsg.METHOD_AUTO)
+    badc = (A[0], A[1].copy(), A[2])
+    badc[1][0], badc[1][1] = -1, 2**31 - 1
+    with pytest.raises(tsg.TsgError):
+        run(handle, 50, 50, 50, badc, A, tsg.METHOD_AUTO)
 
 
 def test_transpose_matches_host(handle):

@@ -62,7 +62,8 @@ void check_csr_args(int m, int n, const int *ia, const int *ja, int64_t nnz, con
               std::string(what) + ": inputs are limited to 2^31-1 entries (int32 indices)");
 }
 
-int choose_method(tsg_handle_t h, const CsrIn &A, bool sorted) {
+int choose_method(tsg_handle_t h, const CsrIn &A, bool sorted,
+                  const unsigned long long *sample) {
     if (h->method == TSG_METHOD_TILE)
         TSG_CHECK(sorted, TSG_ERR_INVALID,
                   "the tile path needs column-sorted, duplicate-free rows");
@@ -71,7 +72,7 @@ int choose_method(tsg_handle_t h, const CsrIn &A, bool sorted) {
     if (A.nnz == 0 || A.rows == 0) return TSG_METHOD_ROW;
     // Tile path when A's 16x16 tiles hold enough entries to amortize the
     // per-tile metadata (~60 bytes per tile), row path otherwise.
-    const double per_tile = tile_density(A, h->tile_density, h->ws, h->stream);
+    const double per_tile = tile_density(A, h->tile_density, h->ws, h->stream, sample);
     return per_tile >= h->tile_density ? TSG_METHOD_TILE : TSG_METHOD_ROW;
 }
 
@@ -245,11 +246,14 @@ int tsg_spgemm_symbolic(tsg_handle_t h, int m, int n, int k, const int *d_ia, co
         h->tile.valid = false;
         CsrIn A{m, n, d_ia, d_ja, nullptr, nnz_a};
         CsrIn B{n, k, d_ib, d_jb, nullptr, nnz_b};
-        const int vf = validate_csr_pair(A, B, h->ws, h->stream);
+        // with AUTO, A's tile-density sample shares the validation's round trip
+        unsigned long long sample[2] = {0, 0};
+        const bool want_sample = h->method == TSG_METHOD_AUTO;
+        const int vf = validate_csr_pair(A, B, h->ws, h->stream, want_sample ? sample : nullptr);
         TSG_CHECK((vf & 0x303) == 0, TSG_ERR_INVALID,
                   std::string("invalid CSR input: ") + ((vf & 0x3) ? "A" : "B") +
                       ((vf & 0x101) ? " has bad row pointers" : " has a column out of range"));
-        const int method = choose_method(h, A, (vf & 0x404) == 0);
+        const int method = choose_method(h, A, (vf & 0x404) == 0, want_sample ? sample : nullptr);
         h->method_used = method;
         if (method == TSG_METHOD_TILE) {
             const bool b_is_a = d_ia == d_ib && d_ja == d_jb && m == n && n == k;

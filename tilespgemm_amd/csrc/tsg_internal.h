@@ -247,10 +247,17 @@ void batch_rebase(tsg_handle_t h, const int64_t *d_ic, int64_t *d_out);
 // Validates A and B on the device (one sync).  Returns flags of A in bits
 // 0-7 and of B in bits 8-15: 1 bad row pointers, 2 column out of range,
 // 4 a row not strictly ascending.
-int validate_csr_pair(const CsrIn &A, const CsrIn &B, Arena &ws, hipStream_t s);
+// Validation of A and B (flags: A in bits 0-7, B in bits 8-15; 1 bad row
+// pointers, 2 column out of range, 4 not strictly ascending).  With
+// density_sample, A's tile-density sample (tile_sample_launch) runs beside
+// the checks and its {tiles, entries} come back in the same host round trip.
+int validate_csr_pair(const CsrIn &A, const CsrIn &B, Arena &ws, hipStream_t s,
+                      unsigned long long *density_sample = nullptr);
 // entries per non-empty 16x16 tile of X (tsg_tile.hip; a sample decides when
 // clear of the threshold)
-double tile_density(const CsrIn &X, double threshold, Arena &ws, hipStream_t s);
+void tile_sample_launch(const CsrIn &X, unsigned long long *sums, hipStream_t s);
+double tile_density(const CsrIn &X, double threshold, Arena &ws, hipStream_t s,
+                    const unsigned long long *sample = nullptr);
 void csr_transpose(int m, int n, const int *ia, const int *ja, const double *a,
                    int64_t nnz, int *tp, int *ti, double *tv, Arena &ws,
                    hipStream_t s, int *tpos = nullptr);

@@ -109,7 +109,12 @@ constexpr int TC_WORDS = 2048;
 // With step > 1 only tile rows 0, step, 2*step, ... are counted (wave i takes
 // tile row i*step) and, when sums is given, their tile and entry counts are
 // added to sums[0] and sums[1].
-__global__ void __launch_bounds__(64 * TC_WAVES) k_tile_count(int m, int blkm, int step,
+// Row pointers are clamped to [0, nnz] and columns to [0, ncols): every
+// access stays in bounds (and the tile-column loop bounded) whatever the
+// entries hold, so the sample that picks the method can run beside the
+// input's validation instead of after it.
+__global__ void __launch_bounds__(64 * TC_WAVES) k_tile_count(int m, int ncols, int64_t nnz,
+                                                              int blkm, int step,
                                                               const int *__restrict__ ia,
                                                               const int *__restrict__ ja,
                                                               int *__restrict__ tcount,
@@ -118,12 +123,19 @@ __global__ void __launch_bounds__(64 * TC_WAVES) k_tile_count(int m, int blkm, i
     unsigned *bits = s_bits[threadIdx.x >> 6];
     const int64_t Iw = (int64_t)(blockIdx.x * TC_WAVES + (threadIdx.x >> 6)) * step;
     const int l = __lane_id();
-    if (Iw >= blkm) return;
-    const int I = (int)Iw;
-    const int p0 = ia[I * 16], p1 = ia[min(m, I * 16 + 16)];
+    // waves past the last tile row count nothing but still reach the block's
+    // barrier below
+    const bool active = Iw < blkm;
+    const int I = active ? (int)Iw : 0;
+    int p0 = 0, p1 = 0;
+    if (active) {
+        p0 = (int)min((int64_t)max(ia[I * 16], 0), nnz);
+        p1 = (int)min((int64_t)max(ia[min(m, I * 16 + 16)], p0), nnz);
+    }
+    const int cmax = max(ncols - 1, 0);
     int lo = INT_MAX, hi = -1;
     for (int p = p0 + l; p < p1; p += 64) {
-        const int J = ja[p] >> 4;
+        const int J = min(max(ja[p], 0), cmax) >> 4;
         lo = min(lo, J);
         hi = max(hi, J);
     }
@@ -136,7 +148,7 @@ __global__ void __launch_bounds__(64 * TC_WAVES) k_tile_count(int m, int blkm, i
         for (int i = l; i < nwords; i += 64) bits[i] = 0u;
         wave_lds_fence();
         for (int p = p0 + l; p < p1; p += 64) {
-            const int J = ja[p] >> 4;
+            const int J = min(max(ja[p], 0), cmax) >> 4;
             if (J >= w0 && J < w1) atomicOr(&bits[(J - w0) >> 5], 1u << ((J - w0) & 31));
         }
         wave_lds_fence();
@@ -144,11 +156,23 @@ __global__ void __launch_bounds__(64 * TC_WAVES) k_tile_count(int m, int blkm, i
         wave_lds_fence();
     }
     count = wave_sum(count);
-    if (l == 0) {
-        if (tcount) tcount[I] = count;
-        if (sums) {
-            atomicAdd(&sums[0], (unsigned long long)count);
-            atomicAdd(&sums[1], (unsigned long long)(p1 - p0));
+    if (active && l == 0 && tcount) tcount[I] = count;
+    if (sums) {
+        // one pair of atomics per block
+        __shared__ unsigned long long s_sum[TC_WAVES][2];
+        if (l == 0) {
+            s_sum[threadIdx.x >> 6][0] = (unsigned long long)count;
+            s_sum[threadIdx.x >> 6][1] = (unsigned long long)(p1 - p0);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long t0 = 0, t1 = 0;
+            for (int w = 0; w < TC_WAVES; w++) {
+                t0 += s_sum[w][0];
+                t1 += s_sum[w][1];
+            }
+            atomicAdd(&sums[0], t0);
+            atomicAdd(&sums[1], t1);
         }
     }
 }
@@ -397,7 +421,7 @@ void convert(TileMat &T, const CsrIn &X, bool fill_values, Arena &ws, hipStream_
     T.tile_ptr = ws.alloc_n<int>((size_t)T.blkm + 1);
     if (T.blkm > 0) {
         hipLaunchKernelGGL(k_tile_count, dim3(grid_for(T.blkm, TC_WAVES)), dim3(64 * TC_WAVES), 0,
-                           s, X.rows, T.blkm, 1, X.ptr, X.idx, tcount, nullptr);
+                           s, X.rows, X.cols, X.nnz, T.blkm, 1, X.ptr, X.idx, tcount, nullptr);
         TSG_LAUNCH_CHECK();
     }
     exclusive_scan_i32(tcount, T.tile_ptr, (size_t)T.blkm, tmp, s);
@@ -549,27 +573,47 @@ void tile_numeric(TilePlan &p, const CsrIn &A, const CsrIn &B, const int64_t *d_
     tm.end();
 }
 
-// Number of non-empty 16x16 tiles of X (the tile-count pass of the
-// conversion); used to choose between the tile path and the row path.
-double tile_density(const CsrIn &X, double threshold, Arena &ws, hipStream_t s) {
+// Entries per non-empty 16x16 tile of X, used to choose between the tile
+// path and the row path.  A deterministic sample of <= 1024 tile rows
+// decides unless it lands near the threshold, where every tile row is
+// counted.  tile_sample_launch enqueues the sample's count (sums = {tiles,
+// entries}, zeroed by the caller) so that it can share a host round trip
+// with other work; tile_density takes its result, or counts itself.
+static int tile_sample_step(int blkm) { return blkm > 1024 ? (blkm + 1023) / 1024 : 1; }
+
+static void tile_count_launch(const CsrIn &X, int step, unsigned long long *sums,
+                              hipStream_t s) {
+    const int blkm = (X.rows + 15) / 16;
+    const int nrows = (blkm + step - 1) / step;
+    hipLaunchKernelGGL(k_tile_count, dim3(grid_for(nrows, TC_WAVES)), dim3(64 * TC_WAVES), 0, s,
+                       X.rows, X.cols, X.nnz, blkm, step, X.ptr, X.idx, nullptr, sums);
+    TSG_LAUNCH_CHECK();
+}
+
+void tile_sample_launch(const CsrIn &X, unsigned long long *sums, hipStream_t s) {
+    const int blkm = (X.rows + 15) / 16;
+    if (blkm == 0 || X.nnz == 0) return;
+    tile_count_launch(X, tile_sample_step(blkm), sums, s);
+}
+
+double tile_density(const CsrIn &X, double threshold, Arena &ws, hipStream_t s,
+                    const unsigned long long *sample) {
     const int blkm = (X.rows + 15) / 16;
     if (blkm == 0 || X.nnz == 0) return 0.0;
     unsigned long long *sums = ws.alloc_n<unsigned long long>(2);
+    auto ratio = [](const unsigned long long *h) {
+        return h[0] > 0 ? (double)h[1] / (double)h[0] : 0.0;
+    };
     auto count = [&](int step) {
         TSG_HIP_TRY(hipMemsetAsync(sums, 0, 2 * sizeof(unsigned long long), s));
-        const int nrows = (blkm + step - 1) / step;
-        hipLaunchKernelGGL(k_tile_count, dim3(grid_for(nrows, TC_WAVES)), dim3(64 * TC_WAVES), 0,
-                           s, X.rows, blkm, step, X.ptr, X.idx, nullptr, sums);
-        TSG_LAUNCH_CHECK();
+        tile_count_launch(X, step, sums, s);
         unsigned long long h[2] = {0, 0};
         TSG_HIP_TRY(hipMemcpyAsync(h, sums, sizeof(h), hipMemcpyDeviceToHost, s));
         TSG_HIP_TRY(hipStreamSynchronize(s));
-        return h[0] > 0 ? (double)h[1] / (double)h[0] : 0.0;
+        return ratio(h);
     };
-    // a deterministic sample of <= 4096 tile rows decides unless it lands
-    // near the threshold, where every tile row is counted
-    const int step = blkm > 4096 ? (blkm + 4095) / 4096 : 1;
-    const double est = count(step);
+    const int step = tile_sample_step(blkm);
+    const double est = sample ? ratio(sample) : count(step);
     if (step == 1 || est < 0.75 * threshold || est > 1.33 * threshold) return est;
     return count(1);
 }

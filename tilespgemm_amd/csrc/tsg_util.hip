@@ -4,6 +4,7 @@
 #include "tsg_device.h"
 
 #include <algorithm>
+#include <cstring>
 #include <hipcub/hipcub.hpp>
 
 // ================================================================= arena
@@ -244,9 +245,13 @@ __global__ void __launch_bounds__(256) k_validate(int m, int n, const int *__res
     if (f) atomicOr(&out[0], f);
 }
 
-int validate_csr_pair(const CsrIn &A, const CsrIn &B, Arena &ws, hipStream_t s) {
-    int *d = ws.alloc_n<int>(6);
-    TSG_HIP_TRY(hipMemsetAsync(d, 0, 6 * sizeof(int), s));
+int validate_csr_pair(const CsrIn &A, const CsrIn &B, Arena &ws, hipStream_t s,
+                      unsigned long long *density_sample) {
+    // six flag/count ints, then the two sums of the density sample
+    unsigned long long *buf = ws.alloc_n<unsigned long long>(5);
+    int *d = reinterpret_cast<int *>(buf);
+    TSG_HIP_TRY(hipMemsetAsync(buf, 0, 5 * sizeof(unsigned long long), s));
+    if (density_sample) tile_sample_launch(A, buf + 3, s);
     // a fixed grid (grid-stride inside): few per-block atomics on the counters
     auto threads = [](const CsrIn &X) {
         return std::min(std::max((size_t)X.rows, (size_t)X.nnz), (size_t)256 * 2048);
@@ -261,9 +266,15 @@ int validate_csr_pair(const CsrIn &A, const CsrIn &B, Arena &ws, hipStream_t s) 
                            B.cols, B.ptr, B.idx, B.nnz, d + 3);
         TSG_LAUNCH_CHECK();
     }
-    int h[6] = {0, 0, 0, 0, 0, 0};
-    TSG_HIP_TRY(hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, s));
+    unsigned long long hb[5] = {0, 0, 0, 0, 0};
+    TSG_HIP_TRY(hipMemcpyAsync(hb, buf, sizeof(hb), hipMemcpyDeviceToHost, s));
     TSG_HIP_TRY(hipStreamSynchronize(s));
+    int h[6];
+    std::memcpy(h, hb, sizeof(h));
+    if (density_sample) {
+        density_sample[0] = hb[3];
+        density_sample[1] = hb[4];
+    }
     int fa = h[0], fb = h[3];
     if (h[1] != h[2]) fa |= 4;
     if (h[4] != h[5]) fb |= 4;
