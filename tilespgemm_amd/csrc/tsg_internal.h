@@ -118,6 +118,7 @@ struct RowPlan {
     // bins: small rows (by products) and big rows
     int *bin_rows = nullptr;         // [m] rows grouped by bin
     int *bin_start = nullptr;        // [nbins+1] device
+    uint8_t *narrow3 = nullptr;      // [m] rows of the 1024-product bin for the narrow kernel
     int bin_start_h[16] = {0};       // host copy (after symbolic)
     int nbins = 0;
     // big-row structures

@@ -52,6 +52,11 @@ constexpr int ITEM_TAGS = 256;
 constexpr int ITEM_WAVES = 1;          // waves per block (one item per wave)
 constexpr int RANK_U = 8;
 constexpr int SMALL_U = 2;             // product batches in flight per lane
+// Rows of the 64-lane bin whose column span fits 1024 bitmap words and whose
+// outputs number at most 512 (meshes and stencils) run in a variant with a
+// 1024-entry table: 13 KB of LDS per row instead of 25 KB.
+constexpr int NARROW_WORDS = 1024;
+constexpr int NARROW_OUT = NARROW_WORDS / 2;
 constexpr int BIG_SYM_U = 4;
 constexpr int ITEM_U = 4;
 
@@ -150,7 +155,8 @@ __global__ void __launch_bounds__(256) k_small_symbolic(int nrows, const int *__
                                                         const int *__restrict__ ja,
                                                         const int *__restrict__ ib,
                                                         const int *__restrict__ jb,
-                                                        int64_t *__restrict__ rownnz) {
+                                                        int64_t *__restrict__ rownnz,
+                                                        uint8_t *__restrict__ narrow) {
     constexpr int RPB = 256 / G;
     __shared__ __align__(16) int s_keys[RPB * SLOTS];
     __shared__ int s_mark[256];
@@ -188,7 +194,10 @@ __global__ void __launch_bounds__(256) k_small_symbolic(int nrows, const int *__
             cnt += __popc(wv.x) + __popc(wv.y) + __popc(wv.z) + __popc(wv.w);
         }
         cnt = group_sum<G>(cnt);
-        if (l == 0) rownnz[row] = cnt;
+        if (l == 0) {
+            rownnz[row] = cnt;
+            if (narrow) narrow[row] = nw <= NARROW_WORDS && cnt <= NARROW_OUT;
+        }
         return;
     }
     for (int i = l; i < SLOTS; i += G) keys[i] = -1;
@@ -204,7 +213,10 @@ __global__ void __launch_bounds__(256) k_small_symbolic(int nrows, const int *__
                                       }
                                   });
     created_cnt = group_sum<G>(created_cnt);
-    if (l == 0) rownnz[row] = created_cnt;
+    if (l == 0) {
+        rownnz[row] = created_cnt;
+        if (narrow) narrow[row] = 0;
+    }
 }
 
 // ------------------------------------------------------- small rows: numeric
@@ -230,7 +242,9 @@ __global__ void __launch_bounds__(RPB * G) k_small_numeric(int nrows, const int 
                                                        const double *__restrict__ b,
                                                        const int64_t *__restrict__ ic,
                                                        int *__restrict__ jc,
-                                                       double *__restrict__ c, int b_strict) {
+                                                       double *__restrict__ c, int b_strict,
+                                                       const uint8_t *__restrict__ cls,
+                                                       int want) {
     constexpr int NT = 4 * G;
     constexpr int TW = small_tag_words<G, SLOTS>();
     extern __shared__ __align__(16) unsigned char smem[];
@@ -250,6 +264,7 @@ __global__ void __launch_bounds__(RPB * G) k_small_numeric(int nrows, const int 
     wave_lds_fence();
     if (r >= nrows) return;
     const int row = rows[r];
+    if (cls && cls[row] != want) return;     // a row of the other kernel
     {
         int lo, hi;
         row_span<G>(ia[row], ia[row + 1], ja, ib, jb, lo, hi);
@@ -1101,21 +1116,22 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
         int n1 = st.s[2] - st.s[1];
         if (n1 > 0) {
             hipLaunchKernelGGL((k_small_symbolic<16, 128>), dim3(grid_for(n1, 16)), dim3(256), 0,
-                               s, n1, rows1, A.ptr, A.idx, B.ptr, B.idx, p.rownnz);
+                               s, n1, rows1, A.ptr, A.idx, B.ptr, B.idx, p.rownnz, nullptr);
             TSG_LAUNCH_CHECK();
         }
         const int *rows2 = p.bin_rows + st.s[2];
         int n2 = st.s[3] - st.s[2];
         if (n2 > 0) {
             hipLaunchKernelGGL((k_small_symbolic<32, 512>), dim3(grid_for(n2, 8)), dim3(256), 0,
-                               s, n2, rows2, A.ptr, A.idx, B.ptr, B.idx, p.rownnz);
+                               s, n2, rows2, A.ptr, A.idx, B.ptr, B.idx, p.rownnz, nullptr);
             TSG_LAUNCH_CHECK();
         }
         const int *rows3 = p.bin_rows + st.s[3];
         int n3 = st.s[4] - st.s[3];
         if (n3 > 0) {
+            p.narrow3 = ws.alloc_n<uint8_t>((size_t)m);
             hipLaunchKernelGGL((k_small_symbolic<64, 2048>), dim3(grid_for(n3, 4)), dim3(256), 0,
-                               s, n3, rows3, A.ptr, A.idx, B.ptr, B.idx, p.rownnz);
+                               s, n3, rows3, A.ptr, A.idx, B.ptr, B.idx, p.rownnz, p.narrow3);
             TSG_LAUNCH_CHECK();
         }
     }
@@ -1292,19 +1308,26 @@ static void row_numeric_t(RowPlan &p, const CsrIn &A, const CsrIn &B, const int6
     tm.begin("row.small_numeric");
     {
         // rows of <= 64, <= 256 and <= 1024 products: 16, 2 and 1 rows per block
-        auto launch = [&](auto kern, int rpb, int g, size_t per_row, int b) {
+        // rows of <= 1024 products: the narrow ones (flagged by the symbolic
+        // phase) in the 1024-entry variant, 2 rows per block; both kernels
+        // see the whole bin and skip the other's rows
+        auto launch = [&](auto kern, int rpb, int g, size_t per_row, int b,
+                          const uint8_t *cls = nullptr, int want = 0) {
             const int nr = st[b + 1] - st[b];
             if (nr <= 0) return;
             const size_t sm = per_row * rpb;
             set_smem(kern, sm);
             hipLaunchKernelGGL(kern, dim3(grid_for(nr, rpb)), dim3(rpb * g), sm, s, nr,
                                bins + st[b], A.ptr, A.idx, A.val, B.ptr, B.idx, B.val, d_ic,
-                               d_jc, d_c, (int)p.b_strict);
+                               d_jc, d_c, (int)p.b_strict, cls, want);
             TSG_LAUNCH_CHECK();
         };
         launch(k_small_numeric<16, 128, FUSED, 16>, 16, 16, small_numeric_row_bytes<16, 128>(), 1);
         launch(k_small_numeric<32, 512, FUSED, 2>, 2, 32, small_numeric_row_bytes<32, 512>(), 2);
-        launch(k_small_numeric<64, 2048, FUSED, 1>, 1, 64, small_numeric_row_bytes<64, 2048>(), 3);
+        launch(k_small_numeric<64, NARROW_WORDS, FUSED, 2>, 2, 64,
+               small_numeric_row_bytes<64, NARROW_WORDS>(), 3, p.narrow3, 1);
+        launch(k_small_numeric<64, 2048, FUSED, 1>, 1, 64, small_numeric_row_bytes<64, 2048>(), 3,
+               p.narrow3, 0);
     }
     tm.end();
     if (p.nitems > 0 && p.use_rank) {
