@@ -946,10 +946,9 @@ template <bool FUSED, int WORDS>
 __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric_rank(
     int64_t nitems, const int4 *__restrict__ items, const int64_t *__restrict__ item_off,
     const unsigned *__restrict__ rowbits, const int *__restrict__ span_sb0,
-    const int64_t *__restrict__ sb_off, const int *__restrict__ ia, const int *__restrict__ ja, const double *__restrict__ a,
-    const int *__restrict__ ib, const int *__restrict__ jb, const double *__restrict__ b,
-    BlockPtrs bp, int *__restrict__ jc,
-    double *__restrict__ c, int b_strict) {
+    const int64_t *__restrict__ sb_off, const int *__restrict__ ia, const int *__restrict__ ja,
+    const double *__restrict__ a, const int *__restrict__ jb, const double *__restrict__ b,
+    BlockPtrs bp, int *__restrict__ jc, double *__restrict__ c, int b_strict) {
     extern __shared__ __align__(16) unsigned char smem[];
     RankSmem<WORDS> &S = reinterpret_cast<RankSmem<WORDS> *>(smem)[threadIdx.x >> 6];
     const int l = __lane_id();
@@ -1069,7 +1068,7 @@ void set_smem(K kernel, size_t bytes) {
 
 // =================================================================== host
 void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int64_t *nnz_c,
-                  Arena &ws, hipStream_t s, StageTimer &tm, int cus, const RowTuning &tune) {
+                  Arena &ws, hipStream_t s, StageTimer &tm, int /*cus*/, const RowTuning &tune) {
     const int m = A.rows, n = A.cols, k = B.cols;
     TSG_CHECK(tune.bin1 >= 0 && tune.bin1 <= bin_limit[1] && tune.bin1 <= tune.bin2 &&
                   tune.bin2 <= bin_limit[2] && tune.bin2 <= tune.bin3 &&
@@ -1302,7 +1301,7 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
 
 template <bool FUSED>
 static void row_numeric_t(RowPlan &p, const CsrIn &A, const CsrIn &B, const int64_t *d_ic,
-                          int *d_jc, double *d_c, hipStream_t s, StageTimer &tm, int cus) {
+                          int *d_jc, double *d_c, hipStream_t s, StageTimer &tm) {
     const int *bins = p.bin_rows;
     const int *st = p.bin_start_h;
     tm.begin("row.small_numeric");
@@ -1337,7 +1336,7 @@ static void row_numeric_t(RowPlan &p, const CsrIn &A, const CsrIn &B, const int6
             const unsigned grid = item_grid(p.nitems);
             hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * ITEM_WAVES), sm, s, p.nitems, p.items,
                                p.item_off, p.rowbits, p.span_sb0, p.sb_off, A.ptr, A.idx, A.val,
-                               B.ptr, B.idx, B.val, block_ptrs(p), d_jc, d_c,
+                               B.idx, B.val, block_ptrs(p), d_jc, d_c,
                                (int)p.b_strict);
             TSG_LAUNCH_CHECK();
         };
@@ -1363,12 +1362,12 @@ static void row_numeric_t(RowPlan &p, const CsrIn &A, const CsrIn &B, const int6
 }
 
 void row_numeric(RowPlan &p, const CsrIn &A, const CsrIn &B, const int64_t *d_ic, int *d_jc,
-                 double *d_c, int fused, Arena &ws, hipStream_t s, StageTimer &tm, int cus) {
-    (void)ws;
+                 double *d_c, int fused, Arena & /*ws*/, hipStream_t s, StageTimer &tm,
+                 int /*cus*/) {
     TSG_CHECK(p.valid && p.m == A.rows && p.n == A.cols && p.k == B.cols, TSG_ERR_STATE,
               "row numeric phase without a matching symbolic phase");
     if (fused)
-        row_numeric_t<true>(p, A, B, d_ic, d_jc, d_c, s, tm, cus);
+        row_numeric_t<true>(p, A, B, d_ic, d_jc, d_c, s, tm);
     else
-        row_numeric_t<false>(p, A, B, d_ic, d_jc, d_c, s, tm, cus);
+        row_numeric_t<false>(p, A, B, d_ic, d_jc, d_c, s, tm);
 }
