@@ -97,15 +97,21 @@ def test_batch_result_equals_single_call(handle):
                          (to_host(ic1), to_host(jc1), to_host(c1)))
 
 
-@pytest.mark.parametrize("method", [tsg.METHOD_ROW, tsg.METHOD_TILE])
-def test_graph_captured_numeric_replays(handle, method):
+@pytest.mark.parametrize("method,big", [(tsg.METHOD_ROW, False), (tsg.METHOD_TILE, False),
+                                        (tsg.METHOD_ROW, True)])
+def test_graph_captured_numeric_replays(handle, method, big):
     """The batched numeric phase captured once in a HIP graph and replayed
-    with new values: every replay equals the oracle."""
+    with new values: every replay equals the oracle.  big: every row on the
+    big-row path, bands and items in column order (block-major block
+    pointers, 8192-column items)."""
     import torch
     prods = [p for p in mixed_products(seed=4, count=12) if p[5]]
     pairs = [(dev(m, n, A),) * 2 for m, n, k, A, B, same in prods]
     s = torch.cuda.Stream()
     handle.set_stream(s)
+    if big:
+        handle.set_tuning({"row_bins": (0, 0, 0)})
+        handle.set_item_order(0)
     try:
         handle.set_method(method)
         with torch.cuda.stream(s):
@@ -135,6 +141,8 @@ def test_graph_captured_numeric_replays(handle, method):
                 assert_csr_equal(got, O.csr_spgemm(m, k, X, X))
     finally:
         handle.set_stream(None)
+        handle.set_tuning({"row_bins": (64, 256, 1024)})
+        handle.set_item_order(1 << 20)
 
 
 def test_auto_batch_tile_dense_with_unsorted_rows(handle):
