@@ -614,10 +614,13 @@ __global__ void __launch_bounds__(256) k_bptr_rows(int n, const int *__restrict_
 }
 
 // Block-major table: bptr[b*n + t] = first position of B row t with column
-// >= b << bs.  A block of 4 waves takes 64 B rows; for each range of 128 blocks
+// >= b << bs.  A block of 4 waves takes 64 B rows; for each range of 64 blocks
 // the waves fill an LDS tile [block][row], a lane per block searching inside
 // its row, then write it out as 64 consecutive rows per block (256-byte runs).
-constexpr int BPT_ROWS = 64, BPT_BLK = 128;
+// The 16.6 KB tile keeps 8 blocks per CU resident; the searches are latency
+// bound, so residency decides the time (a 128-block tile took 6.2 ms instead
+// of 4.5 ms for R-MAT 2^20's 16,385 x 2^20 table, a 32-block one 7.1 ms).
+constexpr int BPT_ROWS = 64, BPT_BLK = 64;
 __global__ void __launch_bounds__(256) k_bptr(int n, const int *__restrict__ ib,
                                               const int *__restrict__ jb, int nblk, int bs,
                                               int *__restrict__ bptr) {

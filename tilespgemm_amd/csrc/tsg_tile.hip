@@ -105,6 +105,7 @@ __global__ void __launch_bounds__(256) k_tile_convert(
 // the bitmap window covers 65536 tile columns per pass.
 constexpr int TC_WAVES = 4;
 constexpr int TC_WORDS = 2048;
+constexpr int TC_U = 4;
 
 // With step > 1 only tile rows 0, step, 2*step, ... are counted (wave i takes
 // tile row i*step) and, when sums is given, their tile and entry counts are
@@ -133,11 +134,27 @@ __global__ void __launch_bounds__(64 * TC_WAVES) k_tile_count(int m, int ncols, 
         p1 = (int)min((int64_t)max(ia[min(m, I * 16 + 16)], p0), nnz);
     }
     const int cmax = max(ncols - 1, 0);
+    // TC_U entries per lane and pass, their loads issued together (a heavy
+    // tile row is one wave's serial work); a bit already set is not set again
+    // (power-law rows repeat a few tile columns, whose words would serialise
+    // the LDS atomics)
+    auto tile_cols = [&](int p, int (&J)[TC_U]) {
+#pragma unroll
+        for (int q = 0; q < TC_U; q++) {
+            const int pq = p + q * 64;
+            J[q] = pq < p1 ? min(max(ja[pq], 0), cmax) >> 4 : -1;
+        }
+    };
     int lo = INT_MAX, hi = -1;
-    for (int p = p0 + l; p < p1; p += 64) {
-        const int J = min(max(ja[p], 0), cmax) >> 4;
-        lo = min(lo, J);
-        hi = max(hi, J);
+    for (int p = p0 + l; p < p1; p += 64 * TC_U) {
+        int J[TC_U];
+        tile_cols(p, J);
+#pragma unroll
+        for (int q = 0; q < TC_U; q++)
+            if (J[q] >= 0) {
+                lo = min(lo, J[q]);
+                hi = max(hi, J[q]);
+            }
     }
     lo = wave_min(lo);
     hi = wave_max(hi);
@@ -147,9 +164,16 @@ __global__ void __launch_bounds__(64 * TC_WAVES) k_tile_count(int m, int ncols, 
         const int nwords = (w1 - w0 + 31) >> 5;
         for (int i = l; i < nwords; i += 64) bits[i] = 0u;
         wave_lds_fence();
-        for (int p = p0 + l; p < p1; p += 64) {
-            const int J = min(max(ja[p], 0), cmax) >> 4;
-            if (J >= w0 && J < w1) atomicOr(&bits[(J - w0) >> 5], 1u << ((J - w0) & 31));
+        for (int p = p0 + l; p < p1; p += 64 * TC_U) {
+            int J[TC_U];
+            tile_cols(p, J);
+#pragma unroll
+            for (int q = 0; q < TC_U; q++)
+                if (J[q] >= w0 && J[q] < w1) {
+                    const unsigned bit = 1u << ((J[q] - w0) & 31);
+                    unsigned *word = &bits[(J[q] - w0) >> 5];
+                    if (!(*word & bit)) atomicOr(word, bit);
+                }
         }
         wave_lds_fence();
         for (int i = l; i < nwords; i += 64) count += __popc(bits[i]);
