@@ -64,6 +64,11 @@ typedef struct tsg_handle_s *tsg_handle_t;
  * $TSG_DEVICE).  The symbolic plan of the request=1 call is cached and reused
  * by the request=0 call when (m, n, k, ia, ja, ib, jb) match; otherwise the
  * symbolic phase is recomputed.
+ * The request=0 call keeps C whole on the device when it fits, else it
+ * streams C in row slabs (tsg_set_host_slab_entries): each slab of
+ * consecutive rows of A is a product of its own (symbolic, numeric, copy of
+ * its jc and c to the caller's arrays at ic[first row]); the result is the
+ * same bit for bit.
  * Errors (the reference has none): *info < 0 with a TSG_ERR_* code;
  * TSG_ERR_OVERFLOW when nnz(C) > INT32_MAX (use tsg_dcsrmultcsr_i64).
  * ==================================================================== */
@@ -80,6 +85,18 @@ int tsg_dcsrmultcsr_i64(int request, int m, int n, int k,
                         const double *a, const int *ja, const int *ia,
                         const double *b, const int *jb, const int *ib,
                         double *c, int *jc, int64_t *ic);
+
+/* Row-slab streaming of the two host entries above (no counterpart in the
+ * reference, whose C lives in host memory).  entries caps the C entries the
+ * request=0 call holds on the device at once (12 bytes each); a product with
+ * more runs in slabs of consecutive rows whose entries fit, a row with more
+ * entries than the cap in a slab of its own.  0 (the default) chooses: C
+ * whole when it fits in 90% of the free device memory, else slabs of 35% of
+ * it.  Returns TSG_ERR_INVALID for entries < 0. */
+int tsg_set_host_slab_entries(int64_t entries);
+int64_t tsg_get_host_slab_entries(void);
+/* Slabs run by the last request=0 call of the host entries (1: C whole). */
+int tsg_host_last_slabs(void);
 
 /* ====================================================================
  * Device API.  All matrix pointers are device pointers on the handle's

@@ -45,6 +45,24 @@ def test_binding_loads_and_reports_errors_without_gpu_work():
     assert lib.tsg_workspace_bytes(None) == 0
 
 
+@pytest.mark.skipif(not os.path.exists(tsg.lib_path()), reason="libtsg.so not built")
+def test_host_slab_setting_without_gpu_work():
+    """The host entries' slab cap is a plain setting: negative caps are
+    rejected, others read back; no call has run, so no slabs either."""
+    lib = tsg.load()
+    assert lib.tsg_host_last_slabs() == 0
+    assert lib.tsg_set_host_slab_entries(-1) == -1
+    try:
+        tsg.set_host_slab_entries(12345)
+        assert tsg.host_slab_entries() == 12345
+        with pytest.raises(tsg.TsgError):
+            tsg.set_host_slab_entries(-5)
+        assert tsg.host_slab_entries() == 12345
+    finally:
+        tsg.set_host_slab_entries(0)
+    assert tsg.host_slab_entries() == 0
+
+
 def test_plan_array_enum_matches_header():
     text = open(HEADER).read()
     block = text[text.index("TSG_A_TILE_PTR = 0"):text.index("TSG_PLAN_ARRAY_COUNT")]

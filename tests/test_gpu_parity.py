@@ -395,6 +395,73 @@ def test_reference_host_entry(name):
     assert_csr_equal((ic64, jc64, c64), (g["ic"], g["jc"], g["c_fma"]))
 
 
+@pytest.mark.parametrize("case", ["rmat_aat", "banded_tile", "rect_203x157x91"])
+def test_host_entry_row_slabs(case):
+    """Row-slab streaming of the host entries: with C capped below nnz(C),
+    request=0 runs slabs of consecutive rows of A as products of their own
+    and copies each slab's columns and values out at ic[first row]; C equals
+    the whole-C call and the oracle bit for bit.  Caps below the longest row
+    put that row in a slab of its own; a cap of one entry runs every row
+    alone.  R-MAT rows reach the big-row path, the banded matrix the tile
+    path (AUTO), the rectangular golden product k != n."""
+    if case == "rmat_aat":
+        m = n = k = 2048
+        A = gen.rmat(11, 20_000, seed=4)
+        B = gen.transpose(2048, 2048, *A)
+    elif case == "banded_tile":
+        m = n = k = 3000
+        A = _banded(3000, 16, seed=2)
+        B = A
+    else:
+        g = load_golden(case)
+        m, n, k = g["m"], g["n"], g["k"]
+        A, B = g["A"], g["B"]
+    want = O.csr_spgemm(m, k, A, B)
+    nnz = int(want[0][-1])
+    longest = int(np.diff(want[0]).max())
+    try:
+        tsg.set_host_slab_entries(0)
+        ic = tsg.dcsrmultcsr_i64(1, m, n, k, A, B)
+        np.testing.assert_array_equal(ic, want[0])
+        whole = tsg.dcsrmultcsr_i64(0, m, n, k, A, B, ic=ic)
+        assert tsg.host_last_slabs() == 1
+        assert_csr_equal((ic,) + whole, want)
+        caps = [max(1, nnz // 5), max(1, longest // 2)]
+        if m <= 256:
+            caps.append(1)
+        for cap in caps:
+            tsg.set_host_slab_entries(cap)
+            assert tsg.host_slab_entries() == cap
+            jc, c = tsg.dcsrmultcsr_i64(0, m, n, k, A, B, ic=ic)
+            slabs = tsg.host_last_slabs()
+            assert slabs > 1, (cap, slabs)
+            if cap == 1:
+                assert slabs == m
+            np.testing.assert_array_equal(jc, whole[0])
+            np.testing.assert_array_equal(c.view(np.int64), whole[1].view(np.int64))
+        # the int32 reference entry streams the same way
+        tsg.set_host_slab_entries(max(1, nnz // 3))
+        ic32 = tsg.thmkl_dcsrmultcsr(1, m, n, k, A, B)
+        jc32, c32 = tsg.thmkl_dcsrmultcsr(0, m, n, k, A, B, ic=ic32)
+        assert tsg.host_last_slabs() > 1
+        assert_csr_equal((ic32, jc32, c32), want)
+        # slabs place C by the caller's ic, which must ascend
+        if nnz > 1 and m > 1:
+            bad = ic.copy()
+            r = int(np.argmax(np.diff(ic) > 0))
+            bad[r + 1] = ic[-1]
+            assert np.any(np.diff(bad) < 0)
+            with pytest.raises(tsg.TsgError):
+                tsg.dcsrmultcsr_i64(0, m, n, k, A, B, ic=bad)
+        # back to C whole after slabs: the whole product's plan is rebuilt
+        tsg.set_host_slab_entries(0)
+        jc, c = tsg.dcsrmultcsr_i64(0, m, n, k, A, B, ic=ic)
+        assert tsg.host_last_slabs() == 1
+        assert_csr_equal((ic, jc, c), want)
+    finally:
+        tsg.set_host_slab_entries(0)
+
+
 def _banded(n, bw, seed=0):
     rng = np.random.default_rng(seed)
     rows = np.repeat(np.arange(n), 2 * bw + 1)

@@ -51,6 +51,9 @@ _vp, _i, _i64, _sz = C.c_void_p, C.c_int, C.c_int64, C.c_size_t
 SIGNATURES = {
     "thmkl_dcsrmultcsr": (None, [_vp] * 17),
     "tsg_dcsrmultcsr_i64": (_i, [_i, _i, _i, _i] + [_vp] * 9),
+    "tsg_set_host_slab_entries": (_i, [_i64]),
+    "tsg_get_host_slab_entries": (_i64, []),
+    "tsg_host_last_slabs": (_i, []),
     "tsg_create": (_i, [_vp, _i]),
     "tsg_destroy": (_i, [_vp]),
     "tsg_set_stream": (_i, [_vp, _vp]),
@@ -553,3 +556,18 @@ def dcsrmultcsr_i64(request: int, m: int, n: int, k: int, A, B, ic=None):
         return ic
     nnz = int(ic[-1])
     return jc[:nnz], c[:nnz]
+
+
+def set_host_slab_entries(entries: int) -> None:
+    """Cap on the C entries the host entries' request=0 call keeps on the
+    device at once; larger products stream C in row slabs (0: automatic)."""
+    _check(load().tsg_set_host_slab_entries(int(entries)))
+
+
+def host_slab_entries() -> int:
+    return int(load().tsg_get_host_slab_entries())
+
+
+def host_last_slabs() -> int:
+    """Row slabs the last request=0 host call ran (1: C whole on the device)."""
+    return int(load().tsg_host_last_slabs())

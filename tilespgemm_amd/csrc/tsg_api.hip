@@ -439,15 +439,23 @@ namespace {
 struct HostCtx {
     std::mutex mu;
     tsg_handle_t h = nullptr;
-    Arena buf;
+    Arena buf;     // device copies of the inputs, ic, and C when kept whole
+    Arena sbuf;    // per-slab row pointers and C of the slabbed numeric phase
     // identity of the cached symbolic phase
     bool have = false;
+    bool plan_whole = false;   // the handle holds the whole product's plan
     int m = -1, n = -1, k = -1;
     const void *ia = nullptr, *ja = nullptr, *ib = nullptr, *jb = nullptr;
     int64_t nnz_a = -1, nnz_b = -1, nnz_c = -1;
     int *d_ia = nullptr, *d_ja = nullptr, *d_ib = nullptr, *d_jb = nullptr;
     double *d_a = nullptr, *d_b = nullptr;
     int64_t *d_ic = nullptr;
+    int *d_jc = nullptr;       // whole C (allocated by the first numeric call)
+    double *d_c = nullptr;
+    // C entries kept on the device at once (0: automatic) and the slabs the
+    // last numeric call ran
+    int64_t slab_cap = 0;
+    int last_slabs = 0;
 };
 
 HostCtx &host_ctx() {
@@ -472,7 +480,10 @@ void host_symbolic(HostCtx &x, int m, int n, int k, const int *ia, const int *ja
     const int64_t nnz_a = ia[m] - ia[0], nnz_b = ib[n] - ib[0];
     TSG_CHECK(ia[0] == 0 && ib[0] == 0, TSG_ERR_INVALID, "row pointers must start at 0");
     x.have = false;
+    x.plan_whole = false;
     x.buf.reset();
+    x.d_jc = nullptr;
+    x.d_c = nullptr;
     x.d_ia = x.buf.alloc_n<int>((size_t)m + 1);
     x.d_ja = x.buf.alloc_n<int>((size_t)nnz_a + 1);
     x.d_a = x.buf.alloc_n<double>((size_t)nnz_a + 1);
@@ -501,6 +512,7 @@ void host_symbolic(HostCtx &x, int m, int n, int k, const int *ia, const int *ja
                                  x.d_ic, &nnz_c);
     if (rc != TSG_OK) throw TsgError{rc, tsg_last_error(x.h)};
     x.have = true;
+    x.plan_whole = true;
     x.m = m; x.n = n; x.k = k;
     x.ia = ia; x.ja = ja; x.ib = ib; x.jb = jb;
     x.nnz_a = nnz_a; x.nnz_b = nnz_b; x.nnz_c = nnz_c;
@@ -513,24 +525,113 @@ bool host_matches(const HostCtx &x, int m, int n, int k, const int *ia, const in
            x.nnz_b == (int64_t)(ib[n] - ib[0]);
 }
 
-// Numeric phase from host buffers into host buffers.
-void host_numeric(HostCtx &x, const double *a, const double *b, int *jc, double *c) {
+// C entries that may stay on the device at once: the caller's cap, else all
+// of C when its 12 bytes per entry fit in 90% of the free device memory,
+// else 35% of the free memory plus what the whole product's plan holds (the
+// slabs' plans reuse that memory).
+int64_t host_slab_cap(const HostCtx &x) {
+    if (x.slab_cap > 0) return x.slab_cap;
+    size_t fr = 0, tot = 0;
+    TSG_HIP_TRY(hipMemGetInfo(&fr, &tot));
+    if (12.0 * (double)x.nnz_c + double(1 << 20) <= 0.9 * (double)fr) return x.nnz_c;
+    return std::max<int64_t>(1, (int64_t)(0.35 * (double)(fr + x.h->ws.held()) / 12.0));
+}
+
+// Numeric phase from host buffers into host buffers.  When C exceeds the
+// slab cap, A's rows run in slabs: consecutive rows whose entries (from the
+// caller's ic) fit the cap, a row larger than the cap alone; each slab is
+// its own product A[r0:r1]·B (A's row pointers rebased, its column and
+// value arrays offset), symbolic then numeric, and its columns and values
+// are copied to jc and c at ic[r0].  Every row's entries and sums depend on
+// that row alone, so the slabs give the whole product's C bit for bit.
+template <class IC>
+void host_numeric(HostCtx &x, const int *ia, const IC *ic, const double *a, const double *b,
+                  int *jc, double *c) {
     DeviceGuard dg(x.h->device);
     hipStream_t s = x.h->stream;
     if (x.nnz_a)
         TSG_HIP_TRY(hipMemcpyAsync(x.d_a, a, sizeof(double) * x.nnz_a, hipMemcpyHostToDevice, s));
     if (x.d_b != x.d_a && x.nnz_b)
         TSG_HIP_TRY(hipMemcpyAsync(x.d_b, b, sizeof(double) * x.nnz_b, hipMemcpyHostToDevice, s));
-    int *d_jc = x.buf.alloc_n<int>((size_t)x.nnz_c + 1);
-    double *d_c = x.buf.alloc_n<double>((size_t)x.nnz_c + 1);
-    int rc = tsg_spgemm_numeric(x.h, x.m, x.n, x.k, x.d_ia, x.d_ja, x.d_a, x.nnz_a, x.d_ib,
-                                x.d_jb, x.d_b, x.nnz_b, x.d_ic, d_jc, d_c);
-    if (rc != TSG_OK) throw TsgError{rc, tsg_last_error(x.h)};
-    if (x.nnz_c) {
-        TSG_HIP_TRY(hipMemcpyAsync(jc, d_jc, sizeof(int) * x.nnz_c, hipMemcpyDeviceToHost, s));
-        TSG_HIP_TRY(hipMemcpyAsync(c, d_c, sizeof(double) * x.nnz_c, hipMemcpyDeviceToHost, s));
+    const int64_t cap = host_slab_cap(x);
+    if (x.nnz_c <= cap) {
+        if (!x.plan_whole) {
+            int64_t nz = 0;
+            int rc = tsg_spgemm_symbolic(x.h, x.m, x.n, x.k, x.d_ia, x.d_ja, x.nnz_a, x.d_ib,
+                                         x.d_jb, x.nnz_b, x.d_ic, &nz);
+            if (rc != TSG_OK) throw TsgError{rc, tsg_last_error(x.h)};
+            TSG_CHECK(nz == x.nnz_c, TSG_ERR_STATE, "symbolic phase disagrees with ic");
+            x.plan_whole = true;
+        }
+        if (!x.d_jc) {
+            x.d_jc = x.buf.alloc_n<int>((size_t)x.nnz_c + 1);
+            x.d_c = x.buf.alloc_n<double>((size_t)x.nnz_c + 1);
+        }
+        int rc = tsg_spgemm_numeric(x.h, x.m, x.n, x.k, x.d_ia, x.d_ja, x.d_a, x.nnz_a, x.d_ib,
+                                    x.d_jb, x.d_b, x.nnz_b, x.d_ic, x.d_jc, x.d_c);
+        if (rc != TSG_OK) throw TsgError{rc, tsg_last_error(x.h)};
+        if (x.nnz_c) {
+            TSG_HIP_TRY(hipMemcpyAsync(jc, x.d_jc, sizeof(int) * x.nnz_c, hipMemcpyDeviceToHost,
+                                       s));
+            TSG_HIP_TRY(hipMemcpyAsync(c, x.d_c, sizeof(double) * x.nnz_c,
+                                       hipMemcpyDeviceToHost, s));
+        }
+        TSG_HIP_TRY(hipStreamSynchronize(s));
+        x.last_slabs = 1;
+        return;
     }
+    // the whole product's plan and C give way to the slabs'
     TSG_HIP_TRY(hipStreamSynchronize(s));
+    x.h->row.valid = false;
+    x.h->tile.valid = false;
+    x.h->ws.release();
+    x.plan_whole = false;
+    // the copies land at the caller's ic: it must ascend to ic[m] = nnz(C)
+    for (int r = 0; r < x.m; r++)
+        TSG_CHECK(ic[r + 1] >= ic[r], TSG_ERR_STATE, "ic is not non-decreasing");
+    std::vector<int> row0{0};
+    int max_rows = 0;
+    int64_t max_nnz = 0;
+    for (int r = 0; r < x.m;) {
+        int r1 = r + 1;
+        while (r1 < x.m && (int64_t)ic[r1 + 1] - (int64_t)ic[r] <= cap) r1++;
+        row0.push_back(r1);
+        max_rows = std::max(max_rows, r1 - r);
+        max_nnz = std::max<int64_t>(max_nnz, (int64_t)ic[r1] - (int64_t)ic[r]);
+        r = r1;
+    }
+    x.sbuf.reset();
+    int *d_ia_s = x.sbuf.alloc_n<int>((size_t)max_rows + 1);
+    int64_t *d_ic_s = x.sbuf.alloc_n<int64_t>((size_t)max_rows + 1);
+    int *d_jc_s = x.sbuf.alloc_n<int>((size_t)max_nnz + 1);
+    double *d_c_s = x.sbuf.alloc_n<double>((size_t)max_nnz + 1);
+    std::vector<int> ia_s((size_t)max_rows + 1);
+    const int nslab = (int)row0.size() - 1;
+    for (int i = 0; i < nslab; i++) {
+        const int r0 = row0[i], r1 = row0[i + 1], ms = r1 - r0;
+        const int64_t e0 = ia[r0], ea = (int64_t)ia[r1] - ia[r0];
+        for (int j = 0; j <= ms; j++) ia_s[j] = ia[r0 + j] - ia[r0];
+        TSG_HIP_TRY(hipMemcpyAsync(d_ia_s, ia_s.data(), sizeof(int) * ((size_t)ms + 1),
+                                   hipMemcpyHostToDevice, s));
+        int64_t nz = 0;
+        int rc = tsg_spgemm_symbolic(x.h, ms, x.n, x.k, d_ia_s, x.d_ja + e0, ea, x.d_ib, x.d_jb,
+                                     x.nnz_b, d_ic_s, &nz);
+        if (rc != TSG_OK) throw TsgError{rc, tsg_last_error(x.h)};
+        const int64_t o = ic[r0];
+        TSG_CHECK(nz == (int64_t)ic[r1] - o, TSG_ERR_STATE,
+                  "a slab's symbolic phase disagrees with ic");
+        rc = tsg_spgemm_numeric(x.h, ms, x.n, x.k, d_ia_s, x.d_ja + e0, x.d_a + e0, ea, x.d_ib,
+                                x.d_jb, x.d_b, x.nnz_b, d_ic_s, d_jc_s, d_c_s);
+        if (rc != TSG_OK) throw TsgError{rc, tsg_last_error(x.h)};
+        if (nz) {
+            TSG_HIP_TRY(hipMemcpyAsync(jc + o, d_jc_s, sizeof(int) * nz, hipMemcpyDeviceToHost, s));
+            TSG_HIP_TRY(hipMemcpyAsync(c + o, d_c_s, sizeof(double) * nz, hipMemcpyDeviceToHost,
+                                       s));
+        }
+        // ia_s is rewritten for the next slab
+        TSG_HIP_TRY(hipStreamSynchronize(s));
+    }
+    x.last_slabs = nslab;
 }
 
 template <class IC>
@@ -557,7 +658,7 @@ int host_entry(int request, int m, int n, int k, const double *a, const int *ja,
         TSG_CHECK((int64_t)ic[m] == x.nnz_c && ic[0] == 0, TSG_ERR_STATE,
                   "ic does not match the symbolic phase of these inputs");
         TSG_CHECK(x.nnz_c == 0 || (jc && c), TSG_ERR_INVALID, "null jc or c");
-        host_numeric(x, a, b, jc, c);
+        host_numeric(x, ia, ic, a, b, jc, c);
     });
 }
 
@@ -584,6 +685,26 @@ int tsg_dcsrmultcsr_i64(int request, int m, int n, int k, const double *a, const
                         const int *ia, const double *b, const int *jb, const int *ib, double *c,
                         int *jc, int64_t *ic) {
     return host_entry<int64_t>(request, m, n, k, a, ja, ia, b, jb, ib, c, jc, ic);
+}
+
+int tsg_set_host_slab_entries(int64_t entries) {
+    if (entries < 0) return TSG_ERR_INVALID;
+    HostCtx &x = host_ctx();
+    std::lock_guard<std::mutex> lk(x.mu);
+    x.slab_cap = entries;
+    return TSG_OK;
+}
+
+int64_t tsg_get_host_slab_entries(void) {
+    HostCtx &x = host_ctx();
+    std::lock_guard<std::mutex> lk(x.mu);
+    return x.slab_cap;
+}
+
+int tsg_host_last_slabs(void) {
+    HostCtx &x = host_ctx();
+    std::lock_guard<std::mutex> lk(x.mu);
+    return x.last_slabs;
 }
 
 }  // extern "C"
