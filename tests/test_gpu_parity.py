@@ -462,6 +462,38 @@ def test_host_entry_row_slabs(case):
         tsg.set_host_slab_entries(0)
 
 
+def test_host_entry_large_c_matches_device_api(handle):
+    """C of 165M entries (columns 661 MB, values 1.3 GB) back through the host
+    entry's pinned staging buffers, many chunks each, whole and in slabs:
+    equal bit for bit to the device API's C for the same product (R-MAT
+    n=2^16 A*A^T)."""
+    m = 1 << 16
+    A = gen.rmat(16, 16 << 16, seed=1)
+    B = gen.transpose(m, m, *A)
+    handle.set_method(tsg.METHOD_AUTO)
+    handle.set_fp_mode(tsg.FP_FUSED)
+    dA, dB = to_dev(m, m, A), to_dev(m, m, B)
+    ic_d, jc_d, c_d = handle.spgemm(dA, dB)
+    want = (ic_d.cpu().numpy(), jc_d.cpu().numpy(), c_d.cpu().numpy())
+    del ic_d, jc_d, c_d, dA, dB
+    nnz = int(want[0][-1])
+    print(f"[host large] nnz(C)={nnz}", flush=True)
+    try:
+        tsg.set_host_slab_entries(0)
+        ic = tsg.dcsrmultcsr_i64(1, m, m, m, A, B)
+        np.testing.assert_array_equal(ic, want[0])
+        for cap in (0, nnz // 3):
+            tsg.set_host_slab_entries(cap)
+            jc, c = tsg.dcsrmultcsr_i64(0, m, m, m, A, B, ic=ic)
+            slabs = tsg.host_last_slabs()
+            assert (slabs == 1) if cap == 0 else (slabs >= 3), (cap, slabs)
+            np.testing.assert_array_equal(jc, want[1])
+            np.testing.assert_array_equal(c.view(np.int64), want[2].view(np.int64))
+            del jc, c
+    finally:
+        tsg.set_host_slab_entries(0)
+
+
 def _banded(n, bw, seed=0):
     rng = np.random.default_rng(seed)
     rows = np.repeat(np.arange(n), 2 * bw + 1)

@@ -112,6 +112,14 @@ def load():
         if not os.path.exists(path):
             raise FileNotFoundError(
                 f"{path} not found: build it with `python -m tilespgemm_amd.build`")
+        # torch ships its own HIP runtime under ROCm's soname
+        # (libamdhip64.so.7): with torch loaded first, libtsg binds to that
+        # runtime and one runtime serves both; loaded the other way round, the
+        # process would hold two runtimes, and torch's could not use the GPU
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = C.CDLL(path)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(lib, name)

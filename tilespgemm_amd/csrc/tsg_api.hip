@@ -3,10 +3,12 @@
 #include "tsg_internal.h"
 
 #include <algorithm>
+#include <condition_variable>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <thread>
 
 
 namespace {
@@ -436,8 +438,114 @@ int tsg_plan_array_copy(tsg_handle_t h, int which, void *dst, size_t bytes) {
 // device copies and the plan between the request=1 and request=0 calls.
 namespace {
 
+// Host-side copy split over a few threads: a pageable destination fills at
+// the speed of several cores (page first-touch included) instead of one.
+class CopyPool {
+public:
+    explicit CopyPool(int nthreads) : n_(std::max(1, nthreads)) {
+        for (int i = 1; i < n_; i++) th_.emplace_back([this, i] { work(i); });
+    }
+    ~CopyPool() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto &t : th_) t.join();
+    }
+    void copy(void *dst, const void *src, size_t bytes) {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            dst_ = static_cast<char *>(dst);
+            src_ = static_cast<const char *>(src);
+            bytes_ = bytes;
+            pending_ = n_ - 1;
+            gen_++;
+        }
+        cv_.notify_all();
+        stripe(0);
+        std::unique_lock<std::mutex> lk(mu_);
+        done_.wait(lk, [this] { return pending_ == 0; });
+    }
+
+private:
+    void stripe(int i) {
+        // 4 KiB-aligned stripes, one per thread
+        const size_t per = ((bytes_ + n_ - 1) / n_ + 4095) & ~size_t(4095);
+        const size_t b0 = std::min(bytes_, per * i), b1 = std::min(bytes_, b0 + per);
+        if (b1 > b0) std::memcpy(dst_ + b0, src_ + b0, b1 - b0);
+    }
+    void work(int i) {
+        uint64_t seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+                if (stop_) return;
+                seen = gen_;
+            }
+            stripe(i);
+            std::lock_guard<std::mutex> lk(mu_);
+            if (--pending_ == 0) done_.notify_one();
+        }
+    }
+    const int n_;
+    std::vector<std::thread> th_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_;
+    char *dst_ = nullptr;
+    const char *src_ = nullptr;
+    size_t bytes_ = 0;
+    int pending_ = 0;
+    uint64_t gen_ = 0;
+    bool stop_ = false;
+};
+
+// Device -> pageable host copy through two pinned staging buffers: chunk
+// i+1 is DMA'd into one while the pool copies chunk i out of the other.
+// Waits for work queued on s before the first chunk; returns when dst is
+// complete.
+constexpr size_t STAGE_BYTES = size_t(64) << 20;
+
+struct Stager {
+    void *buf[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    std::unique_ptr<CopyPool> pool;
+
+    void init() {
+        if (pool) return;
+        for (int i = 0; i < 2; i++) {
+            TSG_HIP_TRY(hipHostMalloc(&buf[i], STAGE_BYTES, hipHostMallocDefault));
+            TSG_HIP_TRY(hipEventCreateWithFlags(&ev[i], hipEventDisableTiming));
+        }
+        const unsigned hc = std::thread::hardware_concurrency();
+        pool.reset(new CopyPool((int)std::min(8u, std::max(1u, hc))));
+    }
+    void d2h(void *dst, const void *src, size_t bytes, hipStream_t s) {
+        if (bytes == 0) return;
+        init();
+        const size_t nch = (bytes + STAGE_BYTES - 1) / STAGE_BYTES;
+        auto issue = [&](size_t i) {
+            const size_t off = i * STAGE_BYTES, len = std::min(STAGE_BYTES, bytes - off);
+            TSG_HIP_TRY(hipMemcpyAsync(buf[i & 1], static_cast<const char *>(src) + off, len,
+                                       hipMemcpyDeviceToHost, s));
+            TSG_HIP_TRY(hipEventRecord(ev[i & 1], s));
+        };
+        issue(0);
+        for (size_t i = 0; i < nch; i++) {
+            // chunk i+1 goes to the buffer whose chunk (i-1) was copied out
+            // in the previous iteration
+            if (i + 1 < nch) issue(i + 1);
+            TSG_HIP_TRY(hipEventSynchronize(ev[i & 1]));
+            const size_t off = i * STAGE_BYTES, len = std::min(STAGE_BYTES, bytes - off);
+            pool->copy(static_cast<char *>(dst) + off, buf[i & 1], len);
+        }
+    }
+};
+
 struct HostCtx {
     std::mutex mu;
+    Stager stage;   // C's way back to the caller's arrays
     tsg_handle_t h = nullptr;
     Arena buf;     // device copies of the inputs, ic, and C when kept whole
     Arena sbuf;    // per-slab row pointers and C of the slabbed numeric phase
@@ -570,12 +678,8 @@ void host_numeric(HostCtx &x, const int *ia, const IC *ic, const double *a, cons
         int rc = tsg_spgemm_numeric(x.h, x.m, x.n, x.k, x.d_ia, x.d_ja, x.d_a, x.nnz_a, x.d_ib,
                                     x.d_jb, x.d_b, x.nnz_b, x.d_ic, x.d_jc, x.d_c);
         if (rc != TSG_OK) throw TsgError{rc, tsg_last_error(x.h)};
-        if (x.nnz_c) {
-            TSG_HIP_TRY(hipMemcpyAsync(jc, x.d_jc, sizeof(int) * x.nnz_c, hipMemcpyDeviceToHost,
-                                       s));
-            TSG_HIP_TRY(hipMemcpyAsync(c, x.d_c, sizeof(double) * x.nnz_c,
-                                       hipMemcpyDeviceToHost, s));
-        }
+        x.stage.d2h(jc, x.d_jc, sizeof(int) * x.nnz_c, s);
+        x.stage.d2h(c, x.d_c, sizeof(double) * x.nnz_c, s);
         TSG_HIP_TRY(hipStreamSynchronize(s));
         x.last_slabs = 1;
         return;
@@ -623,11 +727,8 @@ void host_numeric(HostCtx &x, const int *ia, const IC *ic, const double *a, cons
         rc = tsg_spgemm_numeric(x.h, ms, x.n, x.k, d_ia_s, x.d_ja + e0, x.d_a + e0, ea, x.d_ib,
                                 x.d_jb, x.d_b, x.nnz_b, d_ic_s, d_jc_s, d_c_s);
         if (rc != TSG_OK) throw TsgError{rc, tsg_last_error(x.h)};
-        if (nz) {
-            TSG_HIP_TRY(hipMemcpyAsync(jc + o, d_jc_s, sizeof(int) * nz, hipMemcpyDeviceToHost, s));
-            TSG_HIP_TRY(hipMemcpyAsync(c + o, d_c_s, sizeof(double) * nz, hipMemcpyDeviceToHost,
-                                       s));
-        }
+        x.stage.d2h(jc + o, d_jc_s, sizeof(int) * nz, s);
+        x.stage.d2h(c + o, d_c_s, sizeof(double) * nz, s);
         // ia_s is rewritten for the next slab
         TSG_HIP_TRY(hipStreamSynchronize(s));
     }
