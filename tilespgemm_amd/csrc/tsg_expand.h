@@ -295,13 +295,22 @@ __device__ __forceinline__ bool radix_sort_group(int *skeys, double *svals, int 
             if (act && (m & below) == 0) hist[d] += (unsigned)__popcll(m);
             wave_lds_fence();
         }
-        // exclusive scan of the 256 counts
-        if (l == 0) {
-            unsigned run = 0;
-            for (int i = 0; i < 256; i++) {
-                const unsigned c = hist[i];
-                hist[i] = run;
-                run += c;
+        // exclusive scan of the 256 counts: 256/G consecutive bins per lane,
+        // then one group scan of the lanes' sums
+        {
+            constexpr int PER = 256 / G;
+            int cnt[PER];
+            int sum = 0;
+#pragma unroll
+            for (int q = 0; q < PER; q++) {
+                cnt[q] = (int)hist[l * PER + q];
+                sum += cnt[q];
+            }
+            int run = group_incl_scan<G>(sum) - sum;
+#pragma unroll
+            for (int q = 0; q < PER; q++) {
+                hist[l * PER + q] = (unsigned)run;
+                run += cnt[q];
             }
         }
         wave_lds_fence();
