@@ -462,6 +462,29 @@ def test_host_entry_row_slabs(case):
         tsg.set_host_slab_entries(0)
 
 
+def test_auto_near_threshold_counts_every_tile_row(handle):
+    """AUTO's density sample (every 2nd of R-MAT 2^15's 2048 tile rows here)
+    decides only when clear of the threshold; near it every tile row is
+    counted.  Thresholds a hair below and above A's exact entries per
+    non-empty 16x16 tile (numpy) must pick the tile and the row path."""
+    n = 1 << 15
+    R = gen.rmat(15, 1 << 19, seed=7)
+    ia, ja = np.asarray(R[0]), np.asarray(R[1])
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(ia))
+    ntiles = np.unique((rows >> 4) * (n >> 4) + (ja >> 4)).size
+    d = ja.size / ntiles
+    print(f"[near threshold] {ja.size} entries in {ntiles} tiles: {d:.6f} per tile", flush=True)
+    want = O.csr_spgemm(n, n, R, R)
+    try:
+        for thr, method in ((d * (1 - 1e-9), tsg.METHOD_TILE), (d * (1 + 1e-9), tsg.METHOD_ROW)):
+            handle.set_tuning({"tile_density": thr})
+            got = run(handle, n, n, n, R, R, tsg.METHOD_AUTO, same=True)
+            assert handle.method_used == method, (thr, d)
+            assert_csr_equal(got, want)
+    finally:
+        handle.set_tuning({"tile_density": 32.0})
+
+
 def test_host_entry_large_c_matches_device_api(handle):
     """C of 165M entries (columns 661 MB, values 1.3 GB) back through the host
     entry's pinned staging buffers, many chunks each, whole and in slabs:

@@ -107,66 +107,41 @@ constexpr int TC_WAVES = 4;
 constexpr int TC_WORDS = 2048;
 constexpr int TC_U = 4;
 
-// With step > 1 only tile rows 0, step, 2*step, ... are counted (wave i takes
-// tile row i*step) and, when sums is given, their tile and entry counts are
-// added to sums[0] and sums[1].
 // Row pointers are clamped to [0, nnz] and columns to [0, ncols): every
 // access stays in bounds (and the tile-column loop bounded) whatever the
-// entries hold, so the sample that picks the method can run beside the
-// input's validation instead of after it.
-__global__ void __launch_bounds__(64 * TC_WAVES) k_tile_count(int m, int ncols, int64_t nnz,
-                                                              int blkm, int step,
-                                                              const int *__restrict__ ia,
-                                                              const int *__restrict__ ja,
-                                                              int *__restrict__ tcount,
-                                                              unsigned long long *__restrict__ sums) {
-    __shared__ unsigned s_bits[TC_WAVES][TC_WORDS];
-    unsigned *bits = s_bits[threadIdx.x >> 6];
-    const int64_t Iw = (int64_t)(blockIdx.x * TC_WAVES + (threadIdx.x >> 6)) * step;
-    const int l = __lane_id();
-    // waves past the last tile row count nothing but still reach the block's
-    // barrier below
-    const bool active = Iw < blkm;
-    const int I = active ? (int)Iw : 0;
-    int p0 = 0, p1 = 0;
-    if (active) {
-        p0 = (int)min((int64_t)max(ia[I * 16], 0), nnz);
-        p1 = (int)min((int64_t)max(ia[min(m, I * 16 + 16)], p0), nnz);
-    }
-    const int cmax = max(ncols - 1, 0);
-    // TC_U entries per lane and pass, their loads issued together (a heavy
-    // tile row is one wave's serial work); a bit already set is not set again
-    // (power-law rows repeat a few tile columns, whose words would serialise
-    // the LDS atomics)
-    auto tile_cols = [&](int p, int (&J)[TC_U]) {
+// entries hold.  A bit already set is not set again (power-law rows repeat a
+// few tile columns, whose words would serialise the LDS atomics); TC_U
+// entries per lane and pass have their loads issued together.
+template <int NT>
+struct TileCols {
+    int p1, cmax;
+    const int *ja;
+    __device__ void load(int p, int (&J)[TC_U]) const {
 #pragma unroll
         for (int q = 0; q < TC_U; q++) {
-            const int pq = p + q * 64;
+            const int pq = p + q * NT;
             J[q] = pq < p1 ? min(max(ja[pq], 0), cmax) >> 4 : -1;
         }
-    };
-    int lo = INT_MAX, hi = -1;
-    for (int p = p0 + l; p < p1; p += 64 * TC_U) {
-        int J[TC_U];
-        tile_cols(p, J);
-#pragma unroll
-        for (int q = 0; q < TC_U; q++)
-            if (J[q] >= 0) {
-                lo = min(lo, J[q]);
-                hi = max(hi, J[q]);
-            }
     }
-    lo = wave_min(lo);
-    hi = wave_max(hi);
-    int count = 0;
-    for (int w0 = lo; w0 <= hi; w0 += TC_WORDS * 32) {
-        const int w1 = min(hi + 1, w0 + TC_WORDS * 32);
-        const int nwords = (w1 - w0 + 31) >> 5;
-        for (int i = l; i < nwords; i += 64) bits[i] = 0u;
-        wave_lds_fence();
-        for (int p = p0 + l; p < p1; p += 64 * TC_U) {
+    // tile-column range of the entries [p0 + t, p1) taken by thread t
+    __device__ void range(int p0, int t, int &lo, int &hi) const {
+        lo = INT_MAX;
+        hi = -1;
+        for (int p = p0 + t; p < p1; p += NT * TC_U) {
             int J[TC_U];
-            tile_cols(p, J);
+            load(p, J);
+#pragma unroll
+            for (int q = 0; q < TC_U; q++)
+                if (J[q] >= 0) {
+                    lo = min(lo, J[q]);
+                    hi = max(hi, J[q]);
+                }
+        }
+    }
+    __device__ void mark(int p0, int t, int w0, int w1, unsigned *bits) const {
+        for (int p = p0 + t; p < p1; p += NT * TC_U) {
+            int J[TC_U];
+            load(p, J);
 #pragma unroll
             for (int q = 0; q < TC_U; q++)
                 if (J[q] >= w0 && J[q] < w1) {
@@ -175,29 +150,91 @@ __global__ void __launch_bounds__(64 * TC_WAVES) k_tile_count(int m, int ncols, 
                     if (!(*word & bit)) atomicOr(word, bit);
                 }
         }
+    }
+};
+
+__global__ void __launch_bounds__(64 * TC_WAVES) k_tile_count(int m, int ncols, int64_t nnz,
+                                                              int blkm,
+                                                              const int *__restrict__ ia,
+                                                              const int *__restrict__ ja,
+                                                              int *__restrict__ tcount) {
+    __shared__ unsigned s_bits[TC_WAVES][TC_WORDS];
+    unsigned *bits = s_bits[threadIdx.x >> 6];
+    const int I = blockIdx.x * TC_WAVES + (threadIdx.x >> 6);
+    const int l = __lane_id();
+    if (I >= blkm) return;
+    const int p0 = (int)min((int64_t)max(ia[I * 16], 0), nnz);
+    const int p1 = (int)min((int64_t)max(ia[min(m, I * 16 + 16)], p0), nnz);
+    const TileCols<64> tc{p1, max(ncols - 1, 0), ja};
+    int lo, hi;
+    tc.range(p0, l, lo, hi);
+    lo = wave_min(lo);
+    hi = wave_max(hi);
+    int count = 0;
+    for (int w0 = lo; w0 <= hi; w0 += TC_WORDS * 32) {
+        const int w1 = min(hi + 1, w0 + TC_WORDS * 32);
+        const int nwords = (w1 - w0 + 31) >> 5;
+        for (int i = l; i < nwords; i += 64) bits[i] = 0u;
+        wave_lds_fence();
+        tc.mark(p0, l, w0, w1, bits);
         wave_lds_fence();
         for (int i = l; i < nwords; i += 64) count += __popc(bits[i]);
         wave_lds_fence();
     }
     count = wave_sum(count);
-    if (active && l == 0 && tcount) tcount[I] = count;
-    if (sums) {
-        // one pair of atomics per block
-        __shared__ unsigned long long s_sum[TC_WAVES][2];
-        if (l == 0) {
-            s_sum[threadIdx.x >> 6][0] = (unsigned long long)count;
-            s_sum[threadIdx.x >> 6][1] = (unsigned long long)(p1 - p0);
-        }
+    if (l == 0) tcount[I] = count;
+}
+
+// AUTO's density sample: tile rows 0, step, 2*step, ... (the grid has one
+// block per sampled row, so every block's row is < blkm), one 256-thread
+// block each sharing a bitmap of 262144 tile columns per pass -- a
+// power-law matrix puts most of its entries in its first tile rows, and one
+// wave per row left the sample waiting on those.  The tile and entry counts
+// are added to sums[0] and sums[1].  Indices are clamped as above, so the
+// sample can run beside the input's validation instead of after it.
+constexpr int TS_WORDS = 8192;
+
+__global__ void __launch_bounds__(256) k_tile_sample(int m, int ncols, int64_t nnz, int step,
+                                                     const int *__restrict__ ia,
+                                                     const int *__restrict__ ja,
+                                                     unsigned long long *__restrict__ sums) {
+    __shared__ unsigned bits[TS_WORDS];
+    __shared__ int s_lo[4], s_hi[4];
+    __shared__ int s_cnt[4];
+    const int I = blockIdx.x * step;
+    const int t = threadIdx.x, l = t & 63, w = t >> 6;
+    const int p0 = (int)min((int64_t)max(ia[I * 16], 0), nnz);
+    const int p1 = (int)min((int64_t)max(ia[min(m, I * 16 + 16)], p0), nnz);
+    const TileCols<256> tc{p1, max(ncols - 1, 0), ja};
+    int lo, hi;
+    tc.range(p0, t, lo, hi);
+    lo = wave_min(lo);
+    hi = wave_max(hi);
+    if (l == 0) {
+        s_lo[w] = lo;
+        s_hi[w] = hi;
+    }
+    __syncthreads();
+    lo = min(min(s_lo[0], s_lo[1]), min(s_lo[2], s_lo[3]));
+    hi = max(max(s_hi[0], s_hi[1]), max(s_hi[2], s_hi[3]));
+    int count = 0;
+    // lo and hi are the block's: every thread takes every pass and barrier
+    for (int w0 = lo; w0 <= hi; w0 += TS_WORDS * 32) {
+        const int w1 = min(hi + 1, w0 + TS_WORDS * 32);
+        const int nwords = (w1 - w0 + 31) >> 5;
+        for (int i = t; i < nwords; i += 256) bits[i] = 0u;
         __syncthreads();
-        if (threadIdx.x == 0) {
-            unsigned long long t0 = 0, t1 = 0;
-            for (int w = 0; w < TC_WAVES; w++) {
-                t0 += s_sum[w][0];
-                t1 += s_sum[w][1];
-            }
-            atomicAdd(&sums[0], t0);
-            atomicAdd(&sums[1], t1);
-        }
+        tc.mark(p0, t, w0, w1, bits);
+        __syncthreads();
+        for (int i = t; i < nwords; i += 256) count += __popc(bits[i]);
+        __syncthreads();
+    }
+    count = wave_sum(count);
+    if (l == 0) s_cnt[w] = count;
+    __syncthreads();
+    if (t == 0) {
+        atomicAdd(&sums[0], (unsigned long long)(s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3]));
+        atomicAdd(&sums[1], (unsigned long long)(p1 - p0));
     }
 }
 
@@ -445,7 +482,7 @@ void convert(TileMat &T, const CsrIn &X, bool fill_values, Arena &ws, hipStream_
     T.tile_ptr = ws.alloc_n<int>((size_t)T.blkm + 1);
     if (T.blkm > 0) {
         hipLaunchKernelGGL(k_tile_count, dim3(grid_for(T.blkm, TC_WAVES)), dim3(64 * TC_WAVES), 0,
-                           s, X.rows, X.cols, X.nnz, T.blkm, 1, X.ptr, X.idx, tcount, nullptr);
+                           s, X.rows, X.cols, X.nnz, T.blkm, X.ptr, X.idx, tcount);
         TSG_LAUNCH_CHECK();
     }
     exclusive_scan_i32(tcount, T.tile_ptr, (size_t)T.blkm, tmp, s);
@@ -608,9 +645,11 @@ static int tile_sample_step(int blkm) { return blkm > 1024 ? (blkm + 1023) / 102
 static void tile_count_launch(const CsrIn &X, int step, unsigned long long *sums,
                               hipStream_t s) {
     const int blkm = (X.rows + 15) / 16;
+    // one block per sampled tile row: block b takes tile row b*step < blkm
     const int nrows = (blkm + step - 1) / step;
-    hipLaunchKernelGGL(k_tile_count, dim3(grid_for(nrows, TC_WAVES)), dim3(64 * TC_WAVES), 0, s,
-                       X.rows, X.cols, X.nnz, blkm, step, X.ptr, X.idx, nullptr, sums);
+    if (nrows == 0) return;
+    hipLaunchKernelGGL(k_tile_sample, dim3((unsigned)nrows), dim3(256), 0, s, X.rows, X.cols,
+                       X.nnz, step, X.ptr, X.idx, sums);
     TSG_LAUNCH_CHECK();
 }
 
