@@ -136,6 +136,8 @@ def bench_batch(args):
     torch.cuda.synchronize()
 
     tuned = None
+    if args.row_bins is not None:
+        h.set_tuning({"row_bins": tuple(int(x) for x in args.row_bins.split(","))})
     if not args.no_autotune:
         from tilespgemm_amd import autotune as AT
         t0 = time.time()

@@ -119,6 +119,8 @@ struct RowPlan {
     int *bin_rows = nullptr;         // [m] rows grouped by bin
     int *bin_start = nullptr;        // [nbins+1] device
     uint8_t *narrow3 = nullptr;      // [m] rows of the 1024-product bin for the narrow kernel
+    int *rows3 = nullptr;            // that bin's rows, the narrow ones first
+    int n_narrow3 = 0;               // (host) narrow rows at the front of rows3
     int bin_start_h[16] = {0};       // host copy (after symbolic)
     int nbins = 0;
     // big-row structures

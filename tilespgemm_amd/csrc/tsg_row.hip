@@ -123,6 +123,40 @@ __global__ void __launch_bounds__(256) k_bin_scatter(int m, const int64_t *__res
     if (b > 0) bin_rows[st.s[b] + s_base[b] + pos] = i;
 }
 
+// Rows of the <= 1024-product bin split by the symbolic phase's narrow flag:
+// narrow rows to the front of out, the others to the back (any order inside
+// each part, as rows are independent); count[0] and count[1] (zero on entry)
+// end as the two parts' sizes.
+__global__ void __launch_bounds__(256) k_split_narrow(int n, const int *__restrict__ rows,
+                                                      const uint8_t *__restrict__ narrow,
+                                                      int *__restrict__ out,
+                                                      int *__restrict__ count) {
+    __shared__ int s_cnt[2], s_base[2];
+    if (threadIdx.x < 2) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int row = i < n ? rows[i] : -1;
+    const int cls = i < n ? (narrow[row] ? 0 : 1) : -1;
+    const uint64_t below = (uint64_t(1) << lane) - 1;
+    int pos = 0;
+    for (int q = 0; q < 2; q++) {
+        const uint64_t mask = __ballot(cls == q);
+        if (!mask) continue;
+        const int leader = __ffsll((unsigned long long)mask) - 1;
+        int wbase = 0;
+        if (lane == leader) wbase = atomicAdd(&s_cnt[q], __popcll(mask));
+        wbase = __shfl(wbase, leader);
+        if (cls == q) pos = wbase + __popcll(mask & below);
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 && s_cnt[threadIdx.x])
+        s_base[threadIdx.x] = atomicAdd(&count[threadIdx.x], s_cnt[threadIdx.x]);
+    __syncthreads();
+    if (cls == 0) out[s_base[0] + pos] = row;
+    else if (cls == 1) out[n - 1 - (s_base[1] + pos)] = row;
+}
+
 // Column span [lo, hi] of a row's products, from the first and last column
 // of each B row its A entries name (lo > hi for a row without products); G
 // lanes of a group.
@@ -242,9 +276,7 @@ __global__ void __launch_bounds__(RPB * G) k_small_numeric(int nrows, const int 
                                                        const double *__restrict__ b,
                                                        const int64_t *__restrict__ ic,
                                                        int *__restrict__ jc,
-                                                       double *__restrict__ c, int b_strict,
-                                                       const uint8_t *__restrict__ cls,
-                                                       int want) {
+                                                       double *__restrict__ c, int b_strict) {
     constexpr int NT = 4 * G;
     constexpr int TW = small_tag_words<G, SLOTS>();
     extern __shared__ __align__(16) unsigned char smem[];
@@ -264,7 +296,6 @@ __global__ void __launch_bounds__(RPB * G) k_small_numeric(int nrows, const int 
     wave_lds_fence();
     if (r >= nrows) return;
     const int row = rows[r];
-    if (cls && cls[row] != want) return;     // a row of the other kernel
     {
         int lo, hi;
         row_span<G>(ia[row], ia[row + 1], ja, ib, jb, lo, hi);
@@ -1135,6 +1166,14 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
             hipLaunchKernelGGL((k_small_symbolic<64, 2048>), dim3(grid_for(n3, 4)), dim3(256), 0,
                                s, n3, rows3, A.ptr, A.idx, B.ptr, B.idx, p.rownnz, p.narrow3);
             TSG_LAUNCH_CHECK();
+            // narrow rows first, so each numeric variant launches over its own
+            // rows; the count comes back with the phase's last host round trip
+            p.rows3 = ws.alloc_n<int>((size_t)n3);
+            hipLaunchKernelGGL(k_split_narrow, dim3(grid_for(n3, 256)), dim3(256), 0, s, n3, rows3,
+                               p.narrow3, p.rows3, d_cnt + 2 * NBINS);
+            TSG_LAUNCH_CHECK();
+            TSG_HIP_TRY(hipMemcpyAsync(&p.n_narrow3, d_cnt + 2 * NBINS, sizeof(int),
+                                       hipMemcpyDeviceToHost, s));
         }
     }
     tm.end();
@@ -1309,27 +1348,29 @@ static void row_numeric_t(RowPlan &p, const CsrIn &A, const CsrIn &B, const int6
     const int *st = p.bin_start_h;
     tm.begin("row.small_numeric");
     {
-        // rows of <= 64, <= 256 and <= 1024 products: 16, 2 and 1 rows per block
-        // rows of <= 1024 products: the narrow ones (flagged by the symbolic
-        // phase) in the 1024-entry variant, 2 rows per block; both kernels
-        // see the whole bin and skip the other's rows
-        auto launch = [&](auto kern, int rpb, int g, size_t per_row, int b,
-                          const uint8_t *cls = nullptr, int want = 0) {
-            const int nr = st[b + 1] - st[b];
+        // rows of <= 64, <= 256 and <= 1024 products: 16, 2 and 1 rows per block;
+        // of the <= 1024-product rows the narrow ones (the symbolic phase put
+        // them first) in the 1024-entry variant, 2 rows per block
+        auto launch = [&](auto kern, int rpb, int g, size_t per_row, const int *rows, int nr) {
             if (nr <= 0) return;
             const size_t sm = per_row * rpb;
             set_smem(kern, sm);
-            hipLaunchKernelGGL(kern, dim3(grid_for(nr, rpb)), dim3(rpb * g), sm, s, nr,
-                               bins + st[b], A.ptr, A.idx, A.val, B.ptr, B.idx, B.val, d_ic,
-                               d_jc, d_c, (int)p.b_strict, cls, want);
+            hipLaunchKernelGGL(kern, dim3(grid_for(nr, rpb)), dim3(rpb * g), sm, s, nr, rows,
+                               A.ptr, A.idx, A.val, B.ptr, B.idx, B.val, d_ic, d_jc, d_c,
+                               (int)p.b_strict);
             TSG_LAUNCH_CHECK();
         };
-        launch(k_small_numeric<16, 128, FUSED, 16>, 16, 16, small_numeric_row_bytes<16, 128>(), 1);
-        launch(k_small_numeric<32, 512, FUSED, 2>, 2, 32, small_numeric_row_bytes<32, 512>(), 2);
-        launch(k_small_numeric<64, NARROW_WORDS, FUSED, 2>, 2, 64,
-               small_numeric_row_bytes<64, NARROW_WORDS>(), 3, p.narrow3, 1);
-        launch(k_small_numeric<64, 2048, FUSED, 1>, 1, 64, small_numeric_row_bytes<64, 2048>(), 3,
-               p.narrow3, 0);
+        launch(k_small_numeric<16, 128, FUSED, 16>, 16, 16, small_numeric_row_bytes<16, 128>(),
+               bins + st[1], st[2] - st[1]);
+        launch(k_small_numeric<32, 512, FUSED, 2>, 2, 32, small_numeric_row_bytes<32, 512>(),
+               bins + st[2], st[3] - st[2]);
+        const int n3 = st[4] - st[3];
+        if (n3 > 0) {
+            launch(k_small_numeric<64, NARROW_WORDS, FUSED, 2>, 2, 64,
+                   small_numeric_row_bytes<64, NARROW_WORDS>(), p.rows3, p.n_narrow3);
+            launch(k_small_numeric<64, 2048, FUSED, 1>, 1, 64, small_numeric_row_bytes<64, 2048>(),
+                   p.rows3 + p.n_narrow3, n3 - p.n_narrow3);
+        }
     }
     tm.end();
     if (p.nitems > 0 && p.use_rank) {
