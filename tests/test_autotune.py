@@ -47,6 +47,25 @@ def test_descent_respects_bin_order():
         assert b1 <= b2 <= b3
 
 
+def test_tune_takes_a_measure_and_applies_the_best():
+    """tune() with a measure of its own (the numeric-phase-only objective a
+    captured graph needs): the cheapest setting is applied to the handle and
+    reported beside the default's time."""
+    class H:
+        applied = None
+
+        def set_tuning(self, t):
+            H.applied = dict(t)
+
+    def cost(t):
+        return 10.0 - (t["row_bins"][2] == 256) * 3.0
+
+    res = AT.tune(H(), None, measure=cost)
+    assert res["row_bins"] == [64, 256, 256]
+    assert res["ms"] == 7.0 and res["default_ms"] == 10.0
+    assert tuple(H.applied["row_bins"]) == (64, 256, 256)
+
+
 def test_signature_is_stable():
     class M:
         def __init__(self, rows, nnz):

@@ -62,6 +62,9 @@ def _cases():
     L64 = gen.laplacian2d(64)
     S12 = gen.stencil27(12, seed=3)
     E = gen.erdos_renyi(4000, 12.0, seed=21)
+    # rows of ~576 products spanning <= 20000 columns: those with more than
+    # 512 outputs run in the rank-only kernel, the others in the narrow one
+    E24 = gen.erdos_renyi(20000, 24.0, seed=22)
     R12 = gen.rmat(12, 40_000, seed=5)
     R12T = gen.transpose(4096, 4096, *R12)
     R14 = gen.rmat(14, 250_000, seed=6)
@@ -70,6 +73,7 @@ def _cases():
         ("lap64_AA", 4096, 4096, 4096, L64, L64, True),
         ("st27_12_AA", 1728, 1728, 1728, S12, S12, True),
         ("er4000_AA", 4000, 4000, 4000, E, E, True),
+        ("er20000_d24_AA", 20000, 20000, 20000, E24, E24, True),
         ("rmat12_AAT", 4096, 4096, 4096, R12, R12T, False),
         ("rmat14_AAT", 16384, 16384, 16384, R14, R14T, False),
     ]

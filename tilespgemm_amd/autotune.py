@@ -128,6 +128,31 @@ def batch_run(handle, pairs):
     return run
 
 
+def batch_numeric_measure(handle, pairs, reps: int = 3) -> Callable[[dict], float]:
+    """A measure() for the numeric phase alone -- what a captured HIP graph
+    replays: per tuning, one batched symbolic phase (untimed), then the
+    numeric phase once untimed and reps times timed; minimum in ms."""
+    import torch
+
+    def measure(t: dict) -> float:
+        handle.set_tuning(t)
+        arr, ic, per, roff, tot = handle.batch_symbolic(pairs)
+        dev = ic.device
+        jc = torch.empty(max(tot, 1), dtype=torch.int32, device=dev)
+        c = torch.empty(max(tot, 1), dtype=torch.float64, device=dev)
+        handle.batch_numeric(arr, ic, jc, c)
+        torch.cuda.synchronize()
+        best = math.inf
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            handle.batch_numeric(arr, ic, jc, c)
+            torch.cuda.synchronize()
+            best = min(best, (time.perf_counter() - t0) * 1e3)
+        return best
+
+    return measure
+
+
 def single_run(handle, A, B):
     """One product C = A * B."""
     def run():
@@ -146,9 +171,11 @@ def signature(pairs) -> str:
 
 
 def tune(handle, run, reps: int = 3, cache_path: Optional[str] = None,
-         key: Optional[str] = None, log=None) -> dict:
-    """Tune handle for run(); returns {"tile_density", "row_bins", "ms",
-    "default_ms", "trials"} and leaves the best tuning applied.  With
+         key: Optional[str] = None, log=None,
+         measure: Optional[Callable[[dict], float]] = None) -> dict:
+    """Tune handle for run() (or for a measure(tuning) -> ms given instead,
+    such as batch_numeric_measure); returns {"tile_density", "row_bins",
+    "ms", "default_ms", "trials"} and leaves the best tuning applied.  With
     cache_path and key, a stored result is reused instead of searching."""
     if cache_path and key and os.path.exists(cache_path):
         with open(cache_path) as f:
@@ -157,7 +184,8 @@ def tune(handle, run, reps: int = 3, cache_path: Optional[str] = None,
             res = stored[key]
             handle.set_tuning(res)
             return res
-    measure = measure_runs(handle, run, reps)
+    if measure is None:
+        measure = measure_runs(handle, run, reps)
     best, best_ms, trials = coordinate_descent(measure, log=log)
     default_ms = next(ms for t, ms in trials if t == {**DEFAULT,
                                                       "row_bins": tuple(DEFAULT["row_bins"])})

@@ -118,9 +118,11 @@ struct RowPlan {
     // bins: small rows (by products) and big rows
     int *bin_rows = nullptr;         // [m] rows grouped by bin
     int *bin_start = nullptr;        // [nbins+1] device
-    uint8_t *narrow3 = nullptr;      // [m] rows of the 1024-product bin for the narrow kernel
-    int *rows3 = nullptr;            // that bin's rows, the narrow ones first
-    int n_narrow3 = 0;               // (host) narrow rows at the front of rows3
+    uint8_t *narrow3 = nullptr;      // [m] class of each row of the 1024-product bin:
+                                     // 1 narrow, 2 span <= 2048 words, 0 wider
+    int *rows3 = nullptr;            // that bin's class-1 rows first, class-0 rows last
+    int *rows3r = nullptr;           // its class-2 rows
+    int n3cls[3] = {0, 0, 0};        // (host) rows per class
     int bin_start_h[16] = {0};       // host copy (after symbolic)
     int nbins = 0;
     // big-row structures

@@ -128,24 +128,25 @@ __global__ void __launch_bounds__(256) k_bin_scatter(int m, const int64_t *__res
     if (b > 0) bin_rows[st.s[b] + s_base[b] + pos] = i;
 }
 
-// Rows of the <= 1024-product bin split by the symbolic phase's narrow flag:
-// narrow rows to the front of out, the others to the back (any order inside
-// each part, as rows are independent); count[0] and count[1] (zero on entry)
-// end as the two parts' sizes.
-__global__ void __launch_bounds__(256) k_split_narrow(int n, const int *__restrict__ rows,
-                                                      const uint8_t *__restrict__ narrow,
-                                                      int *__restrict__ out,
-                                                      int *__restrict__ count) {
-    __shared__ int s_cnt[2], s_base[2];
-    if (threadIdx.x < 2) s_cnt[threadIdx.x] = 0;
+// Rows of the <= 1024-product bin split by the class the symbolic phase gave
+// them (1: narrow; 2: span within 2048 bitmap words; 0: wider): class-1 rows
+// to the front of out, class-0 rows to its back, class-2 rows to the front
+// of out2 (any order inside each part, as rows are independent); count[c]
+// (zero on entry) ends as class c's size.
+__global__ void __launch_bounds__(256) k_split_bin3(int n, const int *__restrict__ rows,
+                                                    const uint8_t *__restrict__ cls_of,
+                                                    int *__restrict__ out, int *__restrict__ out2,
+                                                    int *__restrict__ count) {
+    __shared__ int s_cnt[3], s_base[3];
+    if (threadIdx.x < 3) s_cnt[threadIdx.x] = 0;
     __syncthreads();
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const int row = i < n ? rows[i] : -1;
-    const int cls = i < n ? (narrow[row] ? 0 : 1) : -1;
+    const int cls = i < n ? (int)cls_of[row] : -1;
     const uint64_t below = (uint64_t(1) << lane) - 1;
     int pos = 0;
-    for (int q = 0; q < 2; q++) {
+    for (int q = 0; q < 3; q++) {
         const uint64_t mask = __ballot(cls == q);
         if (!mask) continue;
         const int leader = __ffsll((unsigned long long)mask) - 1;
@@ -155,11 +156,12 @@ __global__ void __launch_bounds__(256) k_split_narrow(int n, const int *__restri
         if (cls == q) pos = wbase + __popcll(mask & below);
     }
     __syncthreads();
-    if (threadIdx.x < 2 && s_cnt[threadIdx.x])
+    if (threadIdx.x < 3 && s_cnt[threadIdx.x])
         s_base[threadIdx.x] = atomicAdd(&count[threadIdx.x], s_cnt[threadIdx.x]);
     __syncthreads();
-    if (cls == 0) out[s_base[0] + pos] = row;
-    else if (cls == 1) out[n - 1 - (s_base[1] + pos)] = row;
+    if (cls == 1) out[s_base[1] + pos] = row;
+    else if (cls == 0) out[n - 1 - (s_base[0] + pos)] = row;
+    else if (cls == 2) out2[s_base[2] + pos] = row;
 }
 
 // Column span [lo, hi] of a row's products, from the first and last column
@@ -235,7 +237,7 @@ __global__ void __launch_bounds__(256) k_small_symbolic(int nrows, const int *__
         cnt = group_sum<G>(cnt);
         if (l == 0) {
             rownnz[row] = cnt;
-            if (narrow) narrow[row] = nw <= NARROW_WORDS && cnt <= NARROW_OUT;
+            if (narrow) narrow[row] = nw <= NARROW_WORDS && cnt <= NARROW_OUT ? 1 : 2;
         }
         return;
     }
@@ -494,6 +496,102 @@ __global__ void __launch_bounds__(RPB * G) k_small_numeric(int nrows, const int 
         for (int i = l; i < nnz; i += G) {
             jc[o + i] = sk[i];
             c[o + i] = sv[i];
+        }
+    }
+}
+
+// Rows of the <= 1024-product bin whose column span fits 2048 bitmap words
+// and that the narrow variant does not take: the rank path alone, 64 lanes
+// per row.  LDS: the span bitmap (8 KB), one output count per 4 words (2 KB),
+// the row's <= 1024 values (8 KB) and the accumulation tags -- 19.7 KB
+// instead of the hash-capable kernel's 25 KB, so 8 rows per CU are resident
+// instead of 6.  Mark, prefix, accumulate at each column's rank, write.
+constexpr int RK_WORDS = 2048;
+template <bool FUSED>
+__global__ void __launch_bounds__(64) k_small_numeric_rank(
+    int nrows, const int *__restrict__ rows, const int *__restrict__ ia,
+    const int *__restrict__ ja, const double *__restrict__ a, const int *__restrict__ ib,
+    const int *__restrict__ jb, const double *__restrict__ b, const int64_t *__restrict__ ic,
+    int *__restrict__ jc, double *__restrict__ c, int b_strict) {
+    constexpr int G = 64, NT = 4 * G;
+    __shared__ __align__(16) unsigned bmp[RK_WORDS];
+    __shared__ int pfx[RK_WORDS / 4];
+    __shared__ double acc[RK_WORDS / 2];
+    __shared__ unsigned tags[NT];
+    __shared__ int s_mark[G];
+    const int l = threadIdx.x;
+    for (int i = l; i < NT; i += G) tags[i] = 0xffffffffu;
+    s_mark[l] = -1;
+    wave_lds_fence();
+    if ((int)blockIdx.x >= nrows) return;
+    const int row = rows[blockIdx.x];
+    int lo, hi;
+    row_span<G>(ia[row], ia[row + 1], ja, ib, jb, lo, hi);
+    // the symbolic phase put the row here because its span fits RK_WORDS
+    const int nw = hi >= lo ? min(((hi - lo) >> 5) + 1, RK_WORDS) : 0;
+    const int nw4 = (nw + 3) >> 2;
+    uint4 *bmp4 = reinterpret_cast<uint4 *>(bmp);
+    const int64_t o = ic[row];
+    const int64_t nrow = ic[row + 1] - o;
+    const int nnz = (int)(nrow < RK_WORDS / 2 ? nrow : RK_WORDS / 2);
+    // columns outside [lo, lo + 32*nw) cannot occur (B rows are ascending);
+    // the unsigned test keeps every LDS index in range regardless
+    const unsigned span_cols = (unsigned)nw << 5;
+    for (int i = l; i < nw4; i += G) bmp4[i] = make_uint4(0u, 0u, 0u, 0u);
+    wave_lds_fence();
+    for_products<G, SMALL_U, false, false>(ia[row], ia[row + 1], 0, G, ja, nullptr, ib, jb,
+                                           nullptr, BlockPtrs(), 0, 0, s_mark,
+                                           [&](bool act, int col, double, double, bool) {
+                                               const int cc = col - lo;
+                                               if (act && (unsigned)cc < span_cols)
+                                                   atomicOr(&bmp[cc >> 5], 1u << (cc & 31));
+                                           });
+    wave_lds_fence();
+    int carry = 0;
+    for (int q0 = 0; q0 < nw4; q0 += G) {
+        const int q = q0 + l;
+        const uint4 wv = q < nw4 ? bmp4[q] : make_uint4(0u, 0u, 0u, 0u);
+        const int pc = __popc(wv.x) + __popc(wv.y) + __popc(wv.z) + __popc(wv.w);
+        const int incl = group_incl_scan<G>(pc);
+        if (q < nw4) pfx[q] = carry + incl - pc;
+        carry += group_last<G>(incl);
+    }
+    for (int i = l; i < nnz; i += G) acc[i] = 0.0;
+    wave_lds_fence();
+    for_products<G, SMALL_U, false, true>(
+        ia[row], ia[row + 1], 0, G, ja, a, ib, jb, b, BlockPtrs(), 0, 0, s_mark,
+        [&](bool act, int col, double av, double bv, bool single) {
+            int rank = 0;
+            const int cc = col - lo;
+            const int w = cc >> 5;
+            const bool in = act && (unsigned)cc < span_cols;
+            if (in) {
+                const uint4 wv = bmp4[w >> 2];
+                const int j = w & 3;
+                rank = pfx[w >> 2] + (j > 0 ? __popc(wv.x) : 0) + (j > 1 ? __popc(wv.y) : 0) +
+                       (j > 2 ? __popc(wv.z) : 0);
+                const unsigned cur = j == 0 ? wv.x : j == 1 ? wv.y : j == 2 ? wv.z : wv.w;
+                rank += __popc(cur & ((1u << (cc & 31)) - 1u));
+            }
+            ordered_accumulate<FUSED, NT>(in && rank < nnz, rank, av, bv, acc, tags,
+                                          (unsigned)l, single && b_strict);
+        });
+    wave_lds_fence();
+    for (int i = l; i < nnz; i += G) c[o + i] = acc[i];
+    for (int q = l; q < nw4; q += G) {
+        const uint4 wv = bmp4[q];
+        const unsigned ws[4] = {wv.x, wv.y, wv.z, wv.w};
+        int pos = pfx[q];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            unsigned word = ws[j];
+            const int cb = lo + ((4 * q + j) << 5);
+            while (word) {
+                const int bit = __ffs(word) - 1;
+                word &= word - 1;
+                if (pos < nnz) jc[o + pos] = cb + bit;
+                pos++;
+            }
         }
     }
 }
@@ -1121,12 +1219,12 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
     p.rowprod = ws.alloc_n<int64_t>((size_t)m + 1);
     p.rownnz = ws.alloc_n<int64_t>((size_t)m + 1);
     p.bin_rows = ws.alloc_n<int>((size_t)m + 1);
-    int *d_cnt = ws.alloc_n<int>(2 * NBINS + 2);
+    int *d_cnt = ws.alloc_n<int>(2 * NBINS + 4);
     int64_t *scan_tmp = ws.alloc_n<int64_t>(scan_tmp_elems((size_t)m + 1));
 
     tm.begin("row.products+bins");
     csr_row_products(m, A.ptr, A.idx, A.nnz, B.ptr, p.rowprod, ws, s);
-    TSG_HIP_TRY(hipMemsetAsync(d_cnt, 0, sizeof(int) * (2 * NBINS + 2), s));
+    TSG_HIP_TRY(hipMemsetAsync(d_cnt, 0, sizeof(int) * (2 * NBINS + 4), s));
     if (m > 0) {
         hipLaunchKernelGGL(k_bin_count, dim3(grid_for(m, 256)), dim3(256), 0, s, m, p.rowprod, lim,
                            d_cnt);
@@ -1171,13 +1269,14 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
             hipLaunchKernelGGL((k_small_symbolic<64, 2048>), dim3(grid_for(n3, 4)), dim3(256), 0,
                                s, n3, rows3, A.ptr, A.idx, B.ptr, B.idx, p.rownnz, p.narrow3);
             TSG_LAUNCH_CHECK();
-            // narrow rows first, so each numeric variant launches over its own
-            // rows; the count comes back with the phase's last host round trip
+            // each numeric variant launches over its own rows; the class
+            // sizes come back with the phase's last host round trip
             p.rows3 = ws.alloc_n<int>((size_t)n3);
-            hipLaunchKernelGGL(k_split_narrow, dim3(grid_for(n3, 256)), dim3(256), 0, s, n3, rows3,
-                               p.narrow3, p.rows3, d_cnt + 2 * NBINS);
+            p.rows3r = ws.alloc_n<int>((size_t)n3);
+            hipLaunchKernelGGL(k_split_bin3, dim3(grid_for(n3, 256)), dim3(256), 0, s, n3, rows3,
+                               p.narrow3, p.rows3, p.rows3r, d_cnt + 2 * NBINS);
             TSG_LAUNCH_CHECK();
-            TSG_HIP_TRY(hipMemcpyAsync(&p.n_narrow3, d_cnt + 2 * NBINS, sizeof(int),
+            TSG_HIP_TRY(hipMemcpyAsync(p.n3cls, d_cnt + 2 * NBINS, 3 * sizeof(int),
                                        hipMemcpyDeviceToHost, s));
         }
     }
@@ -1371,10 +1470,19 @@ static void row_numeric_t(RowPlan &p, const CsrIn &A, const CsrIn &B, const int6
                bins + st[2], st[3] - st[2]);
         const int n3 = st[4] - st[3];
         if (n3 > 0) {
+            // classes from the symbolic phase: [narrow | ... | hash] in rows3,
+            // the rank-path rows in rows3r
+            const int n_narrow = p.n3cls[1], n_hash = p.n3cls[0], n_rank = p.n3cls[2];
             launch(k_small_numeric<64, NARROW_WORDS, FUSED, 2>, 2, 64,
-                   small_numeric_row_bytes<64, NARROW_WORDS>(), p.rows3, p.n_narrow3);
+                   small_numeric_row_bytes<64, NARROW_WORDS>(), p.rows3, n_narrow);
             launch(k_small_numeric<64, 2048, FUSED, 1>, 1, 64, small_numeric_row_bytes<64, 2048>(),
-                   p.rows3 + p.n_narrow3, n3 - p.n_narrow3);
+                   p.rows3 + (n3 - n_hash), n_hash);
+            if (n_rank > 0) {
+                hipLaunchKernelGGL(k_small_numeric_rank<FUSED>, dim3((unsigned)n_rank), dim3(64), 0,
+                                   s, n_rank, p.rows3r, A.ptr, A.idx, A.val, B.ptr, B.idx, B.val,
+                                   d_ic, d_jc, d_c, (int)p.b_strict);
+                TSG_LAUNCH_CHECK();
+            }
         }
     }
     tm.end();
