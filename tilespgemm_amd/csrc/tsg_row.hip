@@ -58,8 +58,8 @@ constexpr int RANK_U = 8;
 // 2.93 ms.
 constexpr int SMALL_U = 8;
 // Rows of the 64-lane bin whose column span fits 1024 bitmap words and whose
-// outputs number at most 512 (meshes and stencils) run in a variant with a
-// 1024-entry table: 13 KB of LDS per row instead of 25 KB.
+// outputs number at most 512 (meshes and stencils) run in the rank kernel's
+// 1024-word variant: 10 KB of LDS per row instead of 25 KB.
 constexpr int NARROW_WORDS = 1024;
 constexpr int NARROW_OUT = NARROW_WORDS / 2;
 constexpr int BIG_SYM_U = 4;
@@ -500,14 +500,14 @@ __global__ void __launch_bounds__(RPB * G) k_small_numeric(int nrows, const int 
     }
 }
 
-// Rows of the <= 1024-product bin whose column span fits 2048 bitmap words
-// and that the narrow variant does not take: the rank path alone, 64 lanes
-// per row.  LDS: the span bitmap (8 KB), one output count per 4 words (2 KB),
-// the row's <= 1024 values (8 KB) and the accumulation tags -- 19.7 KB
-// instead of the hash-capable kernel's 25 KB, so 8 rows per CU are resident
-// instead of 6.  Mark, prefix, accumulate at each column's rank, write.
-constexpr int RK_WORDS = 2048;
-template <bool FUSED>
+// Rows of the <= 1024-product bin whose column span fits RK_WORDS bitmap
+// words: the rank path alone, 64 lanes per row.  LDS: the span bitmap, one
+// output count per 4 words, the row's <= RK_WORDS/2 values and the
+// accumulation tags -- 19.7 KB for 2048 words (instead of the hash-capable
+// kernel's 25 KB: 8 rows per CU resident instead of 6) and 10.2 KB for the
+// narrow rows' 1024 (instead of 13.3 KB).  Mark, prefix, accumulate at each
+// column's rank, write.
+template <bool FUSED, int RK_WORDS>
 __global__ void __launch_bounds__(64) k_small_numeric_rank(
     int nrows, const int *__restrict__ rows, const int *__restrict__ ia,
     const int *__restrict__ ja, const double *__restrict__ a, const int *__restrict__ ib,
@@ -1473,16 +1473,17 @@ static void row_numeric_t(RowPlan &p, const CsrIn &A, const CsrIn &B, const int6
             // classes from the symbolic phase: [narrow | ... | hash] in rows3,
             // the rank-path rows in rows3r
             const int n_narrow = p.n3cls[1], n_hash = p.n3cls[0], n_rank = p.n3cls[2];
-            launch(k_small_numeric<64, NARROW_WORDS, FUSED, 2>, 2, 64,
-                   small_numeric_row_bytes<64, NARROW_WORDS>(), p.rows3, n_narrow);
+            auto launch_rank = [&](auto kern, const int *rows, int nr) {
+                if (nr <= 0) return;
+                hipLaunchKernelGGL(kern, dim3((unsigned)nr), dim3(64), 0, s, nr, rows, A.ptr,
+                                   A.idx, A.val, B.ptr, B.idx, B.val, d_ic, d_jc, d_c,
+                                   (int)p.b_strict);
+                TSG_LAUNCH_CHECK();
+            };
+            launch_rank(k_small_numeric_rank<FUSED, NARROW_WORDS>, p.rows3, n_narrow);
+            launch_rank(k_small_numeric_rank<FUSED, 2048>, p.rows3r, n_rank);
             launch(k_small_numeric<64, 2048, FUSED, 1>, 1, 64, small_numeric_row_bytes<64, 2048>(),
                    p.rows3 + (n3 - n_hash), n_hash);
-            if (n_rank > 0) {
-                hipLaunchKernelGGL(k_small_numeric_rank<FUSED>, dim3((unsigned)n_rank), dim3(64), 0,
-                                   s, n_rank, p.rows3r, A.ptr, A.idx, A.val, B.ptr, B.idx, B.val,
-                                   d_ic, d_jc, d_c, (int)p.b_strict);
-                TSG_LAUNCH_CHECK();
-            }
         }
     }
     tm.end();
