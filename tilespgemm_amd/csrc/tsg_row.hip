@@ -190,6 +190,11 @@ __device__ __forceinline__ void row_span(int pa, int pe, const int *__restrict__
 }
 
 // ------------------------------------------------------ small rows: symbolic
+// Output count of each small row (rownnz), G lanes per row: a bitmap over
+// the row's column span when it fits SLOTS words, else an LDS hash of SLOTS
+// keys.  With narrow (the <= 1024-product bin), each row's numeric class: 1
+// when its span fits NARROW_WORDS words and its outputs NARROW_OUT, 2 when
+// its span fits SLOTS words, 0 otherwise.
 template <int G, int SLOTS>
 __global__ void __launch_bounds__(256) k_small_symbolic(int nrows, const int *__restrict__ rows,
                                                         const int *__restrict__ ia,
@@ -1452,9 +1457,10 @@ static void row_numeric_t(RowPlan &p, const CsrIn &A, const CsrIn &B, const int6
     const int *st = p.bin_start_h;
     tm.begin("row.small_numeric");
     {
-        // rows of <= 64, <= 256 and <= 1024 products: 16, 2 and 1 rows per block;
-        // of the <= 1024-product rows the narrow ones (the symbolic phase put
-        // them first) in the 1024-entry variant, 2 rows per block
+        // rows of <= 64 and <= 256 products: 16 and 2 rows per block of the
+        // hash-capable kernel; rows of <= 1024 products by the class the
+        // symbolic phase gave them: the rank-only kernel's 1024- and 2048-word
+        // variants, the hash-capable kernel for wider spans (1 row per block)
         auto launch = [&](auto kern, int rpb, int g, size_t per_row, const int *rows, int nr) {
             if (nr <= 0) return;
             const size_t sm = per_row * rpb;
