@@ -118,6 +118,7 @@ struct RowPlan {
     // bins: small rows (by products) and big rows
     int *bin_rows = nullptr;         // [m] rows grouped by bin
     int *bin_start = nullptr;        // [nbins+1] device
+    int2 *span = nullptr;            // [m] column span (lo, hi) of each small row
     uint8_t *narrow3 = nullptr;      // [m] class of each row of the 1024-product bin:
                                      // 1 narrow, 2 span <= 2048 words, 0 wider
     int *rows3 = nullptr;            // that bin's class-1 rows first, class-0 rows last

@@ -202,6 +202,7 @@ __global__ void __launch_bounds__(256) k_small_symbolic(int nrows, const int *__
                                                         const int *__restrict__ ib,
                                                         const int *__restrict__ jb,
                                                         int64_t *__restrict__ rownnz,
+                                                        int2 *__restrict__ span,
                                                         uint8_t *__restrict__ narrow) {
     constexpr int RPB = 256 / G;
     __shared__ __align__(16) int s_keys[RPB * SLOTS];
@@ -216,6 +217,7 @@ __global__ void __launch_bounds__(256) k_small_symbolic(int nrows, const int *__
     const int row = rows[r];
     int lo, hi;
     row_span<G>(ia[row], ia[row + 1], ja, ib, jb, lo, hi);
+    if (l == 0) span[row] = make_int2(lo, hi);
     const int nw = hi >= lo ? ((hi - lo) >> 5) + 1 : 0;
     if (nw <= SLOTS) {
         // the row's columns fit a bitmap over its span in the table's space:
@@ -288,7 +290,8 @@ __global__ void __launch_bounds__(RPB * G) k_small_numeric(int nrows, const int 
                                                        const double *__restrict__ b,
                                                        const int64_t *__restrict__ ic,
                                                        int *__restrict__ jc,
-                                                       double *__restrict__ c, int b_strict) {
+                                                       double *__restrict__ c, int b_strict,
+                                                       const int2 *__restrict__ span) {
     constexpr int NT = 4 * G;
     constexpr int TW = small_tag_words<G, SLOTS>();
     extern __shared__ __align__(16) unsigned char smem[];
@@ -309,8 +312,9 @@ __global__ void __launch_bounds__(RPB * G) k_small_numeric(int nrows, const int 
     if (r >= nrows) return;
     const int row = rows[r];
     {
-        int lo, hi;
-        row_span<G>(ia[row], ia[row + 1], ja, ib, jb, lo, hi);
+        // the row's column span, from the symbolic phase
+        const int2 sp = span[row];
+        const int lo = sp.x, hi = sp.y;
         const int nw = hi >= lo ? ((hi - lo) >> 5) + 1 : 0;
         if (nw <= SLOTS) {
             // Rank path: the row's span fits a bitmap in the table's space.
@@ -517,7 +521,7 @@ __global__ void __launch_bounds__(64) k_small_numeric_rank(
     int nrows, const int *__restrict__ rows, const int *__restrict__ ia,
     const int *__restrict__ ja, const double *__restrict__ a, const int *__restrict__ ib,
     const int *__restrict__ jb, const double *__restrict__ b, const int64_t *__restrict__ ic,
-    int *__restrict__ jc, double *__restrict__ c, int b_strict) {
+    int *__restrict__ jc, double *__restrict__ c, int b_strict, const int2 *__restrict__ span) {
     constexpr int G = 64, NT = 4 * G;
     __shared__ __align__(16) unsigned bmp[RK_WORDS];
     __shared__ int pfx[RK_WORDS / 4];
@@ -530,9 +534,10 @@ __global__ void __launch_bounds__(64) k_small_numeric_rank(
     wave_lds_fence();
     if ((int)blockIdx.x >= nrows) return;
     const int row = rows[blockIdx.x];
-    int lo, hi;
-    row_span<G>(ia[row], ia[row + 1], ja, ib, jb, lo, hi);
-    // the symbolic phase put the row here because its span fits RK_WORDS
+    // the row's column span, from the symbolic phase, which put the row here
+    // because the span fits RK_WORDS
+    const int2 sp = span[row];
+    const int lo = sp.x, hi = sp.y;
     const int nw = hi >= lo ? min(((hi - lo) >> 5) + 1, RK_WORDS) : 0;
     const int nw4 = (nw + 3) >> 2;
     uint4 *bmp4 = reinterpret_cast<uint4 *>(bmp);
@@ -1253,18 +1258,19 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
     // small rows
     tm.begin("row.small_symbolic");
     {
+        p.span = ws.alloc_n<int2>((size_t)m);
         const int *rows1 = p.bin_rows + st.s[1];
         int n1 = st.s[2] - st.s[1];
         if (n1 > 0) {
             hipLaunchKernelGGL((k_small_symbolic<16, 128>), dim3(grid_for(n1, 16)), dim3(256), 0,
-                               s, n1, rows1, A.ptr, A.idx, B.ptr, B.idx, p.rownnz, nullptr);
+                               s, n1, rows1, A.ptr, A.idx, B.ptr, B.idx, p.rownnz, p.span, nullptr);
             TSG_LAUNCH_CHECK();
         }
         const int *rows2 = p.bin_rows + st.s[2];
         int n2 = st.s[3] - st.s[2];
         if (n2 > 0) {
             hipLaunchKernelGGL((k_small_symbolic<32, 512>), dim3(grid_for(n2, 8)), dim3(256), 0,
-                               s, n2, rows2, A.ptr, A.idx, B.ptr, B.idx, p.rownnz, nullptr);
+                               s, n2, rows2, A.ptr, A.idx, B.ptr, B.idx, p.rownnz, p.span, nullptr);
             TSG_LAUNCH_CHECK();
         }
         const int *rows3 = p.bin_rows + st.s[3];
@@ -1272,7 +1278,7 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
         if (n3 > 0) {
             p.narrow3 = ws.alloc_n<uint8_t>((size_t)m);
             hipLaunchKernelGGL((k_small_symbolic<64, 2048>), dim3(grid_for(n3, 4)), dim3(256), 0,
-                               s, n3, rows3, A.ptr, A.idx, B.ptr, B.idx, p.rownnz, p.narrow3);
+                               s, n3, rows3, A.ptr, A.idx, B.ptr, B.idx, p.rownnz, p.span, p.narrow3);
             TSG_LAUNCH_CHECK();
             // each numeric variant launches over its own rows; the class
             // sizes come back with the phase's last host round trip
@@ -1467,7 +1473,7 @@ static void row_numeric_t(RowPlan &p, const CsrIn &A, const CsrIn &B, const int6
             set_smem(kern, sm);
             hipLaunchKernelGGL(kern, dim3(grid_for(nr, rpb)), dim3(rpb * g), sm, s, nr, rows,
                                A.ptr, A.idx, A.val, B.ptr, B.idx, B.val, d_ic, d_jc, d_c,
-                               (int)p.b_strict);
+                               (int)p.b_strict, p.span);
             TSG_LAUNCH_CHECK();
         };
         launch(k_small_numeric<16, 128, FUSED, 16>, 16, 16, small_numeric_row_bytes<16, 128>(),
@@ -1483,7 +1489,7 @@ static void row_numeric_t(RowPlan &p, const CsrIn &A, const CsrIn &B, const int6
                 if (nr <= 0) return;
                 hipLaunchKernelGGL(kern, dim3((unsigned)nr), dim3(64), 0, s, nr, rows, A.ptr,
                                    A.idx, A.val, B.ptr, B.idx, B.val, d_ic, d_jc, d_c,
-                                   (int)p.b_strict);
+                                   (int)p.b_strict, p.span);
                 TSG_LAUNCH_CHECK();
             };
             launch_rank(k_small_numeric_rank<FUSED, NARROW_WORDS>, p.rows3, n_narrow);
