@@ -634,11 +634,12 @@ bool host_matches(const HostCtx &x, int m, int n, int k, const int *ia, const in
 }
 
 // C entries that may stay on the device at once: the caller's cap, else all
-// of C when its 12 bytes per entry fit in 90% of the free device memory,
-// else 35% of the free memory plus what the whole product's plan holds (the
-// slabs' plans reuse that memory).
+// of C when an earlier call holds it whole already or its 12 bytes per entry
+// fit in 90% of the free device memory, else 35% of the free memory plus
+// what the whole product's plan holds (the slabs' plans reuse that memory).
 int64_t host_slab_cap(const HostCtx &x) {
     if (x.slab_cap > 0) return x.slab_cap;
+    if (x.d_jc) return x.nnz_c;   // C already held whole by an earlier call
     size_t fr = 0, tot = 0;
     TSG_HIP_TRY(hipMemGetInfo(&fr, &tot));
     if (12.0 * (double)x.nnz_c + double(1 << 20) <= 0.9 * (double)fr) return x.nnz_c;
