@@ -65,6 +65,9 @@ def _cases():
     # rows of ~576 products spanning <= 20000 columns: those with more than
     # 512 outputs run in the rank-only kernel, the others in the narrow one
     E24 = gen.erdos_renyi(20000, 24.0, seed=22)
+    # rows of ~400 products and outputs spanning up to 50000 columns (more
+    # than 1024 bitmap words): the rank-only kernel's 512-output variant
+    E20 = gen.erdos_renyi(50000, 20.0, seed=23)
     R12 = gen.rmat(12, 40_000, seed=5)
     R12T = gen.transpose(4096, 4096, *R12)
     R14 = gen.rmat(14, 250_000, seed=6)
@@ -74,6 +77,7 @@ def _cases():
         ("st27_12_AA", 1728, 1728, 1728, S12, S12, True),
         ("er4000_AA", 4000, 4000, 4000, E, E, True),
         ("er20000_d24_AA", 20000, 20000, 20000, E24, E24, True),
+        ("er50000_d20_AA", 50000, 50000, 50000, E20, E20, True),
         ("rmat12_AAT", 4096, 4096, 4096, R12, R12T, False),
         ("rmat14_AAT", 16384, 16384, 16384, R14, R14T, False),
     ]

@@ -120,10 +120,11 @@ struct RowPlan {
     int *bin_start = nullptr;        // [nbins+1] device
     int2 *span = nullptr;            // [m] column span (lo, hi) of each small row
     uint8_t *narrow3 = nullptr;      // [m] class of each row of the 1024-product bin:
-                                     // 1 narrow, 2 span <= 2048 words, 0 wider
+                                     // 1 narrow, 2/3 span <= 2048 words with > / <= 512
+                                     // outputs, 0 wider
     int *rows3 = nullptr;            // that bin's class-1 rows first, class-0 rows last
-    int *rows3r = nullptr;           // its class-2 rows
-    int n3cls[3] = {0, 0, 0};        // (host) rows per class
+    int *rows3r = nullptr;           // its class-2 rows first, class-3 rows last
+    int n3cls[4] = {0, 0, 0, 0};     // (host) rows per class
     int bin_start_h[16] = {0};       // host copy (after symbolic)
     int nbins = 0;
     // big-row structures

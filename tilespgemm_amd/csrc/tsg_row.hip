@@ -129,16 +129,16 @@ __global__ void __launch_bounds__(256) k_bin_scatter(int m, const int64_t *__res
 }
 
 // Rows of the <= 1024-product bin split by the class the symbolic phase gave
-// them (1: narrow; 2: span within 2048 bitmap words; 0: wider): class-1 rows
-// to the front of out, class-0 rows to its back, class-2 rows to the front
-// of out2 (any order inside each part, as rows are independent); count[c]
+// them (see k_small_symbolic): class-1 rows to the front of out, class-0
+// rows to its back, class-2 rows to the front of out2, class-3 rows to its
+// back (any order inside each part, as rows are independent); count[c]
 // (zero on entry) ends as class c's size.
 __global__ void __launch_bounds__(256) k_split_bin3(int n, const int *__restrict__ rows,
                                                     const uint8_t *__restrict__ cls_of,
                                                     int *__restrict__ out, int *__restrict__ out2,
                                                     int *__restrict__ count) {
-    __shared__ int s_cnt[3], s_base[3];
-    if (threadIdx.x < 3) s_cnt[threadIdx.x] = 0;
+    __shared__ int s_cnt[4], s_base[4];
+    if (threadIdx.x < 4) s_cnt[threadIdx.x] = 0;
     __syncthreads();
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
@@ -146,7 +146,7 @@ __global__ void __launch_bounds__(256) k_split_bin3(int n, const int *__restrict
     const int cls = i < n ? (int)cls_of[row] : -1;
     const uint64_t below = (uint64_t(1) << lane) - 1;
     int pos = 0;
-    for (int q = 0; q < 3; q++) {
+    for (int q = 0; q < 4; q++) {
         const uint64_t mask = __ballot(cls == q);
         if (!mask) continue;
         const int leader = __ffsll((unsigned long long)mask) - 1;
@@ -156,12 +156,13 @@ __global__ void __launch_bounds__(256) k_split_bin3(int n, const int *__restrict
         if (cls == q) pos = wbase + __popcll(mask & below);
     }
     __syncthreads();
-    if (threadIdx.x < 3 && s_cnt[threadIdx.x])
+    if (threadIdx.x < 4 && s_cnt[threadIdx.x])
         s_base[threadIdx.x] = atomicAdd(&count[threadIdx.x], s_cnt[threadIdx.x]);
     __syncthreads();
     if (cls == 1) out[s_base[1] + pos] = row;
     else if (cls == 0) out[n - 1 - (s_base[0] + pos)] = row;
     else if (cls == 2) out2[s_base[2] + pos] = row;
+    else if (cls == 3) out2[n - 1 - (s_base[3] + pos)] = row;
 }
 
 // Column span [lo, hi] of a row's products, from the first and last column
@@ -193,8 +194,9 @@ __device__ __forceinline__ void row_span(int pa, int pe, const int *__restrict__
 // Output count of each small row (rownnz), G lanes per row: a bitmap over
 // the row's column span when it fits SLOTS words, else an LDS hash of SLOTS
 // keys.  With narrow (the <= 1024-product bin), each row's numeric class: 1
-// when its span fits NARROW_WORDS words and its outputs NARROW_OUT, 2 when
-// its span fits SLOTS words, 0 otherwise.
+// when its span fits NARROW_WORDS words and its outputs NARROW_OUT, 3 when
+// its span fits SLOTS words and its outputs NARROW_OUT, 2 when its span fits
+// SLOTS words, 0 otherwise.
 template <int G, int SLOTS>
 __global__ void __launch_bounds__(256) k_small_symbolic(int nrows, const int *__restrict__ rows,
                                                         const int *__restrict__ ia,
@@ -244,7 +246,8 @@ __global__ void __launch_bounds__(256) k_small_symbolic(int nrows, const int *__
         cnt = group_sum<G>(cnt);
         if (l == 0) {
             rownnz[row] = cnt;
-            if (narrow) narrow[row] = nw <= NARROW_WORDS && cnt <= NARROW_OUT ? 1 : 2;
+            if (narrow)
+                narrow[row] = cnt <= NARROW_OUT ? (nw <= NARROW_WORDS ? 1 : 3) : 2;
         }
         return;
     }
@@ -510,13 +513,13 @@ __global__ void __launch_bounds__(RPB * G) k_small_numeric(int nrows, const int 
 }
 
 // Rows of the <= 1024-product bin whose column span fits RK_WORDS bitmap
-// words: the rank path alone, 64 lanes per row.  LDS: the span bitmap, one
-// output count per 4 words, the row's <= RK_WORDS/2 values and the
-// accumulation tags -- 19.7 KB for 2048 words (instead of the hash-capable
-// kernel's 25 KB: 8 rows per CU resident instead of 6) and 10.2 KB for the
-// narrow rows' 1024 (instead of 13.3 KB).  Mark, prefix, accumulate at each
-// column's rank, write.
-template <bool FUSED, int RK_WORDS>
+// words and whose outputs number at most RK_OUT: the rank path alone, 64
+// lanes per row.  LDS: the span bitmap, one output count per 4 words, the
+// row's values and the accumulation tags -- 19.7 KB for (2048, 1024)
+// instead of the hash-capable kernel's 25 KB (8 rows per CU resident instead
+// of 6), 15.6 KB for (2048, 512) and 10.5 KB for the narrow rows' (1024,
+// 512).  Mark, prefix, accumulate at each column's rank, write.
+template <bool FUSED, int RK_WORDS, int RK_OUT>
 __global__ void __launch_bounds__(64) k_small_numeric_rank(
     int nrows, const int *__restrict__ rows, const int *__restrict__ ia,
     const int *__restrict__ ja, const double *__restrict__ a, const int *__restrict__ ib,
@@ -525,7 +528,7 @@ __global__ void __launch_bounds__(64) k_small_numeric_rank(
     constexpr int G = 64, NT = 4 * G;
     __shared__ __align__(16) unsigned bmp[RK_WORDS];
     __shared__ int pfx[RK_WORDS / 4];
-    __shared__ double acc[RK_WORDS / 2];
+    __shared__ double acc[RK_OUT];
     __shared__ unsigned tags[NT];
     __shared__ int s_mark[G];
     const int l = threadIdx.x;
@@ -543,7 +546,7 @@ __global__ void __launch_bounds__(64) k_small_numeric_rank(
     uint4 *bmp4 = reinterpret_cast<uint4 *>(bmp);
     const int64_t o = ic[row];
     const int64_t nrow = ic[row + 1] - o;
-    const int nnz = (int)(nrow < RK_WORDS / 2 ? nrow : RK_WORDS / 2);
+    const int nnz = (int)(nrow < RK_OUT ? nrow : RK_OUT);
     // columns outside [lo, lo + 32*nw) cannot occur (B rows are ascending);
     // the unsigned test keeps every LDS index in range regardless
     const unsigned span_cols = (unsigned)nw << 5;
@@ -1287,7 +1290,7 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
             hipLaunchKernelGGL(k_split_bin3, dim3(grid_for(n3, 256)), dim3(256), 0, s, n3, rows3,
                                p.narrow3, p.rows3, p.rows3r, d_cnt + 2 * NBINS);
             TSG_LAUNCH_CHECK();
-            TSG_HIP_TRY(hipMemcpyAsync(p.n3cls, d_cnt + 2 * NBINS, 3 * sizeof(int),
+            TSG_HIP_TRY(hipMemcpyAsync(p.n3cls, d_cnt + 2 * NBINS, 4 * sizeof(int),
                                        hipMemcpyDeviceToHost, s));
         }
     }
@@ -1482,9 +1485,10 @@ static void row_numeric_t(RowPlan &p, const CsrIn &A, const CsrIn &B, const int6
                bins + st[2], st[3] - st[2]);
         const int n3 = st[4] - st[3];
         if (n3 > 0) {
-            // classes from the symbolic phase: [narrow | ... | hash] in rows3,
-            // the rank-path rows in rows3r
-            const int n_narrow = p.n3cls[1], n_hash = p.n3cls[0], n_rank = p.n3cls[2];
+            // classes from the symbolic phase: [1 | ... | 0] in rows3,
+            // [2 | ... | 3] in rows3r
+            const int n_hash = p.n3cls[0], n_narrow = p.n3cls[1], n_rank = p.n3cls[2],
+                      n_rank512 = p.n3cls[3];
             auto launch_rank = [&](auto kern, const int *rows, int nr) {
                 if (nr <= 0) return;
                 hipLaunchKernelGGL(kern, dim3((unsigned)nr), dim3(64), 0, s, nr, rows, A.ptr,
@@ -1492,8 +1496,10 @@ static void row_numeric_t(RowPlan &p, const CsrIn &A, const CsrIn &B, const int6
                                    (int)p.b_strict, p.span);
                 TSG_LAUNCH_CHECK();
             };
-            launch_rank(k_small_numeric_rank<FUSED, NARROW_WORDS>, p.rows3, n_narrow);
-            launch_rank(k_small_numeric_rank<FUSED, 2048>, p.rows3r, n_rank);
+            launch_rank(k_small_numeric_rank<FUSED, NARROW_WORDS, NARROW_OUT>, p.rows3, n_narrow);
+            launch_rank(k_small_numeric_rank<FUSED, 2048, 1024>, p.rows3r, n_rank);
+            launch_rank(k_small_numeric_rank<FUSED, 2048, NARROW_OUT>, p.rows3r + (n3 - n_rank512),
+                        n_rank512);
             launch(k_small_numeric<64, 2048, FUSED, 1>, 1, 64, small_numeric_row_bytes<64, 2048>(),
                    p.rows3 + (n3 - n_hash), n_hash);
         }
