@@ -191,7 +191,8 @@ def bench_batch(args):
     t_graph = (time.perf_counter() - t1) / args.steps
     log(f"[bench] graph-replayed numeric: {t_graph * 1e3:.2f} ms per batch")
     if tuned is not None:
-        h.set_tuning({"tile_density": tuned["tile_density"], "row_bins": tuple(tuned["row_bins"])})
+        h.set_tuning({"tile_density": tuned["tile_density"], "row_bins": tuple(tuned["row_bins"]),
+                      "block_pointer_budget": tuned["block_pointer_budget"]})
 
     # plain per-call launches of the same products
     with torch.cuda.stream(s):
@@ -221,10 +222,12 @@ def bench_batch(args):
                    "graph_numeric_gflops": round(2.0 * products / t_graph / 1e9, 2),
                    "graph_autotune": None if tuned_g is None else {
                        "row_bins": tuned_g["row_bins"], "tile_density": tuned_g["tile_density"],
+                       "block_pointer_budget": tuned_g["block_pointer_budget"],
                        "default_ms": round(tuned_g["default_ms"], 3),
                        "tuned_ms": round(tuned_g["ms"], 3)},
                    "autotune": None if tuned is None else {
                        "tile_density": tuned["tile_density"], "row_bins": tuned["row_bins"],
+                       "block_pointer_budget": tuned["block_pointer_budget"],
                        "default_ms": round(tuned["default_ms"], 3),
                        "tuned_ms": round(tuned["ms"], 3), "trials": len(tuned["trials"])}},
     }

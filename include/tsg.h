@@ -147,8 +147,11 @@ int tsg_set_bitmap_budget(tsg_handle_t handle, int64_t bytes);
 /* Memory for the B block-pointer table of the big-row path (no counterpart
  * in the reference): one position per B row and 1024-column block while the
  * table takes at most this many bytes, else blocks of 2048, 4096, ...
- * columns (up to 2^30) with a binary search inside a block.  Negative (the
- * default): 6 GiB.  Changes speed only. */
+ * columns (up to 2^30) with a binary search inside a block.  0: no table,
+ * every range is searched inside its B row.  Negative (the default): 6 GiB,
+ * and no table when building it would cost more than 8 entries per lookup
+ * it serves (a few big rows against many B rows, as in batches of small
+ * products).  Changes speed only. */
 int tsg_set_block_pointer_budget(tsg_handle_t handle, int64_t bytes);
 int tsg_get_item_order(tsg_handle_t handle, int64_t *min_items);
 /* Device workspace currently held by the handle, bytes. */

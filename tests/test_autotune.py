@@ -23,7 +23,7 @@ def test_descent_finds_separable_minimum():
                 + abs(np.log2(t["tile_density"] / target["tile_density"])))
 
     best, ms, trials = AT.coordinate_descent(cost)
-    assert best == {"tile_density": 64.0, "row_bins": (32, 128, 512)}
+    assert best == {"tile_density": 64.0, "row_bins": (32, 128, 512), "block_pointer_budget": -1}
     assert ms == pytest.approx(1.0)
     # every measured setting is valid and measured once
     keys = [repr(sorted(t.items())) for t, _ in trials]
@@ -33,7 +33,7 @@ def test_descent_finds_separable_minimum():
 
 def test_descent_keeps_default_without_gain():
     best, ms, trials = AT.coordinate_descent(lambda t: 5.0)
-    assert best == {"tile_density": 32.0, "row_bins": (64, 256, 1024)}
+    assert best == {"tile_density": 32.0, "row_bins": (64, 256, 1024), "block_pointer_budget": -1}
     assert ms == 5.0
 
 
@@ -61,7 +61,7 @@ def test_tune_takes_a_measure_and_applies_the_best():
         return 10.0 - (t["row_bins"][2] == 256) * 3.0
 
     res = AT.tune(H(), None, measure=cost)
-    assert res["row_bins"] == [64, 256, 256]
+    assert res["row_bins"] == [64, 256, 256] and res["block_pointer_budget"] == -1
     assert res["ms"] == 7.0 and res["default_ms"] == 10.0
     assert tuple(H.applied["row_bins"]) == (64, 256, 256)
 

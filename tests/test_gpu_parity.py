@@ -162,9 +162,10 @@ def test_big_rows_hash_fallback(handle, fp):
 @pytest.mark.parametrize("cap", [0, 4096 * 40])
 def test_coarse_block_pointers(handle, cap):
     """A block-pointer table capped below one entry per 1024 columns: blocks
-    widen (cap 0: one block of 2^30 columns) and every band and item range
-    comes from a binary search inside a block; same bits as the oracle, also
-    with the hash fallback and with the work sorted by column."""
+    widen, or with cap 0 no table is built and B's row pointers serve as one
+    block; every band and item range comes from a binary search inside a
+    block; same bits as the oracle, also with the hash fallback and with the
+    work sorted by column."""
     handle.set_block_pointer_budget(cap)
     try:
         m, n, k, A, B = _wide_case()

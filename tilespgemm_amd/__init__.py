@@ -347,13 +347,16 @@ class Handle:
         _check(load().tsg_set_method(self._h, method), self._h)
 
     def set_tuning(self, tuning: dict):
-        """Apply {"tile_density": x, "row_bins": (b1, b2, b3)} (either key
-        optional; see tsg_set_tile_density / tsg_set_row_bins)."""
+        """Apply {"tile_density": x, "row_bins": (b1, b2, b3),
+        "block_pointer_budget": bytes} (every key optional; see
+        tsg_set_tile_density / tsg_set_row_bins / tsg_set_block_pointer_budget)."""
         if "tile_density" in tuning:
             _check(load().tsg_set_tile_density(self._h, float(tuning["tile_density"])), self._h)
         if "row_bins" in tuning:
             b1, b2, b3 = (int(x) for x in tuning["row_bins"])
             _check(load().tsg_set_row_bins(self._h, b1, b2, b3), self._h)
+        if "block_pointer_budget" in tuning:
+            self.set_block_pointer_budget(int(tuning["block_pointer_budget"]))
 
     def tuning(self) -> dict:
         d = C.c_double(0)

@@ -11,7 +11,11 @@ so they can be searched on the workload at hand:
 * ``row_bins``       -- rows with at most (b1, b2, b3) intermediate products
                         go to the 16-, 32- and 64-lane LDS-hash kernels (hash
                         tables of 128, 512 and 2048 slots), longer rows to the
-                        big-row bitmap path.
+                        big-row bitmap path;
+* ``block_pointer_budget`` -- -1: the library decides whether the big-row
+                        path builds its B block-pointer table or searches B
+                        rows; 6 GiB: the table is built whenever it fits
+                        (a numeric phase replayed many times may prefer it).
 
 Parameters change speed only: every setting gives bitwise the same C.  The
 search is a coordinate descent from the defaults: one parameter at a time,
@@ -29,7 +33,7 @@ from typing import Callable, Dict, List, Optional
 
 import numpy as np
 
-DEFAULT = {"tile_density": 32.0, "row_bins": (64, 256, 1024)}
+DEFAULT = {"tile_density": 32.0, "row_bins": (64, 256, 1024), "block_pointer_budget": -1}
 
 # candidate values of each coordinate (validity: b1 <= b2 <= b3, b1 <= 64,
 # b2 <= 256, b3 <= 1024)
@@ -38,6 +42,7 @@ SPACE = {
     "bin2": (64, 128, 256),
     "bin1": (16, 32, 64),
     "tile_density": (8.0, 16.0, 32.0, 64.0, 1e9),
+    "block_pointer_budget": (-1, 6 << 30),
 }
 
 
@@ -47,9 +52,12 @@ def valid(t: dict) -> bool:
 
 
 def _with(t: dict, key: str, v) -> dict:
-    out = {"tile_density": t["tile_density"], "row_bins": tuple(t["row_bins"])}
+    out = {"tile_density": t["tile_density"], "row_bins": tuple(t["row_bins"]),
+           "block_pointer_budget": t.get("block_pointer_budget", -1)}
     if key == "tile_density":
         out["tile_density"] = float(v)
+    elif key == "block_pointer_budget":
+        out["block_pointer_budget"] = int(v)
     else:
         b = list(out["row_bins"])
         b[{"bin1": 0, "bin2": 1, "bin3": 2}[key]] = int(v)
@@ -63,7 +71,7 @@ def coordinate_descent(measure: Callable[[dict], float], start: Optional[dict] =
     """Minimize measure(tuning) over the space; returns (best, best_ms,
     trials) with trials a list of (tuning, ms)."""
     space = space or SPACE
-    cur = dict(start or DEFAULT)
+    cur = {**DEFAULT, **(start or {})}
     cur["row_bins"] = tuple(cur["row_bins"])
     cache: Dict[str, float] = {}
     trials: List[tuple] = []
@@ -190,8 +198,10 @@ def tune(handle, run, reps: int = 3, cache_path: Optional[str] = None,
     default_ms = next(ms for t, ms in trials if t == {**DEFAULT,
                                                       "row_bins": tuple(DEFAULT["row_bins"])})
     res = {"tile_density": best["tile_density"], "row_bins": list(best["row_bins"]),
+           "block_pointer_budget": best["block_pointer_budget"],
            "ms": best_ms, "default_ms": default_ms,
-           "trials": [[t["tile_density"], list(t["row_bins"]), ms] for t, ms in trials]}
+           "trials": [[t["tile_density"], list(t["row_bins"]), t["block_pointer_budget"], ms]
+                      for t, ms in trials]}
     handle.set_tuning(best)
     if cache_path and key:
         stored = {}

@@ -138,6 +138,7 @@ struct RowPlan {
     int blk_shift = 0;               // log2 block width (columns)
     int nblk = 0;                    // blocks per B row (+1 stored)
     int *bptr = nullptr;             // [(nblk+1) * n]: block-major with column_order
+    bool bptr_rowptr = false;        // no table: B's row pointers serve as one block
     bool column_order = false;       // big-row bands and items run in column order
     // items
     int64_t nitems = 0;

@@ -619,7 +619,8 @@ __global__ void __launch_bounds__(256) k_big_span(int nbig, const int *__restric
                                                   const int *__restrict__ ib,
                                                   const int *__restrict__ jb,
                                                   int *__restrict__ sb0,
-                                                  int64_t *__restrict__ nsb) {
+                                                  int64_t *__restrict__ nsb,
+                                                  int64_t *__restrict__ nent) {
     const int r = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int l = threadIdx.x & 63;
     if (r >= nbig) return;
@@ -641,6 +642,7 @@ __global__ void __launch_bounds__(256) k_big_span(int nbig, const int *__restric
     if (l == 0) {
         sb0[r] = hi >= lo ? lo : 0;
         nsb[r] = hi >= lo ? (int64_t)(hi - lo + 1) : 0;
+        nent[r] = ia[row + 1] - ia[row];
     }
 }
 
@@ -1178,12 +1180,17 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric_rank(
     }
 }
 
-// The plan's block pointers in the layout they were built in.
-static BlockPtrs block_ptrs(const RowPlan &p) {
+// The plan's block pointers in the layout they were built in; without a
+// table, B's row pointers b_ptr (of the call at hand) as a one-block table.
+static BlockPtrs block_ptrs(const RowPlan &p, const int *b_ptr) {
     BlockPtrs bp;
     bp.p = p.bptr;
     bp.bs = p.blk_shift;
-    if (p.column_order) {
+    if (p.bptr_rowptr) {
+        bp.p = b_ptr;
+        bp.tstride = 1;
+        bp.bstride = 1;
+    } else if (p.column_order) {
         bp.tstride = 1;
         bp.bstride = p.n;
     } else {
@@ -1308,18 +1315,23 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
         p.sb_off = ws.alloc_n<int64_t>((size_t)p.nbig + 1);
         int64_t *band_off = ws.alloc_n<int64_t>((size_t)p.nbig + 1);
         int64_t *tmp_sb = ws.alloc_n<int64_t>(scan_tmp_elems((size_t)p.nbig + 1));
+        int64_t *a_ent = ws.alloc_n<int64_t>((size_t)p.nbig + 1);
         hipLaunchKernelGGL(k_big_span, dim3(grid_for((size_t)p.nbig * 64, 256)), dim3(256), 0, s,
-                           p.nbig, rowsb, A.ptr, A.idx, B.ptr, B.idx, p.span_sb0, p.sb_off);
+                           p.nbig, rowsb, A.ptr, A.idx, B.ptr, B.idx, p.span_sb0, p.sb_off,
+                           a_ent);
         TSG_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_band_count, dim3(grid_for((size_t)p.nbig, 256)), dim3(256), 0, s,
                            p.nbig, p.sb_off, band_off);
         TSG_LAUNCH_CHECK();
         exclusive_scan_i64(p.sb_off, p.sb_off, (size_t)p.nbig, tmp_sb, s);
         exclusive_scan_i64(band_off, band_off, (size_t)p.nbig, tmp_sb, s);
-        int64_t tot2[2] = {0, 0};
+        exclusive_scan_i64(a_ent, a_ent, (size_t)p.nbig, tmp_sb, s);
+        int64_t tot2[3] = {0, 0, 0};
         TSG_HIP_TRY(hipMemcpyAsync(&tot2[0], p.sb_off + p.nbig, sizeof(int64_t),
                                    hipMemcpyDeviceToHost, s));
         TSG_HIP_TRY(hipMemcpyAsync(&tot2[1], band_off + p.nbig, sizeof(int64_t),
+                                   hipMemcpyDeviceToHost, s));
+        TSG_HIP_TRY(hipMemcpyAsync(&tot2[2], a_ent + p.nbig, sizeof(int64_t),
                                    hipMemcpyDeviceToHost, s));
         TSG_HIP_TRY(hipStreamSynchronize(s));
         const int64_t nsb_total = tot2[0], nunits = tot2[1];
@@ -1344,20 +1356,35 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
         tm.end();
 
         // B block pointers at the sub-block width (item and band boundaries
-        // then need no search) while the table stays <= 6 GiB, coarser beyond
+        // then need no search) while the table stays <= 6 GiB, coarser beyond.
+        // No table at all when building it would cost far more than the
+        // lookups it saves -- more than 8 entries per (A entry, window of 16
+        // sub-blocks) pair of the big rows, estimated from their A entries
+        // and average span (batches of small products with a few big rows) --
+        // or when the budget is 0: B's row pointers then act as a one-block
+        // table (block width 2^30 >= k) and every range comes from a search
+        // inside its B row.
         tm.begin("row.bptr");
         const size_t bptr_cap = tune.bptr_budget >= 0 ? (size_t)tune.bptr_budget
                                                       : (size_t(6) << 30);
+        const double fine_entries =
+            (double)n * (double)((((size_t)k + (1u << SUB_SHIFT) - 1) >> SUB_SHIFT) + 1);
+        const double pairs = (double)tot2[2] * ((double)nsb_total / (16.0 * p.nbig) + 1.0);
+        p.bptr_rowptr = k <= (1 << 30) &&
+                        (tune.bptr_budget == 0 || (tune.bptr_budget < 0 && fine_entries > 8 * pairs));
         int bs = SUB_SHIFT;
         while (bs < 30) {
             size_t nblk = ((size_t)k + (1u << bs) - 1) >> bs;
             if ((size_t)n * (nblk + 1) * sizeof(int) <= bptr_cap) break;
             bs++;
         }
+        if (p.bptr_rowptr) bs = 30;
         p.blk_shift = bs;
         p.nblk = (int)(((size_t)k + (1u << bs) - 1) >> bs);
-        p.bptr = ws.alloc_n<int>((size_t)n * (p.nblk + 1));
-        if (n > 0 && p.column_order) {
+        p.bptr = p.bptr_rowptr ? nullptr : ws.alloc_n<int>((size_t)n * (p.nblk + 1));
+        if (p.bptr_rowptr) {
+            // nothing to build
+        } else if (n > 0 && p.column_order) {
             hipLaunchKernelGGL(k_bptr, dim3(grid_for((size_t)n, BPT_ROWS)), dim3(256), 0, s, n,
                                B.ptr, B.idx, p.nblk, bs, p.bptr);
             TSG_LAUNCH_CHECK();
@@ -1388,7 +1415,7 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
         hipLaunchKernelGGL(k_big_symbolic,
                            dim3((unsigned)std::min<int64_t>(nunits, int64_t(1) << 30)),
                            dim3(BAND_T), 0, s, nunits, units, rowsb, k, A.ptr, A.idx, B.ptr,
-                           B.idx, block_ptrs(p), p.span_sb0, p.sb_off, p.subcnt, p.rownnz,
+                           B.idx, block_ptrs(p, B.ptr), p.span_sb0, p.sb_off, p.subcnt, p.rownnz,
                            p.rowbits);
         TSG_LAUNCH_CHECK();
         tm.end();
@@ -1512,7 +1539,7 @@ static void row_numeric_t(RowPlan &p, const CsrIn &A, const CsrIn &B, const int6
             const unsigned grid = item_grid(p.nitems);
             hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * ITEM_WAVES), sm, s, p.nitems, p.items,
                                p.item_off, p.rowbits, p.span_sb0, p.sb_off, A.ptr, A.idx, A.val,
-                               B.idx, B.val, block_ptrs(p), d_jc, d_c,
+                               B.idx, B.val, block_ptrs(p, B.ptr), d_jc, d_c,
                                (int)p.b_strict);
             TSG_LAUNCH_CHECK();
         };
@@ -1530,7 +1557,7 @@ static void row_numeric_t(RowPlan &p, const CsrIn &A, const CsrIn &B, const int6
         set_smem(kern, sm);
         const unsigned grid = item_grid(p.nitems);
         hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * ITEM_WAVES), sm, s, p.nitems, p.items,
-                           p.item_off, A.ptr, A.idx, A.val, B.ptr, B.idx, B.val, block_ptrs(p),
+                           p.item_off, A.ptr, A.idx, A.val, B.ptr, B.idx, B.val, block_ptrs(p, B.ptr),
                            d_jc, d_c, (int)p.b_strict);
         TSG_LAUNCH_CHECK();
         tm.end();
