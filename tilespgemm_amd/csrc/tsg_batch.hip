@@ -54,23 +54,40 @@ __global__ void k_cat_rows(int count, const BDesc *__restrict__ desc,
 // index rows of the concatenated B, so they move by the product's B-row
 // offset; B's columns move by the product's column offset unless LOCAL (each
 // product's C then comes out with its own columns).
+// A block takes CAT_PER * 256 consecutive entries, thread t those at t,
+// t + 256, ... (coalesced); the product of the block's first entry is
+// searched once, and each thread steps to the next product at a boundary
+// (products are contiguous and far longer than a block on average).
+constexpr int CAT_PER = 8;
 template <bool B, bool VALUES, bool LOCAL = false>
-__global__ void k_cat_entries(int count, const BDesc *__restrict__ desc,
-                              const int64_t *__restrict__ off, int64_t nnz,
-                              int *__restrict__ out_idx, double *__restrict__ out_val) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nnz) return;
+__global__ void __launch_bounds__(256) k_cat_entries(int count, const BDesc *__restrict__ desc,
+                                                     const int64_t *__restrict__ off,
+                                                     int64_t nnz, int *__restrict__ out_idx,
+                                                     double *__restrict__ out_val) {
+    const int64_t q0 = (int64_t)blockIdx.x * (256 * CAT_PER);
+    if (q0 >= nnz) return;
     const int64_t *eoff = off + (B ? OFF_B : OFF_A) * (int64_t)(count + 1);
     const int64_t *coff = off + (B ? OFF_BCOL : OFF_BROW) * (int64_t)(count + 1);
-    const int p = find_seg(eoff, count, q);
-    const int64_t local = q - eoff[p];
-    if (VALUES) {
-        out_val[q] = B ? desc[p].b[local] : desc[p].a[local];
-    } else {
-        const int *idx = B ? desc[p].jb : desc[p].ja;
-        out_idx[q] = (B && LOCAL) ? idx[local] : (int)(idx[local] + coff[p]);
+    __shared__ int s_p;
+    if (threadIdx.x == 0) s_p = find_seg(eoff, count, q0);
+    __syncthreads();
+    int p = s_p;
+#pragma unroll
+    for (int j = 0; j < CAT_PER; j++) {
+        const int64_t q = q0 + j * 256 + threadIdx.x;
+        if (q >= nnz) break;
+        while (q >= eoff[p + 1]) p++;   // stops below count: q < eoff[count]
+        const int64_t local = q - eoff[p];
+        if (VALUES) {
+            out_val[q] = B ? desc[p].b[local] : desc[p].a[local];
+        } else {
+            const int *idx = B ? desc[p].jb : desc[p].ja;
+            out_idx[q] = (B && LOCAL) ? idx[local] : (int)(idx[local] + coff[p]);
+        }
     }
 }
+
+static unsigned cat_grid(int64_t nnz) { return grid_for((size_t)nnz, 256 * CAT_PER); }
 
 // Makes C's columns local to each product: 8 lanes per row of C.
 __global__ void __launch_bounds__(256) k_local_cols(int count, const int64_t *__restrict__ off,
@@ -183,7 +200,7 @@ void batch_symbolic(tsg_handle_t h, int count, const tsg_product_t *prods, int64
                        count, dd, S.row_off, (int64_t)S.rows, S.ia);
     TSG_LAUNCH_CHECK();
     if (S.nnz_a > 0) {
-        hipLaunchKernelGGL((k_cat_entries<false, false>), dim3(grid_for((size_t)S.nnz_a, 256)),
+        hipLaunchKernelGGL((k_cat_entries<false, false>), dim3(cat_grid(S.nnz_a)),
                            dim3(256), 0, s, count, dd, S.row_off, S.nnz_a, S.ja, nullptr);
         TSG_LAUNCH_CHECK();
     }
@@ -215,11 +232,11 @@ void batch_symbolic(tsg_handle_t h, int count, const tsg_product_t *prods, int64
     if (S.nnz_b > 0 && S.jb != S.ja) {
         if (S.local_cols)
             hipLaunchKernelGGL((k_cat_entries<true, false, true>),
-                               dim3(grid_for((size_t)S.nnz_b, 256)), dim3(256), 0, s, count, dd,
+                               dim3(cat_grid(S.nnz_b)), dim3(256), 0, s, count, dd,
                                S.row_off, S.nnz_b, S.jb, nullptr);
         else
             hipLaunchKernelGGL((k_cat_entries<true, false, false>),
-                               dim3(grid_for((size_t)S.nnz_b, 256)), dim3(256), 0, s, count, dd,
+                               dim3(cat_grid(S.nnz_b)), dim3(256), 0, s, count, dd,
                                S.row_off, S.nnz_b, S.jb, nullptr);
         TSG_LAUNCH_CHECK();
     }
@@ -255,12 +272,12 @@ void batch_numeric(tsg_handle_t h, int count, const tsg_product_t *prods, const 
     hipStream_t s = h->stream;
     const BDesc *dd = static_cast<const BDesc *>(S.desc);
     if (S.nnz_a > 0) {
-        hipLaunchKernelGGL((k_cat_entries<false, true>), dim3(grid_for((size_t)S.nnz_a, 256)),
+        hipLaunchKernelGGL((k_cat_entries<false, true>), dim3(cat_grid(S.nnz_a)),
                            dim3(256), 0, s, count, dd, S.row_off, S.nnz_a, nullptr, S.a);
         TSG_LAUNCH_CHECK();
     }
     if (!S.b_is_a && S.nnz_b > 0) {
-        hipLaunchKernelGGL((k_cat_entries<true, true>), dim3(grid_for((size_t)S.nnz_b, 256)),
+        hipLaunchKernelGGL((k_cat_entries<true, true>), dim3(cat_grid(S.nnz_b)),
                            dim3(256), 0, s, count, dd, S.row_off, S.nnz_b, nullptr, S.b);
         TSG_LAUNCH_CHECK();
     }
