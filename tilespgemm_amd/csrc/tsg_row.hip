@@ -24,6 +24,8 @@
 //     sub-range of a band or item without a search; bands and items run in
 //     order of their first column, so those in flight share a band of B.
 #include "tsg_internal.h"
+
+#include <cmath>
 #include "tsg_device.h"
 #include "tsg_expand.h"
 
@@ -618,9 +620,11 @@ __global__ void __launch_bounds__(256) k_big_span(int nbig, const int *__restric
                                                   const int *__restrict__ ja,
                                                   const int *__restrict__ ib,
                                                   const int *__restrict__ jb,
+                                                  const int64_t *__restrict__ rowprod,
                                                   int *__restrict__ sb0,
                                                   int64_t *__restrict__ nsb,
-                                                  int64_t *__restrict__ nent) {
+                                                  int64_t *__restrict__ nent,
+                                                  int64_t *__restrict__ nprod) {
     const int r = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int l = threadIdx.x & 63;
     if (r >= nbig) return;
@@ -643,6 +647,7 @@ __global__ void __launch_bounds__(256) k_big_span(int nbig, const int *__restric
         sb0[r] = hi >= lo ? lo : 0;
         nsb[r] = hi >= lo ? (int64_t)(hi - lo + 1) : 0;
         nent[r] = ia[row + 1] - ia[row];
+        nprod[r] = rowprod[row];
     }
 }
 
@@ -1316,9 +1321,10 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
         int64_t *band_off = ws.alloc_n<int64_t>((size_t)p.nbig + 1);
         int64_t *tmp_sb = ws.alloc_n<int64_t>(scan_tmp_elems((size_t)p.nbig + 1));
         int64_t *a_ent = ws.alloc_n<int64_t>((size_t)p.nbig + 1);
+        int64_t *a_prod = ws.alloc_n<int64_t>((size_t)p.nbig + 1);
         hipLaunchKernelGGL(k_big_span, dim3(grid_for((size_t)p.nbig * 64, 256)), dim3(256), 0, s,
-                           p.nbig, rowsb, A.ptr, A.idx, B.ptr, B.idx, p.span_sb0, p.sb_off,
-                           a_ent);
+                           p.nbig, rowsb, A.ptr, A.idx, B.ptr, B.idx, p.rowprod, p.span_sb0,
+                           p.sb_off, a_ent, a_prod);
         TSG_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_band_count, dim3(grid_for((size_t)p.nbig, 256)), dim3(256), 0, s,
                            p.nbig, p.sb_off, band_off);
@@ -1326,12 +1332,15 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
         exclusive_scan_i64(p.sb_off, p.sb_off, (size_t)p.nbig, tmp_sb, s);
         exclusive_scan_i64(band_off, band_off, (size_t)p.nbig, tmp_sb, s);
         exclusive_scan_i64(a_ent, a_ent, (size_t)p.nbig, tmp_sb, s);
-        int64_t tot2[3] = {0, 0, 0};
+        exclusive_scan_i64(a_prod, a_prod, (size_t)p.nbig, tmp_sb, s);
+        int64_t tot2[4] = {0, 0, 0, 0};
         TSG_HIP_TRY(hipMemcpyAsync(&tot2[0], p.sb_off + p.nbig, sizeof(int64_t),
                                    hipMemcpyDeviceToHost, s));
         TSG_HIP_TRY(hipMemcpyAsync(&tot2[1], band_off + p.nbig, sizeof(int64_t),
                                    hipMemcpyDeviceToHost, s));
         TSG_HIP_TRY(hipMemcpyAsync(&tot2[2], a_ent + p.nbig, sizeof(int64_t),
+                                   hipMemcpyDeviceToHost, s));
+        TSG_HIP_TRY(hipMemcpyAsync(&tot2[3], a_prod + p.nbig, sizeof(int64_t),
                                    hipMemcpyDeviceToHost, s));
         TSG_HIP_TRY(hipStreamSynchronize(s));
         const int64_t nsb_total = tot2[0], nunits = tot2[1];
@@ -1357,10 +1366,12 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
 
         // B block pointers at the sub-block width (item and band boundaries
         // then need no search) while the table stays <= 6 GiB, coarser beyond.
-        // No table at all when building it would cost far more than the
-        // lookups it saves -- more than 8 entries per (A entry, window of 16
-        // sub-blocks) pair of the big rows, estimated from their A entries
-        // and average span (batches of small products with a few big rows) --
+        // No table at all when building it would cost more than the searches
+        // it saves -- more than 4 log2(1 + L) entries per (A entry, window of
+        // 16 sub-blocks) pair of the big rows, with the pairs estimated from
+        // their A entries and average span and L, the mean length of the B
+        // rows they reference, from their products per A entry (measured:
+        // an entry costs ~5 ps to build, a search over short rows ~70 ps) --
         // or when the budget is 0: B's row pointers then act as a one-block
         // table (block width 2^30 >= k) and every range comes from a search
         // inside its B row.
@@ -1370,8 +1381,10 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
         const double fine_entries =
             (double)n * (double)((((size_t)k + (1u << SUB_SHIFT) - 1) >> SUB_SHIFT) + 1);
         const double pairs = (double)tot2[2] * ((double)nsb_total / (16.0 * p.nbig) + 1.0);
+        const double brow = tot2[2] > 0 ? (double)tot2[3] / (double)tot2[2] : 1.0;
         p.bptr_rowptr = k <= (1 << 30) &&
-                        (tune.bptr_budget == 0 || (tune.bptr_budget < 0 && fine_entries > 8 * pairs));
+                        (tune.bptr_budget == 0 ||
+                         (tune.bptr_budget < 0 && fine_entries > 4.0 * pairs * std::log2(1.0 + brow)));
         int bs = SUB_SHIFT;
         while (bs < 30) {
             size_t nblk = ((size_t)k + (1u << bs) - 1) >> bs;
