@@ -1367,12 +1367,14 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
         // B block pointers at the sub-block width (item and band boundaries
         // then need no search) while the table stays <= 6 GiB, coarser beyond.
         // No table at all when building it would cost more than the searches
-        // it saves -- more than 4 log2(1 + L) entries per (A entry, window of
-        // 16 sub-blocks) pair of the big rows, with the pairs estimated from
-        // their A entries and average span and L, the mean length of the B
-        // rows they reference, from their products per A entry (measured:
-        // an entry costs ~5 ps to build, a search over short rows ~70 ps) --
-        // or when the budget is 0: B's row pointers then act as a one-block
+        // it saves -- more than 8 max(1, log2(L / 16)) entries per (A entry,
+        // window of 16 sub-blocks) pair of the big rows, with the pairs
+        // estimated from their A entries and average span and L, the mean
+        // length of the B rows they reference, from their products per A
+        // entry (measured on batches: an entry costs ~5 ps to build, a search
+        // ~26 ps in B rows of ~30 entries, which span one or two cache lines,
+        // and several times that in rows of hundreds, where each step misses)
+        // -- or when the budget is 0: B's row pointers then act as a one-block
         // table (block width 2^30 >= k) and every range comes from a search
         // inside its B row.
         tm.begin("row.bptr");
@@ -1384,7 +1386,8 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
         const double brow = tot2[2] > 0 ? (double)tot2[3] / (double)tot2[2] : 1.0;
         p.bptr_rowptr = k <= (1 << 30) &&
                         (tune.bptr_budget == 0 ||
-                         (tune.bptr_budget < 0 && fine_entries > 4.0 * pairs * std::log2(1.0 + brow)));
+                         (tune.bptr_budget < 0 &&
+                          fine_entries > 8.0 * pairs * std::max(1.0, std::log2(brow / 16.0))));
         int bs = SUB_SHIFT;
         while (bs < 30) {
             size_t nblk = ((size_t)k + (1u << bs) - 1) >> bs;
