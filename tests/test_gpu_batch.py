@@ -72,6 +72,24 @@ def test_mixed_batch_matches_oracle(handle, method):
         assert_csr_equal((to_host(ic), to_host(jc), to_host(c)), want)
 
 
+@pytest.mark.parametrize("method", [tsg.METHOD_ROW, tsg.METHOD_AUTO])
+def test_batch_with_empty_products(handle, method):
+    """Products whose A, B or both hold no entries between ordinary ones:
+    the concatenation steps over their empty entry ranges (a block's first
+    product is searched once, then each thread moves past empty products) and
+    every C_i equals the oracle's."""
+    E = gen.erdos_renyi(300, 6.0, seed=9)
+    Z = (np.zeros(301, np.int32), np.zeros(0, np.int32), np.zeros(0, np.float64))
+    L = gen.laplacian2d(12)
+    prods = [(300, 300, 300, E, E), (300, 300, 300, Z, E), (144, 144, 144, L, L),
+             (300, 300, 300, E, Z), (300, 300, 300, Z, Z), (300, 300, 300, E, E)]
+    pairs = [(dev(m, n, A), dev(n, k, B)) for m, n, k, A, B in prods]
+    handle.set_method(method)
+    res = handle.batch_spgemm(pairs)
+    for (m, n, k, A, B), (ic, jc, c) in zip(prods, res):
+        assert_csr_equal((to_host(ic), to_host(jc), to_host(c)), O.csr_spgemm(m, k, A, B))
+
+
 def test_all_square_batch_uses_one_operand(handle):
     """All products A_i * A_i: B is the concatenated A itself."""
     prods = [p for p in mixed_products(seed=2, count=16) if p[5]]
