@@ -33,21 +33,36 @@ __device__ __forceinline__ int find_seg(const int64_t *__restrict__ off, int cou
     return lo;
 }
 
+// Row pointers of the concatenated A (B = false) or B (B = true), plus the
+// closing entry.  A block takes CAT_PER * 256 consecutive rows, thread t
+// those at t, t + 256, ...; the product of the block's first row is searched
+// once and each thread steps to the next product at a boundary (products
+// without rows are stepped over).
+constexpr int CAT_PER = 8;
 template <bool B>
-__global__ void k_cat_rows(int count, const BDesc *__restrict__ desc,
-                           const int64_t *__restrict__ off, int64_t rows,
-                           int *__restrict__ out_ptr) {
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_cat_rows(int count, const BDesc *__restrict__ desc,
+                                                  const int64_t *__restrict__ off, int64_t rows,
+                                                  int *__restrict__ out_ptr) {
+    const int64_t r0 = (int64_t)blockIdx.x * (256 * CAT_PER);
+    if (r0 > rows) return;
     const int64_t *roff = off + (B ? OFF_BROW : OFF_ROW) * (int64_t)(count + 1);
     const int64_t *eoff = off + (B ? OFF_B : OFF_A) * (int64_t)(count + 1);
-    if (r > rows) return;
-    if (r == rows) {
-        out_ptr[r] = (int)eoff[count];
-        return;
+    __shared__ int s_p;
+    if (threadIdx.x == 0) s_p = r0 < rows ? find_seg(roff, count, r0) : 0;
+    __syncthreads();
+    int p = s_p;
+#pragma unroll
+    for (int j = 0; j < CAT_PER; j++) {
+        const int64_t r = r0 + j * 256 + threadIdx.x;
+        if (r > rows) break;
+        if (r == rows) {
+            out_ptr[r] = (int)eoff[count];
+            break;
+        }
+        while (r >= roff[p + 1]) p++;   // stops below count: r < roff[count]
+        const int *ptr = B ? desc[p].ib : desc[p].ia;
+        out_ptr[r] = (int)(eoff[p] + ptr[r - roff[p]]);
     }
-    const int p = find_seg(roff, count, r);
-    const int *ptr = B ? desc[p].ib : desc[p].ia;
-    out_ptr[r] = (int)(eoff[p] + ptr[r - roff[p]]);
 }
 
 // Entries of the concatenated A (B = false) or B (B = true).  A's columns
@@ -58,7 +73,6 @@ __global__ void k_cat_rows(int count, const BDesc *__restrict__ desc,
 // t + 256, ... (coalesced); the product of the block's first entry is
 // searched once, and each thread steps to the next product at a boundary
 // (products are contiguous and far longer than a block on average).
-constexpr int CAT_PER = 8;
 template <bool B, bool VALUES, bool LOCAL = false>
 __global__ void __launch_bounds__(256) k_cat_entries(int count, const BDesc *__restrict__ desc,
                                                      const int64_t *__restrict__ off,
@@ -196,7 +210,7 @@ void batch_symbolic(tsg_handle_t h, int count, const tsg_product_t *prods, int64
     TSG_HIP_TRY(hipMemcpyAsync(S.row_off, off.data(), sizeof(int64_t) * off.size(),
                                hipMemcpyHostToDevice, s));
     const BDesc *dd = static_cast<const BDesc *>(S.desc);
-    hipLaunchKernelGGL(k_cat_rows<false>, dim3(grid_for((size_t)S.rows + 1, 256)), dim3(256), 0, s,
+    hipLaunchKernelGGL(k_cat_rows<false>, dim3(cat_grid((int64_t)S.rows + 1)), dim3(256), 0, s,
                        count, dd, S.row_off, (int64_t)S.rows, S.ia);
     TSG_LAUNCH_CHECK();
     if (S.nnz_a > 0) {
@@ -205,7 +219,7 @@ void batch_symbolic(tsg_handle_t h, int count, const tsg_product_t *prods, int64
         TSG_LAUNCH_CHECK();
     }
     if (!b_is_a) {
-        hipLaunchKernelGGL(k_cat_rows<true>, dim3(grid_for((size_t)S.cols_a + 1, 256)), dim3(256),
+        hipLaunchKernelGGL(k_cat_rows<true>, dim3(cat_grid((int64_t)S.cols_a + 1)), dim3(256),
                            0, s, count, dd, S.row_off, (int64_t)S.cols_a, S.ib);
         TSG_LAUNCH_CHECK();
     }
