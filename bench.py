@@ -234,26 +234,6 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
-def served_mix(count, seed, edge_factor):
-    """The served request mix: 70% ER (n uniform in [1000, 16000], mean degree
-    8) C=A*A, 25% 2D 5-point Laplacian (grid side log-uniform in [64, 512])
-    C=A*A, 5% R-MAT scale 18 (edge_factor * 2^18 draws) C=A*A^T.  Operands
-    come from a pool of distinct matrices (64 ER, 16 Laplacian, 2 R-MAT)
-    drawn per request.  Returns (kinds, index into the pool) per request and
-    the pools."""
-    from tilespgemm_amd import gen
-    rng = np.random.default_rng(seed)
-    er = [gen.erdos_renyi(int(n), 8.0, seed=1000 + i)
-          for i, n in enumerate(rng.integers(1000, 16001, 64))]
-    sides = sorted(set(int(x) for x in np.exp(rng.uniform(np.log(64), np.log(512), 16))))
-    lap = [(s, gen.laplacian2d(s)) for s in sides]
-    rm = [gen.rmat(18, edge_factor << 18, seed=s) for s in (1, 2)]
-    u = rng.random(count)
-    kinds = np.where(u < 0.70, 0, np.where(u < 0.95, 1, 2))
-    pick = np.array([rng.integers(0, (len(er), len(lap), len(rm))[k]) for k in kinds])
-    return kinds, pick, er, lap, rm
-
-
 def bench_served(args):
     """Served form: an open-loop Poisson request stream against the request
     server (device-resident operands, admission by the intermediate-product
@@ -267,7 +247,7 @@ def bench_served(args):
     from tilespgemm_amd import gen
     t0 = time.time()
     nreq = args.requests
-    kinds, pick, er, lap, rm = served_mix(nreq, seed=11, edge_factor=args.edge_factor)
+    kinds, pick, er, lap, rm = gen.served_mix(nreq, seed=11, edge_factor=args.edge_factor)
     dev = "cuda:0"
     h = tsg.Handle(0)
     d_er = [tsg.DeviceCsr.from_host(X[0].size - 1, X[0].size - 1, *X, device=dev) for X in er]

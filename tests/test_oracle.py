@@ -18,7 +18,7 @@ import scipy.sparse as sp
 
 from oracle import py_oracle as O
 from tilespgemm_amd import gen
-from tests.util import assert_csr_equal, bits, golden_names, load_golden
+from tests.util import assert_csr_equal, bits, golden_names, load_golden, max_rel_error
 
 
 # ----------------------------------------------------------------- golden
@@ -133,6 +133,59 @@ def test_oracle_matches_dense_brute_force(seed):
     np.testing.assert_array_equal(c, dense[rows, jc])
     for i in range(m):                          # ascending columns per row
         assert np.all(np.diff(jc[ic[i]:ic[i + 1]]) > 0)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_absdot_matches_dense_brute_force(seed):
+    """csr_absdot (the scale of the relative error bound: sum over an
+    output's products of |a_ik * b_kj|) equals |A| @ |B| at C's structure,
+    exactly, on small integer matrices with negative values and stored
+    zeros; a sum of signed products or one operand's sign kept would not."""
+    rng = np.random.default_rng(100 + seed)
+    m, n, k = (int(x) for x in rng.integers(2, 40, 3))
+    Ad = (rng.random((m, n)) < 0.35) * rng.integers(-5, 6, (m, n))
+    Bd = (rng.random((n, k)) < 0.35) * rng.integers(-5, 6, (n, k))
+    pa, pb = rng.random((m, n)) < 0.2, rng.random((n, k)) < 0.2
+    ra, ca = np.nonzero(pa | (Ad != 0))
+    rb, cb = np.nonzero(pb | (Bd != 0))
+    A = gen.from_coo(m, ra, ca, Ad[ra, ca].astype(float))
+    B = gen.from_coo(n, rb, cb, Bd[rb, cb].astype(float))
+    ic, jc, c = O.csr_spgemm(m, k, A, B)
+    got = O.csr_absdot(m, k, A, B, ic, jc)
+    rows = np.repeat(np.arange(m), np.diff(ic))
+    want = (np.abs(Ad) @ np.abs(Bd)).astype(float)[rows, jc]
+    np.testing.assert_array_equal(got, want)
+    assert np.all(got >= np.abs(c))
+    signed = (Ad @ Bd).astype(float)[rows, jc]
+    half = (Ad @ np.abs(Bd)).astype(float)[rows, jc]
+    if np.any(signed != want):
+        assert not np.array_equal(got, signed)
+    if np.any(half != want):
+        assert not np.array_equal(got, half)
+
+
+def test_absdot_is_positive_where_products_cancel():
+    """Outputs whose products cancel to exactly 0.0 keep a positive scale, so
+    a relative error there is measured against the products' magnitude."""
+    z = load_golden("cancel_32")
+    got = O.csr_absdot(z["m"], z["k"], z["A"], z["B"], z["ic"], z["jc"])
+    zero = z["c_fma"] == 0.0
+    assert zero.any() and np.all(got[zero] > 0)
+
+
+def test_max_rel_error_measure():
+    """The reported error is max |got - want| / scale; exact results give 0,
+    one ulp moved at a unit-scale output gives 2^-52, and an output of scale
+    0 must match exactly (else the error is infinite)."""
+    want = np.array([1.0, -3.0, 0.0, 0.5])
+    scale = np.array([1.0, 4.0, 2.0, 0.0])
+    assert max_rel_error(want.copy(), want, scale) == 0.0
+    got = want.copy()
+    got[0] = np.nextafter(1.0, 2.0)
+    assert max_rel_error(got, want, scale) == 2.0 ** -52
+    got = want.copy()
+    got[3] = 0.5000001
+    assert max_rel_error(got, want, scale) == np.inf
 
 
 # ----------------------------------------------------- sensitivity (CPU)

@@ -40,3 +40,19 @@ def assert_csr_equal(got, want, values="bitwise", rtol=1e-12):
     else:
         err = np.abs(gc - wc) / np.maximum(np.abs(wc), 1e-300)
         assert err.max(initial=0.0) <= rtol, f"max rel err {err.max()}"
+
+
+def max_rel_error(got_c, want_c, scale):
+    """max over outputs of |got - want| / scale, where scale is the output's
+    sum of |a_ik * b_kj| over its products (oracle.csr_absdot): the bound a
+    sequential FP64 sum meets independently of summation order.  An output of
+    scale 0 must match exactly (its error counts as infinite otherwise)."""
+    g = np.asarray(got_c, dtype=np.float64)
+    w = np.asarray(want_c, dtype=np.float64)
+    s = np.asarray(scale, dtype=np.float64)
+    d = np.abs(g - w)
+    if d.size == 0:
+        return 0.0
+    with np.errstate(divide="ignore", invalid="ignore"):
+        r = np.where(s > 0, d / np.where(s > 0, s, 1.0), np.where(d > 0, np.inf, 0.0))
+    return float(r.max())

@@ -276,6 +276,23 @@ class Server:
         self.release(ticket)
         return ic, jc, c, res
 
+    def fetch_device(self, ticket: int):
+        """(ic int64, jc int32, c float64) as new torch tensors on the
+        server's device (device-to-device copies), then release."""
+        import torch
+        res = self.wait(ticket)
+        dev = torch.device("cuda", torch.cuda.current_device())
+        ic = torch.empty(res.m + 1, dtype=torch.int64, device=dev)
+        jc = torch.empty(max(res.nnz, 1), dtype=torch.int32, device=dev)
+        c = torch.empty(max(res.nnz, 1), dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        _d2d(ic, res.ic)
+        if res.nnz:
+            _d2d(jc, res.jc)
+            _d2d(c, res.c)
+        self.release(ticket)
+        return ic, jc[:res.nnz], c[:res.nnz], res
+
     def close(self):
         if getattr(self, "_s", None):
             load().tsg_server_destroy(self._s)
@@ -299,6 +316,18 @@ def _d2h(dst: np.ndarray, src_ptr: int):
     rc = f(dst.ctypes.data_as(C.c_void_p), C.c_void_p(src_ptr), dst.nbytes, 2)
     if rc != 0:
         raise TsgError(-5, f"hipMemcpy device->host failed ({rc})")
+
+
+def _d2d(dst, src_ptr: int):
+    """Copy device memory at src_ptr into the torch tensor dst (same
+    device; synchronous, through libtsg's HIP runtime)."""
+    f = load().hipMemcpy
+    f.restype = C.c_int
+    f.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    rc = f(C.c_void_p(dst.data_ptr()), C.c_void_p(src_ptr),
+           dst.numel() * dst.element_size(), 3)
+    if rc != 0:
+        raise TsgError(-5, f"hipMemcpy device->device failed ({rc})")
 
 
 class Handle:

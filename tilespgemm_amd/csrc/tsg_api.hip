@@ -255,12 +255,26 @@ int tsg_spgemm_symbolic(tsg_handle_t h, int m, int n, int k, const int *d_ia, co
         TSG_CHECK((vf & 0x303) == 0, TSG_ERR_INVALID,
                   std::string("invalid CSR input: ") + ((vf & 0x3) ? "A" : "B") +
                       ((vf & 0x101) ? " has bad row pointers" : " has a column out of range"));
-        const int method = choose_method(h, A, (vf & 0x404) == 0, want_sample ? sample : nullptr);
+        int method = choose_method(h, A, (vf & 0x404) == 0, want_sample ? sample : nullptr);
         h->method_used = method;
         if (method == TSG_METHOD_TILE) {
             const bool b_is_a = d_ia == d_ib && d_ja == d_jb && m == n && n == k;
-            tile_symbolic(h->tile, A, B, b_is_a, d_ic, nnz_c, h->ws, h->stream, h->tm, h->cus);
-        } else {
+            try {
+                tile_symbolic(h->tile, A, B, b_is_a, d_ic, nnz_c, h->ws, h->stream, h->tm,
+                              h->cus);
+            } catch (const TsgError &e) {
+                // AUTO's choice falls back to the row path when the tile
+                // path's int32 tile indices would overflow; a forced tile
+                // path reports the overflow
+                if (e.code != TSG_ERR_OVERFLOW || h->method != TSG_METHOD_AUTO) throw;
+                TSG_HIP_TRY(hipStreamSynchronize(h->stream));
+                h->tile.valid = false;
+                h->ws.reset();
+                method = TSG_METHOD_ROW;
+                h->method_used = method;
+            }
+        }
+        if (method == TSG_METHOD_ROW) {
             // The row path reads every B row in column order (spans from a
             // row's first and last column, block pointers by search), so B
             // rows that are not strictly ascending are replaced by a stably
@@ -398,7 +412,7 @@ static bool plan_array(tsg_handle_t h, int which, const void **ptr, size_t *byte
     case TSG_B_MASK: *ptr = B.mask; *bytes = 2 * 16 * (size_t)B.numblk; return true;
     case TSG_B_ROWPTR: *ptr = B.rowptr; *bytes = 16 * (size_t)B.numblk; return true;
     case TSG_B_IDX: *ptr = B.idx; *bytes = (size_t)B.nnz; return true;
-    case TSG_B_VAL: *ptr = B.val; *bytes = 8 * (size_t)B.nnz; return true;
+    case TSG_B_VAL: *ptr = p.b_val_shared ? A.val : B.val; *bytes = 8 * (size_t)B.nnz; return true;
     case TSG_B_CSC_PTR: *ptr = B.csc_ptr; *bytes = 4 * ((size_t)B.blkn + 1); return true;
     case TSG_B_CSC_ROW: *ptr = B.csc_row; *bytes = 4 * (size_t)B.numblk; return true;
     case TSG_B_CSC_TILE: *ptr = B.csc_tile; *bytes = 4 * (size_t)B.numblk; return true;
@@ -549,13 +563,18 @@ struct HostCtx {
     tsg_handle_t h = nullptr;
     Arena buf;     // device copies of the inputs, ic, and C when kept whole
     Arena sbuf;    // per-slab row pointers and C of the slabbed numeric phase
-    // identity of the cached symbolic phase
+    // identity of the cached symbolic phase: sizes, pointers and a checksum
+    // of the index arrays' contents (a caller may rewrite them in place)
     bool have = false;
     bool plan_whole = false;   // the handle holds the whole product's plan
     int m = -1, n = -1, k = -1;
     const void *ia = nullptr, *ja = nullptr, *ib = nullptr, *jb = nullptr;
+    uint64_t sum = 0;
     int64_t nnz_a = -1, nnz_b = -1, nnz_c = -1;
     int *d_ia = nullptr, *d_ja = nullptr, *d_ib = nullptr, *d_jb = nullptr;
+    bool shared_idx = false;   // B's row pointers and columns are A's (same host arrays)
+    // B's values always have a buffer of their own: a caller may pass B with
+    // A's index arrays and different values
     double *d_a = nullptr, *d_b = nullptr;
     int64_t *d_ic = nullptr;
     int *d_jc = nullptr;       // whole C (allocated by the first numeric call)
@@ -574,6 +593,39 @@ HostCtx &host_ctx() {
 int env_device() {
     const char *e = std::getenv("TSG_DEVICE");
     return e ? std::atoi(e) : 0;
+}
+
+// 64-bit checksum of an int32 array: four independent multiply-rotate lanes
+// over 8-byte words (several GB/s on one core), then the tail.
+uint64_t checksum_i32(const int *p, int64_t n, uint64_t seed) {
+    const uint64_t K = 0x9E3779B97F4A7C15ull;
+    auto mix = [K](uint64_t h, uint64_t v) {
+        h ^= v * K;
+        return ((h << 29) | (h >> 35)) * 0xBF58476D1CE4E5B9ull;
+    };
+    uint64_t h[4] = {seed, seed ^ 1, seed ^ 2, seed ^ 3};
+    const int64_t words = n / 2;
+    int64_t w = 0;
+    for (; w + 4 <= words; w += 4) {
+        uint64_t v[4];
+        std::memcpy(v, p + 2 * w, sizeof(v));
+        for (int j = 0; j < 4; j++) h[j] = mix(h[j], v[j]);
+    }
+    uint64_t r = mix(mix(h[0], h[1]), mix(h[2], h[3]));
+    for (int64_t i = 2 * w; i < n; i++) r = mix(r, (uint32_t)p[i]);
+    return mix(r, (uint64_t)n);
+}
+
+uint64_t index_checksum(int m, int n, const int *ia, const int *ja, const int *ib,
+                        const int *jb) {
+    const int64_t nnz_a = ia[m] - ia[0], nnz_b = ib[n] - ib[0];
+    uint64_t s = checksum_i32(ia, (int64_t)m + 1, 1);
+    if (nnz_a) s = checksum_i32(ja, nnz_a, s);
+    if (ib != ia || jb != ja) {
+        s = checksum_i32(ib, (int64_t)n + 1, s);
+        if (nnz_b) s = checksum_i32(jb, nnz_b, s);
+    }
+    return s;
 }
 
 // Uploads the inputs, runs the symbolic phase and remembers its identity.
@@ -596,15 +648,15 @@ void host_symbolic(HostCtx &x, int m, int n, int k, const int *ia, const int *ja
     x.d_ja = x.buf.alloc_n<int>((size_t)nnz_a + 1);
     x.d_a = x.buf.alloc_n<double>((size_t)nnz_a + 1);
     const bool same = ia == ib && ja == jb && m == n;
+    x.shared_idx = same;
     if (same) {
         x.d_ib = x.d_ia;
         x.d_jb = x.d_ja;
-        x.d_b = x.d_a;
     } else {
         x.d_ib = x.buf.alloc_n<int>((size_t)n + 1);
         x.d_jb = x.buf.alloc_n<int>((size_t)nnz_b + 1);
-        x.d_b = x.buf.alloc_n<double>((size_t)nnz_b + 1);
     }
+    x.d_b = x.buf.alloc_n<double>((size_t)nnz_b + 1);
     x.d_ic = x.buf.alloc_n<int64_t>((size_t)m + 1);
     hipStream_t s = x.h->stream;
     TSG_HIP_TRY(hipMemcpyAsync(x.d_ia, ia, sizeof(int) * ((size_t)m + 1), hipMemcpyHostToDevice, s));
@@ -623,14 +675,19 @@ void host_symbolic(HostCtx &x, int m, int n, int k, const int *ia, const int *ja
     x.plan_whole = true;
     x.m = m; x.n = n; x.k = k;
     x.ia = ia; x.ja = ja; x.ib = ib; x.jb = jb;
+    x.sum = index_checksum(m, n, ia, ja, ib, jb);
     x.nnz_a = nnz_a; x.nnz_b = nnz_b; x.nnz_c = nnz_c;
 }
 
+// The cached plan serves a request=0 call only for the same sizes, the same
+// arrays and the same contents of those arrays: a caller that rewrote ia, ja,
+// ib or jb in place between the two calls gets a fresh symbolic phase.
 bool host_matches(const HostCtx &x, int m, int n, int k, const int *ia, const int *ja,
                   const int *ib, const int *jb) {
     return x.have && x.m == m && x.n == n && x.k == k && x.ia == ia && x.ja == ja &&
            x.ib == ib && x.jb == jb && x.nnz_a == (int64_t)(ia[m] - ia[0]) &&
-           x.nnz_b == (int64_t)(ib[n] - ib[0]);
+           x.nnz_b == (int64_t)(ib[n] - ib[0]) &&
+           x.sum == index_checksum(m, n, ia, ja, ib, jb);
 }
 
 // C entries that may stay on the device at once: the caller's cap, else all
@@ -660,8 +717,15 @@ void host_numeric(HostCtx &x, const int *ia, const IC *ic, const double *a, cons
     hipStream_t s = x.h->stream;
     if (x.nnz_a)
         TSG_HIP_TRY(hipMemcpyAsync(x.d_a, a, sizeof(double) * x.nnz_a, hipMemcpyHostToDevice, s));
-    if (x.d_b != x.d_a && x.nnz_b)
-        TSG_HIP_TRY(hipMemcpyAsync(x.d_b, b, sizeof(double) * x.nnz_b, hipMemcpyHostToDevice, s));
+    // B's values are A's device copy only when B is A (same index arrays and
+    // the same value array); otherwise they go to B's own buffer
+    const double *d_b = x.d_a;
+    if (!(x.shared_idx && b == a)) {
+        if (x.nnz_b)
+            TSG_HIP_TRY(hipMemcpyAsync(x.d_b, b, sizeof(double) * x.nnz_b, hipMemcpyHostToDevice,
+                                       s));
+        d_b = x.d_b;
+    }
     const int64_t cap = host_slab_cap(x);
     if (x.nnz_c <= cap) {
         if (!x.plan_whole) {
@@ -677,7 +741,7 @@ void host_numeric(HostCtx &x, const int *ia, const IC *ic, const double *a, cons
             x.d_c = x.buf.alloc_n<double>((size_t)x.nnz_c + 1);
         }
         int rc = tsg_spgemm_numeric(x.h, x.m, x.n, x.k, x.d_ia, x.d_ja, x.d_a, x.nnz_a, x.d_ib,
-                                    x.d_jb, x.d_b, x.nnz_b, x.d_ic, x.d_jc, x.d_c);
+                                    x.d_jb, d_b, x.nnz_b, x.d_ic, x.d_jc, x.d_c);
         if (rc != TSG_OK) throw TsgError{rc, tsg_last_error(x.h)};
         x.stage.d2h(jc, x.d_jc, sizeof(int) * x.nnz_c, s);
         x.stage.d2h(c, x.d_c, sizeof(double) * x.nnz_c, s);
@@ -726,7 +790,7 @@ void host_numeric(HostCtx &x, const int *ia, const IC *ic, const double *a, cons
         TSG_CHECK(nz == (int64_t)ic[r1] - o, TSG_ERR_STATE,
                   "a slab's symbolic phase disagrees with ic");
         rc = tsg_spgemm_numeric(x.h, ms, x.n, x.k, d_ia_s, x.d_ja + e0, x.d_a + e0, ea, x.d_ib,
-                                x.d_jb, x.d_b, x.nnz_b, d_ic_s, d_jc_s, d_c_s);
+                                x.d_jb, d_b, x.nnz_b, d_ic_s, d_jc_s, d_c_s);
         if (rc != TSG_OK) throw TsgError{rc, tsg_last_error(x.h)};
         x.stage.d2h(jc + o, d_jc_s, sizeof(int) * nz, s);
         x.stage.d2h(c + o, d_c_s, sizeof(double) * nz, s);

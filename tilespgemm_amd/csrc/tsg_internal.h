@@ -61,8 +61,10 @@ private:
 size_t scan_tmp_elems(size_t n);
 void exclusive_scan_i64(const int64_t *in, int64_t *out, size_t n, int64_t *tmp,
                         hipStream_t s);
+// The int32 scan sums in int64; with total set, the exact int64 grand total
+// is written there too (the caller checks it against INT32_MAX).
 void exclusive_scan_i32(const int32_t *in, int32_t *out, size_t n, int64_t *tmp,
-                        hipStream_t s);
+                        hipStream_t s, int64_t *total = nullptr);
 
 // -------------------------------------------------------------- helpers
 static inline unsigned grid_for(size_t work, unsigned per_block, unsigned cap = 1u << 30) {
@@ -185,7 +187,8 @@ struct TileMat {
 
 struct TilePlan {
     bool valid = false;
-    bool b_is_a = false;
+    bool b_is_a = false;         // B has A's structure (same index arrays)
+    bool b_val_shared = false;   // last numeric phase: B's values were A's
     TileMat A, B;
     int numblkC = 0;
     int *tptrC = nullptr;        // [blkm+1]
@@ -199,6 +202,8 @@ struct TilePlan {
     int64_t nnzC = 0;
 };
 
+// Throws TSG_ERR_OVERFLOW unless 0 <= n_c_tiles <= INT32_MAX.
+void check_tile_count(int64_t n_c_tiles);
 void tile_symbolic(TilePlan &p, const CsrIn &A, const CsrIn &B, bool b_is_a,
                    int64_t *d_ic, int64_t *nnz_c, Arena &ws, hipStream_t s,
                    StageTimer &tm, int cus);

@@ -514,6 +514,12 @@ void convert(TileMat &T, const CsrIn &X, bool fill_values, Arena &ws, hipStream_
 
 }  // namespace
 
+void check_tile_count(int64_t n_c_tiles) {
+    TSG_CHECK(n_c_tiles >= 0 && n_c_tiles <= INT_MAX, TSG_ERR_OVERFLOW,
+              "the tile path indexes C tiles with int32: " + std::to_string(n_c_tiles) +
+                  " C tiles exceed 2^31-1 (use the row path)");
+}
+
 void tile_symbolic(TilePlan &p, const CsrIn &A, const CsrIn &B, bool b_is_a, int64_t *d_ic,
                    int64_t *nnz_c, Arena &ws, hipStream_t s, StageTimer &tm, int cus) {
     (void)cus;
@@ -523,7 +529,10 @@ void tile_symbolic(TilePlan &p, const CsrIn &A, const CsrIn &B, bool b_is_a, int
     tm.begin("tile.convert");
     convert(p.A, A, false, ws, s);
     if (b_is_a) {
+        // B shares A's structure; its values get a buffer of their own, used
+        // when a numeric phase passes B values other than A's
         p.B = p.A;
+        p.B.val = ws.alloc_n<double>((size_t)B.nnz + 1);
     } else {
         convert(p.B, B, false, ws, s);
     }
@@ -552,10 +561,16 @@ void tile_symbolic(TilePlan &p, const CsrIn &A, const CsrIn &B, bool b_is_a, int
                            nullptr, nullptr, nullptr);
         TSG_LAUNCH_CHECK();
     }
-    exclusive_scan_i32(cnt, p.tptrC, (size_t)blkm, tmp, s);
-    int nC = 0;
-    TSG_HIP_TRY(hipMemcpyAsync(&nC, p.tptrC + blkm, sizeof(int), hipMemcpyDeviceToHost, s));
+    // C tiles are counted in int64: the tile path indexes them with int32,
+    // so a product with 2^31 or more C tiles is refused before anything is
+    // sized from the count
+    int64_t *d_nC = ws.alloc_n<int64_t>(1);
+    exclusive_scan_i32(cnt, p.tptrC, (size_t)blkm, tmp, s, d_nC);
+    int64_t nC64 = 0;
+    TSG_HIP_TRY(hipMemcpyAsync(&nC64, d_nC, sizeof(int64_t), hipMemcpyDeviceToHost, s));
     TSG_HIP_TRY(hipStreamSynchronize(s));
+    check_tile_count(nC64);
+    const int nC = (int)nC64;
     p.numblkC = nC;
     p.tcolC = ws.alloc_n<int>((size_t)nC + 1);
     p.trowC = ws.alloc_n<int>((size_t)nC + 1);
@@ -595,14 +610,14 @@ void tile_symbolic(TilePlan &p, const CsrIn &A, const CsrIn &B, bool b_is_a, int
 }
 
 template <bool FUSED>
-static void tile_numeric_t(TilePlan &p, const int64_t *d_ic, int *d_jc, double *d_c,
-                           hipStream_t s) {
+static void tile_numeric_t(TilePlan &p, const double *valB, const int64_t *d_ic, int *d_jc,
+                           double *d_c, hipStream_t s) {
     const TileMat &TA = p.A, &TB = p.B;
     if (p.numblkC > 0) {
         hipLaunchKernelGGL(k_step3<FUSED>, dim3(grid_for(p.numblkC, S3_TILES)),
                            dim3(64 * S3_WAVES), 0, s, p.numblkC, p.trowC, p.tcolC, TA.tile_ptr,
                            TA.tile_col, TA.tile_off, TA.rowptr, TA.idx, TA.val, TB.csc_ptr,
-                           TB.csc_row, TB.csc_tile, TB.tile_off, TB.rowptr, TB.idx, TB.val,
+                           TB.csc_row, TB.csc_tile, TB.tile_off, TB.rowptr, TB.idx, valB,
                            p.maskC, p.rowoffC, d_ic, d_jc, d_c);
         TSG_LAUNCH_CHECK();
     }
@@ -620,17 +635,22 @@ void tile_numeric(TilePlan &p, const CsrIn &A, const CsrIn &B, const int64_t *d_
                            A.nnz, p.A.perm, A.val, p.A.val);
         TSG_LAUNCH_CHECK();
     }
-    if (!p.b_is_a && B.nnz > 0) {
+    // B's values: A's tile payload when B is A (same structure, same value
+    // array), else scattered into B's own payload (with A's structure, by
+    // A's permutation)
+    p.b_val_shared = p.b_is_a && B.val == A.val;
+    if (!p.b_val_shared && B.nnz > 0) {
         hipLaunchKernelGGL(k_perm_scatter, dim3(grid_for((size_t)B.nnz, 256)), dim3(256), 0, s,
                            B.nnz, p.B.perm, B.val, p.B.val);
         TSG_LAUNCH_CHECK();
     }
+    const double *valB = p.b_val_shared ? p.A.val : p.B.val;
     tm.end();
     tm.begin("tile.step3");
     if (fused)
-        tile_numeric_t<true>(p, d_ic, d_jc, d_c, s);
+        tile_numeric_t<true>(p, valB, d_ic, d_jc, d_c, s);
     else
-        tile_numeric_t<false>(p, d_ic, d_jc, d_c, s);
+        tile_numeric_t<false>(p, valB, d_ic, d_jc, d_c, s);
     tm.end();
 }
 

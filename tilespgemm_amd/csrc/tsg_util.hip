@@ -88,7 +88,8 @@ __global__ void __launch_bounds__(SCAN_T) k_scan_reduce(const T *in, size_t n, l
 
 template <class T>
 __global__ void __launch_bounds__(SCAN_T) k_scan_apply(const T *in, T *out, size_t n,
-                                                        const long long *offs, int has_offs) {
+                                                        const long long *offs, int has_offs,
+                                                        long long *total) {
     size_t base = (size_t)blockIdx.x * SCAN_BLK + threadIdx.x * SCAN_PER;
     T v[SCAN_PER];
     long long s = 0;
@@ -105,19 +106,27 @@ __global__ void __launch_bounds__(SCAN_T) k_scan_apply(const T *in, T *out, size
         if (base + i < n) out[base + i] = (T)run;
         run += (long long)v[i];
     }
-    // out[n] = grand total, written by the thread that owns element n-1
-    if (n > 0 && base <= n - 1 && n - 1 < base + SCAN_PER) out[n] = (T)run;
-    if (n == 0 && blockIdx.x == 0 && threadIdx.x == 0) out[0] = T(0);
+    // out[n] = grand total, written by the thread that owns element n-1; the
+    // exact 64-bit total also goes to *total when asked for (T may be narrower)
+    if (n > 0 && base <= n - 1 && n - 1 < base + SCAN_PER) {
+        out[n] = (T)run;
+        if (total) *total = run;
+    }
+    if (n == 0 && blockIdx.x == 0 && threadIdx.x == 0) {
+        out[0] = T(0);
+        if (total) *total = 0;
+    }
 }
 
 size_t level_elems(size_t n) { return (n + SCAN_BLK - 1) / SCAN_BLK; }
 
 template <class T>
-void scan_rec(const T *in, T *out, size_t n, long long *tmp, hipStream_t s) {
+void scan_rec(const T *in, T *out, size_t n, long long *tmp, hipStream_t s,
+              long long *total = nullptr) {
     size_t nb = level_elems(n);
     if (nb <= 1) {
         hipLaunchKernelGGL(k_scan_apply<T>, dim3(1), dim3(SCAN_T), 0, s, in, out, n,
-                           (const long long *)nullptr, 0);
+                           (const long long *)nullptr, 0, total);
         TSG_LAUNCH_CHECK();
         return;
     }
@@ -127,7 +136,7 @@ void scan_rec(const T *in, T *out, size_t n, long long *tmp, hipStream_t s) {
     TSG_LAUNCH_CHECK();
     scan_rec<long long>(sums, sums, nb, rest, s);
     hipLaunchKernelGGL(k_scan_apply<T>, dim3((unsigned)nb), dim3(SCAN_T), 0, s, in, out, n,
-                       (const long long *)sums, 1);
+                       (const long long *)sums, 1, total);
     TSG_LAUNCH_CHECK();
 }
 }  // namespace
@@ -149,8 +158,9 @@ void exclusive_scan_i64(const int64_t *in, int64_t *out, size_t n, int64_t *tmp,
     scan_rec<long long>((const long long *)in, (long long *)out, n, (long long *)tmp, s);
 }
 
-void exclusive_scan_i32(const int32_t *in, int32_t *out, size_t n, int64_t *tmp, hipStream_t s) {
-    scan_rec<int>(in, out, n, (long long *)tmp, s);
+void exclusive_scan_i32(const int32_t *in, int32_t *out, size_t n, int64_t *tmp, hipStream_t s,
+                        int64_t *total) {
+    scan_rec<int>(in, out, n, (long long *)tmp, s, (long long *)total);
 }
 
 // ================================================================= timer

@@ -21,7 +21,7 @@ from __future__ import annotations
 import numpy as np
 
 __all__ = [
-    "laplacian2d", "stencil27", "rmat", "erdos_renyi", "er_batch", "from_coo",
+    "laplacian2d", "stencil27", "rmat", "erdos_renyi", "er_batch", "served_mix", "from_coo",
     "transpose", "intermediate_products", "ref_values",
 ]
 
@@ -142,6 +142,26 @@ def er_batch(count: int, seed: int = 0, nmin: int = 1000, nmax: int = 65536,
         out.append((n, (indptr.astype(np.int32), (key % n).astype(np.int32),
                         rng.uniform(-1.0, 1.0, key.size))))
     return out
+
+
+def served_mix(count: int, seed: int = 11, edge_factor: int = 4):
+    """The served request mix: 70% ER (n uniform in [1000, 16000], mean
+    degree 8) C=A*A, 25% 2D 5-point Laplacian (grid side log-uniform in
+    [64, 512]) C=A*A, 5% R-MAT scale 18 (edge_factor * 2^18 draws) C=A*A^T.
+    Operands come from a pool of distinct matrices (64 ER, up to 16
+    Laplacians, 2 R-MAT) drawn per request.  Returns (kind per request: 0 ER,
+    1 Laplacian, 2 R-MAT; index into that kind's pool; ER pool; Laplacian pool
+    of (side, CSR); R-MAT pool)."""
+    rng = np.random.default_rng(seed)
+    er = [erdos_renyi(int(n), 8.0, seed=1000 + i)
+          for i, n in enumerate(rng.integers(1000, 16001, 64))]
+    sides = sorted(set(int(x) for x in np.exp(rng.uniform(np.log(64), np.log(512), 16))))
+    lap = [(s, laplacian2d(s)) for s in sides]
+    rm = [rmat(18, edge_factor << 18, seed=s) for s in (1, 2)]
+    u = rng.random(count)
+    kinds = np.where(u < 0.70, 0, np.where(u < 0.95, 1, 2))
+    pick = np.array([rng.integers(0, (len(er), len(lap), len(rm))[k]) for k in kinds])
+    return kinds, pick, er, lap, rm
 
 
 def transpose(m: int, n: int, indptr, indices, data):
