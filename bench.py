@@ -65,10 +65,25 @@ def csr_bytes(rows, nnz, ptr_bytes):
     return (rows + 1) * ptr_bytes + nnz * 12
 
 
-def cpu_baseline(A, B, m, n, k, target_s=15.0):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(A, B, m, n, k, target_s=15.0, gpu_rows=None):
     """The reference thmkl_dcsrmultcsr (compiled from its sources into
     oracle/_ref) on every stride-th row of A, timed as its main() does
-    (both phases plus the malloc between, thmkl_dcsrmultcsr.c:190-210)."""
+    (both phases plus the malloc between, thmkl_dcsrmultcsr.c:190-210).
+    With gpu_rows (rows -> the GPU's (ic, jc, c) for those rows of C), every
+    8th row of the sample is also checked against the reference's output:
+    structure exactly, values bit for bit, and the max relative error
+    |gpu - cpu| / sum |a_ik b_kj| is reported."""
     from oracle import py_oracle as O
     kind = "reference" if O.ref_available("fma") else "port"
     ia, ja, a = (np.asarray(x) for x in A)
@@ -83,13 +98,17 @@ def cpu_baseline(A, B, m, n, k, target_s=15.0):
             np.zeros(0, np.int64)
         return rows.size, (sp, ja[idx], a[idx]), int(blen[ja[idx]].sum())
 
+    last = {}
+
     def run(ms, As):
         t0 = time.perf_counter()
         if kind == "reference":
-            O.ref_thmkl(ms, n, k, As, B, "fma")
+            out = O.ref_thmkl(ms, n, k, As, B, "fma")
         else:
-            O.csr_spgemm(ms, k, As, B)
-        return time.perf_counter() - t0
+            out = O.csr_spgemm(ms, k, As, B)
+        t = time.perf_counter() - t0
+        last["out"] = out
+        return t
 
     # the reference keeps C's row pointers in int32 (common.h:15-17, no overflow
     # check): keep the sample's intermediate products (>= its nnz(C)) < 1.5e9
@@ -104,16 +123,64 @@ def cpu_baseline(A, B, m, n, k, target_s=15.0):
         # grow the sample towards target_s (at most 64x per round)
         stride = max(min_stride, int(stride * max(t, target_s / 64) / target_s))
     threads = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
-    return {"value": 2.0 * prods / t / 1e9, "unit": "GFLOP/s", "cores": threads, "kind": kind,
-            "sample": f"every {stride}th row of A ({ms} rows, {prods} intermediate products), "
-                      f"full B; {t:.2f} s"}
+    res = {"value": 2.0 * prods / t / 1e9, "unit": "GFLOP/s", "cores": threads, "kind": kind,
+           "cpu_model": cpu_model(),
+           "build": "gcc -O3 -march=x86-64-v3 -fopenmp (FMA contraction, as the reference's "
+                    "-march=native build)" if kind == "reference" else "oracle/csr_oracle.c",
+           "sample": f"every {stride}th row of A ({ms} rows, {prods} intermediate products), "
+                     f"full B; {t:.2f} s"}
+    if gpu_rows is not None:
+        ric, rjc, rc = (np.asarray(x) for x in last["out"])
+        sel = np.arange(0, ms, 8)
+        rows = np.arange(0, m, stride)[sel]
+        gic, gjc, gc = gpu_rows(rows)
+        lens = (ric[sel + 1] - ric[sel]).astype(np.int64)
+        wic = np.zeros(sel.size + 1, dtype=np.int64)
+        np.cumsum(lens, out=wic[1:])
+        pos = np.concatenate([np.arange(ric[i], ric[i + 1]) for i in sel])
+        wjc, wc = rjc[pos], rc[pos]
+        same_struct = np.array_equal(np.asarray(gic, np.int64), wic) and \
+            np.array_equal(np.asarray(gjc, np.int64), wjc.astype(np.int64))
+        bitwise = same_struct and np.array_equal(np.asarray(gc).view(np.int64),
+                                                 np.ascontiguousarray(wc).view(np.int64))
+        err = None
+        if same_struct:
+            ia_ = np.asarray(A[0])
+            sub_lens = (ia_[rows + 1] - ia_[rows]).astype(np.int64)
+            sp_ = np.zeros(rows.size + 1, dtype=np.int64)
+            np.cumsum(sub_lens, out=sp_[1:])
+            idx = np.concatenate([np.arange(ia_[r], ia_[r + 1]) for r in rows])
+            As_ = (sp_, np.asarray(A[1])[idx], np.asarray(A[2])[idx])
+            scale = O.csr_absdot(rows.size, k, As_, B, wic, wjc)
+            d = np.abs(np.asarray(gc) - wc)
+            with np.errstate(divide="ignore", invalid="ignore"):
+                r = np.where(scale > 0, d / np.where(scale > 0, scale, 1.0),
+                             np.where(d > 0, np.inf, 0.0))
+            err = float(r.max()) if r.size else 0.0
+        res["parity"] = {"rows": int(rows.size), "outputs": int(pos.size),
+                         "structure_equal": bool(same_struct), "bitwise": bool(bitwise),
+                         "max_rel_err": err,
+                         "what": "every 8th row of the CPU sample: GPU C rows against the "
+                                 "reference's output for them; error scale sum |a_ik b_kj|"}
+    return res
 
 
-def bench_batch(args):
-    """Batched form: 512 independent ER products C_i = A_i * A_i in one
-    block-diagonal launch sequence, against plain per-call launches of the
-    same products on the same handle, and against the numeric phase captured
-    once in a HIP graph and replayed."""
+def _pcts(x_ms):
+    x = np.asarray(x_ms, dtype=np.float64)
+    return round(float(np.percentile(x, 50)), 3), round(float(np.percentile(x, 99)), 3)
+
+
+def batched_section(args, graph_extra=True, graph_tune=True):
+    """Batched form: 512 independent ER products C_i = A_i * A_i (n
+    log-uniform in [1000, 65536], mean degree U[4,32]) as one block-diagonal
+    product -- symbolic and numeric phases of the whole batch every step --
+    against plain per-call launches of the same products on the same handle,
+    each call's result waited for as a client would.  All requests are
+    present at t=0: a request's latency is its batch's completion (batched)
+    or its own call's completion (per-call).  With graph_extra, the numeric
+    phase alone captured once in a HIP graph and replayed is reported as a
+    labelled extra (it skips the symbolic phase, so it is not the batched
+    figure)."""
     import torch
     import tilespgemm_amd as tsg
     from tilespgemm_amd import gen
@@ -153,93 +220,130 @@ def bench_batch(args):
         h.batch_numeric(arr, ic, jc, c)
         return arr, ic, jc, c, tot
 
+    steps = max(1, args.batch_steps)
     with torch.cuda.stream(s):
-        for _ in range(args.warmup):
+        for _ in range(max(1, args.warmup)):
             arr, ic, jc, c, tot = batched_step()
         s.synchronize()
-        t1 = time.perf_counter()
-        for _ in range(args.steps):
+        times = []
+        for _ in range(steps):
+            t1 = time.perf_counter()
             arr, ic, jc, c, tot = batched_step()
-        s.synchronize()
-        t_batch = (time.perf_counter() - t1) / args.steps
+            s.synchronize()
+            times.append(time.perf_counter() - t1)
+    t_batch = float(np.mean(times))
     log(f"[bench] batched: {t_batch * 1e3:.2f} ms per batch, nnz(C) total {tot}")
 
-    # numeric phase captured once, replayed; the graph replays the numeric
-    # phase alone, so its tuning is searched for that phase (the symbolic
-    # plan is then rebuilt with it before capture)
-    tuned_g = None
-    if not args.no_autotune:
-        with torch.cuda.stream(s):
-            tuned_g = AT.tune(h, None, log=log, measure=AT.batch_numeric_measure(h, pairs, reps=2))
-            log(f"[bench] numeric-phase autotune: row_bins {tuned_g['row_bins']}, "
-                f"{tuned_g['default_ms']:.2f} -> {tuned_g['ms']:.2f} ms")
-            del arr, ic, jc, c
-            arr, ic, per, roff, tot = h.batch_symbolic(pairs)
-            jc = torch.empty(max(tot, 1), dtype=torch.int32, device=dev)
-            c = torch.empty(max(tot, 1), dtype=torch.float64, device=dev)
+    extra = None
+    if graph_extra:
+        # numeric phase captured once, replayed; the graph replays the
+        # numeric phase alone, so its tuning is searched for that phase (the
+        # symbolic plan is then rebuilt with it before capture)
+        tuned_g = None
+        if graph_tune and not args.no_autotune:
+            with torch.cuda.stream(s):
+                tuned_g = AT.tune(h, None, log=log,
+                                  measure=AT.batch_numeric_measure(h, pairs, reps=2))
+                log(f"[bench] numeric-phase autotune: row_bins {tuned_g['row_bins']}, "
+                    f"{tuned_g['default_ms']:.2f} -> {tuned_g['ms']:.2f} ms")
+                del arr, ic, jc, c
+                arr, ic, per, roff, tot = h.batch_symbolic(pairs)
+                jc = torch.empty(max(tot, 1), dtype=torch.int32, device=dev)
+                c = torch.empty(max(tot, 1), dtype=torch.float64, device=dev)
+                h.batch_numeric(arr, ic, jc, c)
+            s.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
             h.batch_numeric(arr, ic, jc, c)
-        s.synchronize()
-    g = torch.cuda.CUDAGraph()
-    with torch.cuda.graph(g, stream=s):
-        h.batch_numeric(arr, ic, jc, c)
-    g.replay()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    for _ in range(args.steps):
         g.replay()
-    torch.cuda.synchronize()
-    t_graph = (time.perf_counter() - t1) / args.steps
-    log(f"[bench] graph-replayed numeric: {t_graph * 1e3:.2f} ms per batch")
-    if tuned is not None:
-        h.set_tuning({"tile_density": tuned["tile_density"], "row_bins": tuple(tuned["row_bins"]),
-                      "block_pointer_budget": tuned["block_pointer_budget"]})
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            g.replay()
+        torch.cuda.synchronize()
+        t_graph = (time.perf_counter() - t1) / steps
+        log(f"[bench] graph-replayed numeric phase: {t_graph * 1e3:.2f} ms per batch")
+        del g
+        extra = {"what": "numeric phase only (symbolic plan reused), captured once in a HIP "
+                         "graph and replayed; not comparable with per-call full products",
+                 "req_s": round(len(pairs) / t_graph, 2),
+                 "gflops": round(2.0 * products / t_graph / 1e9, 2),
+                 "autotune": None if tuned_g is None else {
+                     "row_bins": tuned_g["row_bins"], "tile_density": tuned_g["tile_density"],
+                     "block_pointer_budget": tuned_g["block_pointer_budget"],
+                     "default_ms": round(tuned_g["default_ms"], 3),
+                     "tuned_ms": round(tuned_g["ms"], 3)}}
+        if tuned is not None:
+            h.set_tuning({"tile_density": tuned["tile_density"],
+                          "row_bins": tuple(tuned["row_bins"]),
+                          "block_pointer_budget": tuned["block_pointer_budget"]})
 
-    # plain per-call launches of the same products
+    # plain per-call launches of the same products, each waited for
     with torch.cuda.stream(s):
         for d, _ in pairs[:4]:
             h.spgemm(d, d)
         s.synchronize()
+        done = []
         t1 = time.perf_counter()
+        service = []
         for d, _ in pairs:
+            ta = time.perf_counter()
             h.spgemm(d, d)
-        s.synchronize()
+            s.synchronize()
+            tb = time.perf_counter()
+            service.append((tb - ta) * 1e3)
+            done.append((tb - t1) * 1e3)
         t_single = time.perf_counter() - t1
     log(f"[bench] per-call: {t_single * 1e3:.2f} ms for {len(pairs)} products")
-    del g
+    h.close()
     nreq = len(pairs)
+    b50, b99 = _pcts(np.array(times) * 1e3)
+    p50, p99 = _pcts(done)
+    s50, s99 = _pcts(service)
+    return {
+        "workload": f"{nreq} independent ER C=A*A, n log-uniform [1000, 65536], mean degree "
+                    "U[4,32], U[-1,1] values, seed 7; all requests present at t=0",
+        "requests": nreq, "intermediate_products": products, "nnz_c": int(tot),
+        "req_s": round(nreq / t_batch, 2),
+        "gflops": round(2.0 * products / t_batch / 1e9, 2),
+        "ms_per_batch": round(t_batch * 1e3, 3),
+        "latency_p50_ms": b50, "latency_p99_ms": b99,
+        "per_call_req_s": round(nreq / t_single, 2),
+        "per_call_latency_p50_ms": p50, "per_call_latency_p99_ms": p99,
+        "per_call_service_p50_ms": s50, "per_call_service_p99_ms": s99,
+        "speedup_vs_per_call": round(t_single / t_batch, 2),
+        "autotune": None if tuned is None else {
+            "tile_density": tuned["tile_density"], "row_bins": tuned["row_bins"],
+            "block_pointer_budget": tuned["block_pointer_budget"],
+            "default_ms": round(tuned["default_ms"], 3),
+            "tuned_ms": round(tuned["ms"], 3), "trials": len(tuned["trials"])},
+        "graph_replayed_numeric_extra": extra,
+    }
+
+
+def bench_batch(args):
+    """--workload er_batch: the batched section alone as its own line."""
+    b = batched_section(args)
     out = {
-        "metric": "batched req/s", "value": round(nreq / t_batch, 2), "unit": "req/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(t_batch * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "metric": "batched req/s", "value": b["req_s"], "unit": "req/s",
+        "n_gpus": 1, "steps": args.batch_steps, "warmup": args.warmup,
+        "ms_per_step": b["ms_per_batch"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"{nreq} independent ER C=A*A, n log-uniform [1000, 65536], "
-                               "mean degree U[4,32], U[-1,1] values, seed 7",
-                   "intermediate_products": products, "nnz_c": int(tot),
-                   "gflops_batched": round(2.0 * products / t_batch / 1e9, 2),
-                   "per_call_req_s": round(nreq / t_single, 2),
-                   "speedup_vs_per_call": round(t_single / t_batch, 2),
-                   "graph_numeric_req_s": round(nreq / t_graph, 2),
-                   "graph_numeric_gflops": round(2.0 * products / t_graph / 1e9, 2),
-                   "graph_autotune": None if tuned_g is None else {
-                       "row_bins": tuned_g["row_bins"], "tile_density": tuned_g["tile_density"],
-                       "block_pointer_budget": tuned_g["block_pointer_budget"],
-                       "default_ms": round(tuned_g["default_ms"], 3),
-                       "tuned_ms": round(tuned_g["ms"], 3)},
-                   "autotune": None if tuned is None else {
-                       "tile_density": tuned["tile_density"], "row_bins": tuned["row_bins"],
-                       "block_pointer_budget": tuned["block_pointer_budget"],
-                       "default_ms": round(tuned["default_ms"], 3),
-                       "tuned_ms": round(tuned["ms"], 3), "trials": len(tuned["trials"])}},
+        "config": b,
     }
     print(json.dumps(out), flush=True)
 
 
-def bench_served(args):
-    """Served form: an open-loop Poisson request stream against the request
-    server (device-resident operands, admission by the intermediate-product
-    bound, continuous batching on several workers) and against the same
-    server with one worker and batches of one (plain per-call launches).
-    Latency of a request = completion - scheduled arrival."""
+def served_section(args, served_sweep=(0.5, 0.8)):
+    """Served form: an open-loop Poisson request stream (gen.served_mix:
+    70% ER, 25% Laplacian, 5% R-MAT 2^18 A*A^T) against the request server
+    (device-resident operands, admission by the intermediate-product bound,
+    continuous batching on several workers) and against the same server with
+    one worker and batches of one (plain per-call launches).  Latency of a
+    request = completion - scheduled arrival.  Saturation (every request at
+    t=0), then Poisson rates at 0.25/0.5/0.8 of the per-call saturation rate
+    for both, and at served_sweep fractions of the served saturation rate for
+    the server alone."""
     import threading
     import queue as pyqueue
     import torch
@@ -272,6 +376,7 @@ def bench_served(args):
         if key not in prod_cache:
             prod_cache[key] = h.intermediate_products(A, B)
         products += prod_cache[key]
+    h.close()
     log(f"[bench] served mix: {nreq} requests (ER {int((kinds == 0).sum())}, Laplacian "
         f"{int((kinds == 1).sum())}, R-MAT {int((kinds == 2).sum())}), "
         f"{products} intermediate products, setup {time.time() - t0:.1f}s")
@@ -344,30 +449,41 @@ def bench_served(args):
                       "served_mean_batch": round(rs["mean_batch"], 2)})
         log(f"[bench] rate {rate:.0f}/s: served p50 {rs['p50_ms']:.2f} p99 {rs['p99_ms']:.2f} ms"
             f" | per-call p50 {rp['p50_ms']:.2f} p99 {rp['p99_ms']:.2f} ms")
-    for frac in (0.5, 0.8):
+    for frac in served_sweep:
         rate = frac * sat_s["req_s"]
         rs = run(*served, rate)
         sweep.append({"rate": round(rate, 1), "served_p50_ms": round(rs["p50_ms"], 3),
                       "served_p99_ms": round(rs["p99_ms"], 3),
                       "served_mean_batch": round(rs["mean_batch"], 2)})
         log(f"[bench] rate {rate:.0f}/s: served p50 {rs['p50_ms']:.2f} p99 {rs['p99_ms']:.2f} ms")
+    return {
+        "workload": f"served mix of {nreq} requests: 70% ER C=A*A (n U[1000,16000], deg 8), "
+                    "25% 2D Laplacian C=A*A (side 64-512), 5% R-MAT 2^18 C=A*A^T "
+                    f"({args.edge_factor}*2^18 draws); Poisson arrivals",
+        "workers": args.workers, "max_batch": args.max_batch, "mem_budget_gib": args.mem_budget,
+        "intermediate_products": products,
+        "req_s": round(sat_s["req_s"], 1),
+        "gflops": round(2.0 * products * sat_s["req_s"] / nreq / 1e9, 2),
+        "per_call_req_s": round(sat_p["req_s"], 1),
+        "speedup_vs_per_call": round(sat_s["req_s"] / sat_p["req_s"], 2),
+        "saturation_p50_ms": round(sat_s["p50_ms"], 2),
+        "saturation_p99_ms": round(sat_s["p99_ms"], 2),
+        "saturation_per_call_p50_ms": round(sat_p["p50_ms"], 2),
+        "saturation_per_call_p99_ms": round(sat_p["p99_ms"], 2),
+        "mean_batch": round(sat_s["mean_batch"], 2),
+        "sweep": sweep,
+    }
+
+
+def bench_served(args):
+    """--workload served: the served section alone as its own line."""
+    v = served_section(args)
     out = {
-        "metric": "served req/s", "value": round(sat_s["req_s"], 1), "unit": "req/s",
-        "n_gpus": 1, "steps": nreq, "warmup": nreq,
-        "ms_per_step": round(1e3 / sat_s["req_s"], 4), "higher_is_better": True,
+        "metric": "served req/s", "value": v["req_s"], "unit": "req/s",
+        "n_gpus": 1, "steps": args.requests, "warmup": args.requests,
+        "ms_per_step": round(1e3 / v["req_s"], 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"served mix of {nreq} requests: 70% ER C=A*A (n U[1000,16000], "
-                               "deg 8), 25% 2D Laplacian C=A*A (side 64-512), 5% R-MAT 2^18 "
-                               f"C=A*A^T ({args.edge_factor}*2^18 draws); Poisson arrivals",
-                   "workers": args.workers, "max_batch": args.max_batch,
-                   "mem_budget_gib": args.mem_budget,
-                   "intermediate_products": products,
-                   "served_gflops": round(2.0 * products * sat_s["req_s"] / nreq / 1e9, 2),
-                   "per_call_req_s": round(sat_p["req_s"], 1),
-                   "speedup_vs_per_call": round(sat_s["req_s"] / sat_p["req_s"], 2),
-                   "saturation_served_p99_ms": round(sat_s["p99_ms"], 2),
-                   "saturation_per_call_p99_ms": round(sat_p["p99_ms"], 2),
-                   "sweep": sweep},
+        "config": v,
     }
     print(json.dumps(out), flush=True)
 
@@ -468,6 +584,9 @@ def main():
                     help="tsg_set_item_order threshold (big-row items run in first-column "
                          "order from this many items; default: the library's)")
     ap.add_argument("--batch", type=int, default=512, help="products in the er_batch workload")
+    ap.add_argument("--batch-steps", type=int, default=5, help="timed batches (batched section)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="default workload: skip the batched and served sections")
     ap.add_argument("--no-autotune", action="store_true",
                     help="er_batch: keep the default tuning parameters")
     ap.add_argument("--requests", type=int, default=2000, help="requests in the served workload")
@@ -548,6 +667,7 @@ def main():
     ms_step = elapsed * 1e3 / args.steps
     gflops = 2.0 * products / (ms_step * 1e-3) / 1e9
     stages = {kname: v / args.steps for kname, v in stage_sum.items()}
+    method_used = h.method_used
     log(f"[bench] {ms_step:.2f} ms/step, {gflops:.1f} GFLOP/s, stages {stages}")
 
     # roofline of the dominant kernel
@@ -561,25 +681,56 @@ def main():
     dom_bytes = {"row.big_numeric": bytes_a + bytes_b + 12 * nnz_c,
                  "tile.step3": bytes_a + bytes_b + 12 * nnz_c}.get(dom, whole)
     achieved = dom_bytes / (stages[dom] * 1e-3) / 1e9
-    traffic = None
+    # HBM bytes per launch of the dominant kernel come from a committed
+    # rocprofv3 counter profile of this workload (they cannot be counted
+    # inside the run); the line says so
+    traffic, traffic_src = None, None
     tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get(dom)
+            tj = json.load(f)
+        traffic = tj.get(dom)
+        if traffic is not None:
+            traffic_src = (f"profiles/traffic_{args.workload}.md: rocprofv3 --pmc FETCH_SIZE, "
+                           "then WRITE_SIZE, per dispatch of this kernel, scaled by the same "
+                           "counters on a plain 4 GiB copy; profiled, not measured in this run")
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "traffic_source": traffic_src,
             "kernel": dom, "kernel_ms": round(stages[dom], 3),
             "algorithmic_bytes": dom_bytes,
             "whole_step_GBs": round(whole / (ms_step * 1e-3) / 1e9, 1)}
+
+    def gpu_rows(rows):
+        """The GPU's C restricted to the given rows (host arrays)."""
+        ic_h = ic.cpu().numpy()
+        lens = ic_h[rows + 1] - ic_h[rows]
+        gic = np.zeros(rows.size + 1, dtype=np.int64)
+        np.cumsum(lens, out=gic[1:])
+        pos = np.concatenate([np.arange(ic_h[r], ic_h[r + 1]) for r in rows])
+        pos_d = torch.from_numpy(pos).to(jc.device)
+        return gic, jc[pos_d].cpu().numpy(), c[pos_d].cpu().numpy()
 
     cpu = None
     if not args.no_cpu_baseline:
         Bh = (dB.indptr.cpu().numpy(), dB.indices.cpu().numpy(), dB.data.cpu().numpy())
         try:
-            cpu = cpu_baseline(A, Bh, m, n, k)
+            cpu = cpu_baseline(A, Bh, m, n, k, gpu_rows=gpu_rows)
             log(f"[bench] cpu baseline {cpu}")
         except Exception as e:  # noqa: BLE001
             log(f"[bench] cpu baseline failed: {e}")
+
+    batched = served = None
+    if not args.no_extras and args.workload == "rmat20_aat":
+        # free the product's device memory before the other two parts of the metric
+        h.close()
+        del ic, jc, c, dA, dB
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+        batched = batched_section(args, graph_extra=True, graph_tune=False)
+        torch.cuda.empty_cache()
+        served = served_section(args, served_sweep=())
+        torch.cuda.empty_cache()
 
     out = {
         "metric": "FP64 GFLOP/s", "value": round(gflops, 2), "unit": "GFLOP/s", "n_gpus": 1,
@@ -588,11 +739,15 @@ def main():
         "data": "synthetic",
         "config": {"workload": W["name"], "nnz_a": int(A[1].size), "nnz_c": int(nnz_c),
                    "intermediate_products": int(products),
-                   "method": {1: "tile", 2: "row"}.get(h.method_used, "?"),
+                   "method": {1: "tile", 2: "row"}.get(method_used, "?"),
                    "stages_ms": {kk: round(v, 3) for kk, v in stages.items()}},
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if batched is not None:
+        out["batched"] = batched
+    if served is not None:
+        out["served"] = served
     print(json.dumps(out), flush=True)
 
 
