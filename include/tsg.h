@@ -177,6 +177,27 @@ int tsg_spgemm_numeric(tsg_handle_t handle, int m, int n, int k,
                        int64_t nnz_b,
                        const int64_t *d_ic, int *d_jc, double *d_c);
 
+/* Symbolic and numeric phase in one call that allocates nothing (no
+ * counterpart in the reference; for callers that cannot call hipMalloc:
+ * serving loops and harnesses that forbid allocation).  Every workspace
+ * array comes from a fixed device scratch of TSG_NOALLOC_SCRATCH_BYTES per
+ * device, set up by tsg_create; inputs and outputs are the caller's device
+ * buffers: d_ic (m+1 int64) always, d_jc and d_c with room for nzmax
+ * entries.  Writes d_ic and *nnz_c; when nnz(C) <= nzmax also d_jc and d_c,
+ * the same bits as tsg_spgemm_symbolic + tsg_spgemm_numeric on the handle's
+ * settings (method, FP mode, tuning), else returns TSG_ERR_OVERFLOW.  A
+ * product whose workspace exceeds the scratch returns TSG_ERR_INVALID
+ * (m and nnz(A) too large at a glance) or TSG_ERR_ALLOC (the scratch ran
+ * out), with the sizes in tsg_last_error.  The handle's own plan is left as
+ * it was.  Calls on one device share the scratch: they are serialised and
+ * each returns after its work on the handle's stream has finished (the
+ * call synchronises that stream), so calls never overlap on the device. */
+#define TSG_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int tsg_spgemm_noalloc(tsg_handle_t handle, int m, int n, int k,
+                       const int *d_ia, const int *d_ja, const double *d_a, int64_t nnz_a,
+                       const int *d_ib, const int *d_jb, const double *d_b, int64_t nnz_b,
+                       int64_t *d_ic, int *d_jc, double *d_c, int64_t nzmax, int64_t *nnz_c);
+
 /* ====================================================================
  * Batched products (no counterpart in the reference, whose driver runs one
  * product per process).  The batch is laid out as one block-diagonal

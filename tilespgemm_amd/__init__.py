@@ -64,6 +64,8 @@ SIGNATURES = {
     "tsg_spgemm_symbolic": (_i, [_vp, _i, _i, _i, _vp, _vp, _i64, _vp, _vp, _i64, _vp, _vp]),
     "tsg_spgemm_numeric": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _i64,
                                 _vp, _vp, _vp]),
+    "tsg_spgemm_noalloc": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _i64,
+                                _vp, _vp, _vp, _i64, _vp]),
     "tsg_batch_symbolic": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
     "tsg_batch_numeric": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
     "tsg_set_tile_density": (_i, [_vp, C.c_double]),
@@ -452,6 +454,21 @@ class Handle:
         c = torch.empty(max(nnz, 1), dtype=torch.float64, device=dev)
         self.numeric(A, B, ic, jc, c)
         return ic, jc[:nnz], c[:nnz]
+
+    def spgemm_noalloc(self, A: DeviceCsr, B: DeviceCsr, ic, jc, c) -> int:
+        """C = A*B into caller-allocated tensors (ic: m+1 int64; jc, c: room
+        for nzmax = jc.numel() entries) with no allocation inside the call
+        (tsg_spgemm_noalloc).  Returns nnz(C); raises TsgError when C does not
+        fit or the product's workspace exceeds the library's scratch."""
+        if A.cols != B.rows:
+            raise TsgError(-1, "A.cols != B.rows")
+        nzmax = min(int(jc.numel()), int(c.numel()))
+        nnz = C.c_int64(0)
+        _check(load().tsg_spgemm_noalloc(
+            self._h, A.rows, A.cols, B.cols, _ptr(A.indptr), _ptr(A.indices), _ptr(A.data),
+            A.nnz, _ptr(B.indptr), _ptr(B.indices), _ptr(B.data), B.nnz, _ptr(ic), _ptr(jc),
+            _ptr(c), nzmax, C.byref(nnz)), self._h)
+        return int(nnz.value)
 
     # ---------------------------------------------------------- batches
     def batch_symbolic(self, pairs):

@@ -11,7 +11,31 @@
 #include <thread>
 
 
+// Scratch of tsg_spgemm_noalloc: every workspace array of its symbolic and
+// numeric phases lives here instead of in hipMalloc'd memory.  Each device
+// the library runs on holds its own copy until the process ends.  Host code
+// never uses this name as an address: tsg_create looks up each device's copy
+// with hipGetSymbolAddress (scratch_setup) and only that pointer is passed on.
+static __device__ __align__(256) unsigned char scratch[TSG_NOALLOC_SCRATCH_BYTES];
+
 namespace {
+
+constexpr int MAX_DEVICES = 64;
+// per device: the address of its copy of `scratch`, and a lock that
+// serialises tsg_spgemm_noalloc calls on that device
+void *g_scratch_ptr[MAX_DEVICES] = {};
+std::mutex g_scratch_mu[MAX_DEVICES];
+
+// Looks up the current device's copy of `scratch` (once per device).
+void scratch_setup(int device) {
+    if (device < 0 || device >= MAX_DEVICES) return;
+    std::lock_guard<std::mutex> lk(g_scratch_mu[device]);
+    if (g_scratch_ptr[device]) return;
+    void *ptr = nullptr;
+    TSG_HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    TSG_CHECK(ptr != nullptr, TSG_ERR_HIP, "hipGetSymbolAddress returned no address for scratch");
+    g_scratch_ptr[device] = ptr;
+}
 
 std::mutex g_err_mu;
 std::string g_err;
@@ -65,7 +89,7 @@ void check_csr_args(int m, int n, const int *ia, const int *ja, int64_t nnz, con
 }
 
 int choose_method(tsg_handle_t h, const CsrIn &A, bool sorted,
-                  const unsigned long long *sample) {
+                  const unsigned long long *sample, Arena &ws) {
     if (h->method == TSG_METHOD_TILE)
         TSG_CHECK(sorted, TSG_ERR_INVALID,
                   "the tile path needs column-sorted, duplicate-free rows");
@@ -74,7 +98,7 @@ int choose_method(tsg_handle_t h, const CsrIn &A, bool sorted,
     if (A.nnz == 0 || A.rows == 0) return TSG_METHOD_ROW;
     // Tile path when A's 16x16 tiles hold enough entries to amortize the
     // per-tile metadata (~60 bytes per tile), row path otherwise.
-    const double per_tile = tile_density(A, h->tile_density, h->ws, h->stream, sample);
+    const double per_tile = tile_density(A, h->tile_density, ws, h->stream, sample);
     return per_tile >= h->tile_density ? TSG_METHOD_TILE : TSG_METHOD_ROW;
 }
 
@@ -85,6 +109,98 @@ int device_cu_count(int device) {
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) return 256;
     return prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
 }
+
+namespace {
+
+// Symbolic phase of C = A*B on the handle's stream and settings, into the
+// given plans: validation (with AUTO's density sample), the method choice,
+// then the tile or the row path.  The plan's arrays come from ws; aux holds
+// temporaries of B's row sort.  Arguments are checked by the caller.
+void symbolic_into(tsg_handle_t h, RowPlan &row, TilePlan &tile, int &method_used, Arena &ws,
+                   Arena &aux, StageTimer &tm, const RowTuning &tune, int m, int n, int k,
+                   const int *d_ia, const int *d_ja, int64_t nnz_a, const int *d_ib,
+                   const int *d_jb, int64_t nnz_b, int64_t *d_ic, int64_t *nnz_c) {
+    CsrIn A{m, n, d_ia, d_ja, nullptr, nnz_a};
+    CsrIn B{n, k, d_ib, d_jb, nullptr, nnz_b};
+    // with AUTO, A's tile-density sample shares the validation's round trip
+    unsigned long long sample[2] = {0, 0};
+    const bool want_sample = h->method == TSG_METHOD_AUTO;
+    const int vf = validate_csr_pair(A, B, ws, h->stream, want_sample ? sample : nullptr);
+    TSG_CHECK((vf & 0x303) == 0, TSG_ERR_INVALID,
+              std::string("invalid CSR input: ") + ((vf & 0x3) ? "A" : "B") +
+                  ((vf & 0x101) ? " has bad row pointers" : " has a column out of range"));
+    int method = choose_method(h, A, (vf & 0x404) == 0, want_sample ? sample : nullptr, ws);
+    method_used = method;
+    if (method == TSG_METHOD_TILE) {
+        const bool b_is_a = d_ia == d_ib && d_ja == d_jb && m == n && n == k;
+        try {
+            tile_symbolic(tile, A, B, b_is_a, d_ic, nnz_c, ws, h->stream, tm, h->cus);
+        } catch (const TsgError &e) {
+            // AUTO's choice falls back to the row path when the tile path's
+            // int32 tile indices would overflow; a forced tile path reports
+            // the overflow
+            if (e.code != TSG_ERR_OVERFLOW || h->method != TSG_METHOD_AUTO) throw;
+            TSG_HIP_TRY(hipStreamSynchronize(h->stream));
+            tile.valid = false;
+            method = TSG_METHOD_ROW;
+            method_used = method;
+        }
+    }
+    if (method == TSG_METHOD_ROW) {
+        // The row path reads every B row in column order (spans from a row's
+        // first and last column, block pointers by search), so B rows that
+        // are not strictly ascending are replaced by a stably sorted copy:
+        // equal columns keep their storage order, so every output still
+        // accumulates its products in the reference's order.
+        const bool b_validated = B.ptr != A.ptr || B.idx != A.idx || B.cols != A.cols;
+        bool b_strict = b_validated ? (vf & 0x400) == 0 : (vf & 0x4) == 0;
+        CsrIn Bs = B;
+        int *b_idx = nullptr, *b_perm = nullptr;
+        double *b_val = nullptr;
+        if (!b_strict && nnz_b > 0) {
+            b_idx = ws.alloc_n<int>((size_t)nnz_b);
+            b_perm = ws.alloc_n<int>((size_t)nnz_b);
+            b_val = ws.alloc_n<double>((size_t)nnz_b);
+            csr_sort_rows(n, k, d_ib, d_jb, nnz_b, b_idx, b_perm, ws, aux, h->stream);
+            Bs.idx = b_idx;
+            // duplicates remain after sorting: strict only if none
+            b_strict = (validate_csr_pair(Bs, Bs, aux, h->stream) & 0x4) == 0;
+        }
+        row_symbolic(row, A, Bs, d_ic, nnz_c, ws, h->stream, tm, h->cus, tune);
+        row.b_strict = b_strict;
+        row.b_canon = b_idx != nullptr;
+        row.b_idx = b_idx;
+        row.b_perm = b_perm;
+        row.b_val = b_val;
+    }
+}
+
+// Numeric phase from plans made by symbolic_into (same structure; values
+// may differ).  Arguments are checked by the caller.
+void numeric_from(tsg_handle_t h, int method_used, RowPlan &row, TilePlan &tile, Arena &ws,
+                  StageTimer &tm, int m, int n, int k, const int *d_ia, const int *d_ja,
+                  const double *d_a, int64_t nnz_a, const int *d_ib, const int *d_jb,
+                  const double *d_b, int64_t nnz_b, const int64_t *d_ic, int *d_jc,
+                  double *d_c) {
+    CsrIn A{m, n, d_ia, d_ja, d_a, nnz_a};
+    CsrIn B{n, k, d_ib, d_jb, d_b, nnz_b};
+    const int fused = h->fp_mode == TSG_FP_FUSED;
+    if (method_used == TSG_METHOD_TILE) {
+        TSG_CHECK(tile.valid, TSG_ERR_STATE, "numeric phase without a symbolic phase");
+        tile_numeric(tile, A, B, d_ic, d_jc, d_c, fused, ws, h->stream, tm, h->cus);
+    } else {
+        TSG_CHECK(row.valid, TSG_ERR_STATE, "numeric phase without a symbolic phase");
+        CsrIn Bs = B;
+        if (row.b_canon) {
+            gather_values(nnz_b, row.b_perm, d_b, row.b_val, h->stream);
+            Bs.idx = row.b_idx;
+            Bs.val = row.b_val;
+        }
+        row_numeric(row, A, Bs, d_ic, d_jc, d_c, fused, ws, h->stream, tm, h->cus);
+    }
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -107,6 +223,7 @@ int tsg_create(tsg_handle_t *handle, int device) {
         TSG_HIP_TRY(hipStreamCreate(&h->own_stream));
         h->stream = h->own_stream;
         h->tm.stream = h->stream;
+        scratch_setup(device);
         *handle = h.release();
     });
 }
@@ -242,66 +359,13 @@ int tsg_spgemm_symbolic(tsg_handle_t h, int m, int n, int k, const int *d_ia, co
         TSG_CHECK(d_ic && nnz_c, TSG_ERR_INVALID, "null output");
         DeviceGuard dg(h->device);
         h->ws.reset();
+        h->aux.reset();
         h->tm.clear();
         h->tm.stream = h->stream;
         h->row.valid = false;
         h->tile.valid = false;
-        CsrIn A{m, n, d_ia, d_ja, nullptr, nnz_a};
-        CsrIn B{n, k, d_ib, d_jb, nullptr, nnz_b};
-        // with AUTO, A's tile-density sample shares the validation's round trip
-        unsigned long long sample[2] = {0, 0};
-        const bool want_sample = h->method == TSG_METHOD_AUTO;
-        const int vf = validate_csr_pair(A, B, h->ws, h->stream, want_sample ? sample : nullptr);
-        TSG_CHECK((vf & 0x303) == 0, TSG_ERR_INVALID,
-                  std::string("invalid CSR input: ") + ((vf & 0x3) ? "A" : "B") +
-                      ((vf & 0x101) ? " has bad row pointers" : " has a column out of range"));
-        int method = choose_method(h, A, (vf & 0x404) == 0, want_sample ? sample : nullptr);
-        h->method_used = method;
-        if (method == TSG_METHOD_TILE) {
-            const bool b_is_a = d_ia == d_ib && d_ja == d_jb && m == n && n == k;
-            try {
-                tile_symbolic(h->tile, A, B, b_is_a, d_ic, nnz_c, h->ws, h->stream, h->tm,
-                              h->cus);
-            } catch (const TsgError &e) {
-                // AUTO's choice falls back to the row path when the tile
-                // path's int32 tile indices would overflow; a forced tile
-                // path reports the overflow
-                if (e.code != TSG_ERR_OVERFLOW || h->method != TSG_METHOD_AUTO) throw;
-                TSG_HIP_TRY(hipStreamSynchronize(h->stream));
-                h->tile.valid = false;
-                h->ws.reset();
-                method = TSG_METHOD_ROW;
-                h->method_used = method;
-            }
-        }
-        if (method == TSG_METHOD_ROW) {
-            // The row path reads every B row in column order (spans from a
-            // row's first and last column, block pointers by search), so B
-            // rows that are not strictly ascending are replaced by a stably
-            // sorted copy: equal columns keep their storage order, so every
-            // output still accumulates its products in the reference's order.
-            const bool b_validated = B.ptr != A.ptr || B.idx != A.idx || B.cols != A.cols;
-            bool b_strict = b_validated ? (vf & 0x400) == 0 : (vf & 0x4) == 0;
-            CsrIn Bs = B;
-            int *b_idx = nullptr, *b_perm = nullptr;
-            double *b_val = nullptr;
-            if (!b_strict && nnz_b > 0) {
-                b_idx = h->ws.alloc_n<int>((size_t)nnz_b);
-                b_perm = h->ws.alloc_n<int>((size_t)nnz_b);
-                b_val = h->ws.alloc_n<double>((size_t)nnz_b);
-                h->aux.reset();
-                csr_sort_rows(n, k, d_ib, d_jb, nnz_b, b_idx, b_perm, h->ws, h->aux, h->stream);
-                Bs.idx = b_idx;
-                // duplicates remain after sorting: strict only if none
-                b_strict = (validate_csr_pair(Bs, Bs, h->aux, h->stream) & 0x4) == 0;
-            }
-            row_symbolic(h->row, A, Bs, d_ic, nnz_c, h->ws, h->stream, h->tm, h->cus, h->row_tune);
-            h->row.b_strict = b_strict;
-            h->row.b_canon = b_idx != nullptr;
-            h->row.b_idx = b_idx;
-            h->row.b_perm = b_perm;
-            h->row.b_val = b_val;
-        }
+        symbolic_into(h, h->row, h->tile, h->method_used, h->ws, h->aux, h->tm, h->row_tune, m,
+                      n, k, d_ia, d_ja, nnz_a, d_ib, d_jb, nnz_b, d_ic, nnz_c);
     });
 }
 
@@ -316,22 +380,67 @@ int tsg_spgemm_numeric(tsg_handle_t h, int m, int n, int k, const int *d_ia, con
         TSG_CHECK(nnz_a == 0 || d_a, TSG_ERR_INVALID, "null A values");
         TSG_CHECK(nnz_b == 0 || d_b, TSG_ERR_INVALID, "null B values");
         DeviceGuard dg(h->device);
-        CsrIn A{m, n, d_ia, d_ja, d_a, nnz_a};
-        CsrIn B{n, k, d_ib, d_jb, d_b, nnz_b};
-        const int fused = h->fp_mode == TSG_FP_FUSED;
-        if (h->method_used == TSG_METHOD_TILE) {
-            TSG_CHECK(h->tile.valid, TSG_ERR_STATE, "numeric phase without a symbolic phase");
-            tile_numeric(h->tile, A, B, d_ic, d_jc, d_c, fused, h->ws, h->stream, h->tm, h->cus);
-        } else {
-            TSG_CHECK(h->row.valid, TSG_ERR_STATE, "numeric phase without a symbolic phase");
-            CsrIn Bs = B;
-            if (h->row.b_canon) {
-                gather_values(nnz_b, h->row.b_perm, d_b, h->row.b_val, h->stream);
-                Bs.idx = h->row.b_idx;
-                Bs.val = h->row.b_val;
-            }
-            row_numeric(h->row, A, Bs, d_ic, d_jc, d_c, fused, h->ws, h->stream, h->tm, h->cus);
+        numeric_from(h, h->method_used, h->row, h->tile, h->ws, h->tm, m, n, k, d_ia, d_ja, d_a,
+                     nnz_a, d_ib, d_jb, d_b, nnz_b, d_ic, d_jc, d_c);
+    });
+}
+
+int tsg_spgemm_noalloc(tsg_handle_t h, int m, int n, int k, const int *d_ia, const int *d_ja,
+                       const double *d_a, int64_t nnz_a, const int *d_ib, const int *d_jb,
+                       const double *d_b, int64_t nnz_b, int64_t *d_ic, int *d_jc, double *d_c,
+                       int64_t nzmax, int64_t *nnz_c) {
+    if (!h) return TSG_ERR_INVALID;
+    return guarded(h, [&] {
+        check_csr_args(m, n, d_ia, d_ja, nnz_a, "A");
+        check_csr_args(n, k, d_ib, d_jb, nnz_b, "B");
+        TSG_CHECK(nnz_a == 0 || d_a, TSG_ERR_INVALID, "null A values");
+        TSG_CHECK(nnz_b == 0 || d_b, TSG_ERR_INVALID, "null B values");
+        TSG_CHECK(d_ic && nnz_c && nzmax >= 0 && (nzmax == 0 || (d_jc && d_c)), TSG_ERR_INVALID,
+                  "null output or negative nzmax");
+        const size_t cap = TSG_NOALLOC_SCRATCH_BYTES;
+        // the row arrays and per-entry product counts every product needs
+        // (the rest is checked as it is carved out of the scratch)
+        const double least = 28.0 * ((double)m + 1) + 8.0 * ((double)nnz_a + 1);
+        TSG_CHECK(least <= (double)cap, TSG_ERR_INVALID,
+                  "tsg_spgemm_noalloc: m = " + std::to_string(m) + " and nnz(A) = " +
+                      std::to_string(nnz_a) + " need more than its " + std::to_string(cap) +
+                      "-byte scratch");
+        TSG_CHECK(h->device >= 0 && h->device < MAX_DEVICES && g_scratch_ptr[h->device],
+                  TSG_ERR_STATE, "tsg_spgemm_noalloc: no scratch set up on this device");
+        DeviceGuard dg(h->device);
+        std::lock_guard<std::mutex> lk(g_scratch_mu[h->device]);
+        // on every way out (errors included) the work queued on the stream
+        // finishes before the lock is released, so the next call finds the
+        // scratch idle
+        struct Drain {
+            hipStream_t s;
+            ~Drain() { (void)hipStreamSynchronize(s); }
+        } drain{h->stream};
+        Arena ws;
+        ws.attach(g_scratch_ptr[h->device], cap, "tsg_spgemm_noalloc scratch");
+        RowPlan row;
+        TilePlan tile;
+        int method_used = TSG_METHOD_AUTO;
+        StageTimer tm;   // disabled: records no events
+        tm.stream = h->stream;
+        RowTuning tune = h->row_tune;
+        // span bitmaps and block pointers must fit beside the rest
+        if (tune.bitmap_budget < 0 || (size_t)tune.bitmap_budget > cap / 4)
+            tune.bitmap_budget = (int64_t)(cap / 4);
+        if (tune.bptr_budget < 0 || (size_t)tune.bptr_budget > cap / 8)
+            tune.bptr_budget = (int64_t)(cap / 8);
+        int64_t nz = 0;
+        symbolic_into(h, row, tile, method_used, ws, ws, tm, tune, m, n, k, d_ia, d_ja, nnz_a,
+                      d_ib, d_jb, nnz_b, d_ic, &nz);
+        *nnz_c = nz;
+        if (nz > nzmax) {
+            throw TsgError{TSG_ERR_OVERFLOW, "tsg_spgemm_noalloc: nnz(C) = " +
+                                                 std::to_string(nz) + " exceeds nzmax = " +
+                                                 std::to_string(nzmax)};
         }
+        numeric_from(h, method_used, row, tile, ws, tm, m, n, k, d_ia, d_ja, d_a, nnz_a, d_ib,
+                     d_jb, d_b, nnz_b, d_ic, d_jc, d_c);
+        TSG_HIP_TRY(hipStreamSynchronize(h->stream));
     });
 }
 

@@ -37,8 +37,14 @@ struct TsgError {
 // -------------------------------------------------------------- workspace
 // Chunked bump arena over hipMalloc'd blocks.  reset() keeps the blocks, so
 // repeated calls of the same size reach a steady state with no allocation.
+// An arena attached to an external block (attach) bumps inside that block
+// only: it never calls hipMalloc or hipFree, and a request past the block's
+// end throws TSG_ERR_ALLOC instead of growing.
 class Arena {
 public:
+    Arena() = default;
+    Arena(const Arena &) = delete;
+    Arena &operator=(const Arena &) = delete;
     ~Arena();
     void reset();
     void *alloc(size_t bytes);
@@ -47,12 +53,18 @@ public:
     }
     size_t held() const { return held_; }
     void release();
+    // Serve every allocation from [base, base + bytes) (256-byte aligned);
+    // what is held is dropped first.
+    void attach(void *base, size_t bytes, const char *what);
+    size_t used() const;
 
 private:
     struct Block { char *base; size_t size; size_t used; };
     std::vector<Block> blocks_;
     size_t cur_ = 0;
     size_t held_ = 0;
+    bool external_ = false;
+    const char *what_ = "";
 };
 
 // -------------------------------------------------------------- scans

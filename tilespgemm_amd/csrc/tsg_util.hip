@@ -11,10 +11,25 @@
 Arena::~Arena() { release(); }
 
 void Arena::release() {
-    for (auto &b : blocks_) (void)hipFree(b.base);
+    if (!external_)
+        for (auto &b : blocks_) (void)hipFree(b.base);
     blocks_.clear();
     cur_ = 0;
     held_ = 0;
+    external_ = false;
+}
+
+void Arena::attach(void *base, size_t bytes, const char *what) {
+    release();
+    external_ = true;
+    what_ = what;
+    blocks_.push_back(Block{static_cast<char *>(base), bytes & ~size_t(255), 0});
+}
+
+size_t Arena::used() const {
+    size_t u = 0;
+    for (const auto &b : blocks_) u += b.used;
+    return u;
 }
 
 void Arena::reset() {
@@ -33,6 +48,12 @@ void *Arena::alloc(size_t bytes) {
         }
         cur_++;
     }
+    if (external_)
+        throw TsgError{TSG_ERR_ALLOC, std::string(what_) + " of " +
+                                          std::to_string(blocks_.empty() ? 0 : blocks_[0].size) +
+                                          " bytes exhausted: " + std::to_string(used()) +
+                                          " bytes in use, " + std::to_string(bytes) +
+                                          " more needed"};
     size_t sz = std::max(bytes, size_t(64) << 20);
     void *p = nullptr;
     hipError_t e = hipMalloc(&p, sz);
