@@ -47,17 +47,21 @@ def _allocating(handle, dA, dB):
     return ic.cpu().numpy(), jc.cpu().numpy(), c.cpu().numpy()
 
 
-def _pairs():
+def _pairs(method):
     """(name, m, n, k, A, B, same): a large then a smaller product of each
     kind -- R-MAT A*A^T with rows on the big-row path, a 27-point stencil
-    (tile path under AUTO's default density only when forced)."""
-    R = gen.rmat(13, 100_000, seed=3)
-    RT = gen.transpose(1 << 13, 1 << 13, *R)
+    (tile path under AUTO's default density only when forced).  The forced
+    tile path keeps every C tile's structure and matched tile pairs, so its
+    large R-MAT case is scale 12 (about 17 MiB of the scratch; scale 13 with
+    100K draws would need about 70 MiB and is refused)."""
+    sc = 12 if method == tsg.METHOD_TILE else 13
+    R = gen.rmat(sc, 40_000 if sc == 12 else 100_000, seed=3)
+    RT = gen.transpose(1 << sc, 1 << sc, *R)
     r = gen.rmat(10, 6_000, seed=4)
     rT = gen.transpose(1 << 10, 1 << 10, *r)
     S = gen.stencil27(14, seed=1)
     s = gen.stencil27(6, seed=2)
-    return [("rmat13_aat", 1 << 13, 1 << 13, 1 << 13, R, RT, False),
+    return [(f"rmat{sc}_aat", 1 << sc, 1 << sc, 1 << sc, R, RT, False),
             ("rmat10_aat", 1 << 10, 1 << 10, 1 << 10, r, rT, False),
             ("st27_14", 14 ** 3, 14 ** 3, 14 ** 3, S, S, True),
             ("st27_6", 6 ** 3, 6 ** 3, 6 ** 3, s, s, True)]
@@ -68,7 +72,7 @@ def _pairs():
 def test_noalloc_matches_allocating_entry(handle, mname, method):
     handle.set_method(method)
     try:
-        for name, m, n, k, A, B, same in _pairs():
+        for name, m, n, k, A, B, same in _pairs(method):
             dA = _dev(m, n, A)
             dB = dA if same else _dev(n, k, B)
             want = _allocating(handle, dA, dB)

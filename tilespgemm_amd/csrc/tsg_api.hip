@@ -137,9 +137,11 @@ void symbolic_into(tsg_handle_t h, RowPlan &row, TilePlan &tile, int &method_use
             tile_symbolic(tile, A, B, b_is_a, d_ic, nnz_c, ws, h->stream, tm, h->cus);
         } catch (const TsgError &e) {
             // AUTO's choice falls back to the row path when the tile path's
-            // int32 tile indices would overflow; a forced tile path reports
-            // the overflow
-            if (e.code != TSG_ERR_OVERFLOW || h->method != TSG_METHOD_AUTO) throw;
+            // int32 tile indices would overflow or its structures do not fit
+            // in memory; a forced tile path reports the error
+            if ((e.code != TSG_ERR_OVERFLOW && e.code != TSG_ERR_ALLOC) ||
+                h->method != TSG_METHOD_AUTO)
+                throw;
             TSG_HIP_TRY(hipStreamSynchronize(h->stream));
             tile.valid = false;
             method = TSG_METHOD_ROW;

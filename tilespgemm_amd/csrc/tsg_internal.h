@@ -193,6 +193,7 @@ struct TileMat {
     int *csc_ptr = nullptr;      // [blkn+1]
     int *csc_row = nullptr;      // [numblk]
     int *csc_tile = nullptr;     // [numblk]
+    uint16_t *cmask = nullptr;   // [16*numblk] column masks, bit 15-k for row k (B only)
     // entry -> payload position permutation (values refreshed per numeric)
     int *perm = nullptr;         // [nnz] CSR position -> tile payload position
 };
@@ -212,6 +213,15 @@ struct TilePlan {
     int *rowoffC = nullptr;      // [numblkC*16] start of (tile,row) inside CSR row
     int64_t *rownnz = nullptr;   // [m+1] CSR row counts
     int64_t nnzC = 0;
+    // matched (A tile, B tile) pairs of every C tile in ascending K
+    int64_t *tpair = nullptr;    // [numblkC+1] offsets into pairs
+    int2 *pairs = nullptr;       // [npairs] {A tile position, B tile position}
+    int64_t npairs = 0;
+    // numeric work units (tile * 4 + part) by C tile output count: bins 0..3
+    // (<= 16, <= 32, <= 64, 64-output parts of larger tiles) back to back,
+    // tiles without outputs left out
+    int64_t *tbin_units = nullptr;   // [units]
+    int tbin_start[5] = {0, 0, 0, 0, 0};
 };
 
 // Throws TSG_ERR_OVERFLOW unless 0 <= n_c_tiles <= INT32_MAX.

@@ -314,22 +314,82 @@ __global__ void __launch_bounds__(64 * S1_WAVES) k_step1(
 // Calls f(pa, pb) for the matched pairs of C tile (I, J) in ascending K:
 // pa = position of A_IK in A's tile-CSR, pb = position of B_KJ in B's
 // tile-CSR.  Executed by the 16 lanes of group g (lanes 16g .. 16g+15).
+// The tile columns K of one A tile row as an LDS bitmap over the window
+// [kmin, kmin + 32 nw) with the count of set bits before every word, built by
+// a whole wave: the position in A's tile-CSR of tile K is then pa0 + pre[w] +
+// popc(word below K) -- no search.  ok is false (and the bitmap unused) when
+// the window exceeds RT_WORDS words.
+constexpr int RT_WORDS = 512;            // 16384 tile columns
+
+struct RowTiles {
+    unsigned *words;
+    uint16_t *pre;
+    int kmin = 0, nw = 0;
+    bool ok = false;
+
+    __device__ __forceinline__ void build(const int *__restrict__ tcolA, int pa0, int pa1) {
+        const int l = __lane_id();
+        ok = false;
+        nw = 0;
+        if (pa1 <= pa0) return;
+        kmin = tcolA[pa0];
+        nw = ((tcolA[pa1 - 1] - kmin) >> 5) + 1;
+        if (nw > RT_WORDS) return;
+        ok = true;
+        for (int w = l; w < nw; w += 64) words[w] = 0u;
+        wave_lds_fence();
+        for (int p = pa0 + l; p < pa1; p += 64) {
+            const int d = tcolA[p] - kmin;
+            atomicOr(&words[d >> 5], 1u << (d & 31));
+        }
+        wave_lds_fence();
+        int running = 0;
+        for (int base = 0; base < nw; base += 64) {
+            const int w = base + l;
+            const int c = w < nw ? __popc(words[w]) : 0;
+            const int incl = wave_incl_scan(c);
+            if (w < nw) pre[w] = (uint16_t)(running + incl - c);
+            running += __builtin_amdgcn_readlane(incl, 63);
+        }
+        wave_lds_fence();
+    }
+    // position of tile column K in the row (pa0-relative), or -1
+    __device__ __forceinline__ int find(int K) const {
+        const int d = K - kmin;
+        if (d < 0 || d >= nw * 32) return -1;
+        const unsigned word = words[d >> 5];
+        const unsigned bit = 1u << (d & 31);
+        if (!(word & bit)) return -1;
+        return (int)pre[d >> 5] + __popc(word & (bit - 1u));
+    }
+};
+
+// With rt built for A's tile row (ok), a B column list of at most 8 times
+// the row's length is walked in order and each entry looked up in the
+// bitmap; otherwise the shorter list is walked with a binary search in the
+// other.  Either way matches come in ascending K.
 template <class F>
 __device__ __forceinline__ void for_pairs(bool valid, int pa0, int pa1, int cb0, int cb1,
                                           const int *__restrict__ tcolA,
                                           const int *__restrict__ csc_row,
-                                          const int *__restrict__ csc_tile, F &&f) {
+                                          const int *__restrict__ csc_tile, F &&f,
+                                          const RowTiles *rt = nullptr) {
     const int lane = __lane_id();
     const int g = lane >> 4, r = lane & 15;
     const int lenA = valid ? pa1 - pa0 : 0, lenB = valid ? cb1 - cb0 : 0;
-    const bool iterA = lenA <= lenB;
+    const bool bm = rt && rt->ok && lenB <= 8 * lenA;
+    const bool iterA = !bm && lenA <= lenB;
     const int len = iterA ? lenA : lenB;
     for (int i0 = 0; i0 < len; i0 += 16) {
         const int i = i0 + r;
         bool match = false;
         int pa = 0, pb = 0;
         if (i < len) {
-            if (iterA) {
+            if (bm) {
+                const int K = csc_row[cb0 + i];
+                const int pos = rt->find(K);
+                if (pos >= 0) { match = true; pa = pa0 + pos; pb = csc_tile[cb0 + i]; }
+            } else if (iterA) {
                 const int K = tcolA[pa0 + i];
                 const int pos = lower_bound_i32(csc_row, cb0, cb1, K);
                 if (pos < cb1 && csc_row[pos] == K) { match = true; pa = pa0 + i; pb = csc_tile[pos]; }
@@ -359,13 +419,18 @@ __global__ void __launch_bounds__(64 * S2_WAVES) k_step2(
     const int *__restrict__ csc_row, const int *__restrict__ csc_tile,
     const uint16_t *__restrict__ maskB, const int *__restrict__ tptrC,
     const int *__restrict__ tcolC, uint16_t *__restrict__ maskC, uint8_t *__restrict__ rowptrC,
-    int64_t *__restrict__ tnnz, int *__restrict__ rowoffC, int64_t *__restrict__ rownnz) {
+    int64_t *__restrict__ tnnz, int *__restrict__ rowoffC, int64_t *__restrict__ rownnz,
+    int64_t *__restrict__ npair) {
+    __shared__ unsigned s_words[S2_WAVES][RT_WORDS];
+    __shared__ uint16_t s_pre[S2_WAVES][RT_WORDS];
     const int I = blockIdx.x * S2_WAVES + (threadIdx.x >> 6);
     const int lane = __lane_id();
     const int g = lane >> 4, r = lane & 15;
     if (I >= blkm) return;
     const int pa0 = tptrA[I], pa1 = tptrA[I + 1];
     const int t0 = tptrC[I], t1 = tptrC[I + 1];
+    RowTiles rt{s_words[threadIdx.x >> 6], s_pre[threadIdx.x >> 6]};
+    rt.build(tcolA, pa0, pa1);
     int running = 0;
     for (int base = t0; base < t1; base += 4) {
         const int t = base + g;
@@ -373,7 +438,9 @@ __global__ void __launch_bounds__(64 * S2_WAVES) k_step2(
         const int J = valid ? tcolC[t] : 0;
         const int cb0 = valid ? csc_ptr[J] : 0, cb1 = valid ? csc_ptr[J + 1] : 0;
         unsigned mc = 0;
+        int np = 0;
         for_pairs(valid, pa0, pa1, cb0, cb1, tcolA, csc_row, csc_tile, [&](int pa, int pb) {
+            np++;
             unsigned ma = maskA[(size_t)pa * 16 + r];
             const uint16_t *mb = maskB + (size_t)pb * 16;
             while (ma) {
@@ -381,13 +448,14 @@ __global__ void __launch_bounds__(64 * S2_WAVES) k_step2(
                 ma &= ~(1u << bit);
                 mc |= mb[15 - bit];
             }
-        });
+        }, &rt);
         const int pc = __popc(mc);
         const int incl = group_incl_scan<16>(pc);
         if (valid) {
             maskC[(size_t)t * 16 + r] = (uint16_t)mc;
             rowptrC[(size_t)t * 16 + r] = (uint8_t)(incl - pc);
             if (r == 15) tnnz[t] = incl;
+            if (r == 0) npair[t] = np;
         }
         int pre = 0, tot = 0;
 #pragma unroll
@@ -403,72 +471,300 @@ __global__ void __launch_bounds__(64 * S2_WAVES) k_step2(
     if (g == 0 && row < m) rownnz[row] = running;
 }
 
-// ------------------------------------------------------------------ step3
-constexpr int S3_WAVES = 4;
-constexpr int S3_TILES = S3_WAVES * 4;
-constexpr int ACC_LD = 17;               // padded row stride (doubles)
+// ------------------------------------------------------------- pairs, bins
+// Matched (A_IK, B_KJ) pairs of every C tile, in ascending K, at the tile's
+// offset tpair[t] (the exclusive scan of the counts step2 wrote): one wave
+// per C tile row, four tiles at a time, the intersection of step2 again.
+constexpr int FP_WAVES = 4;
 
-template <bool FUSED>
-__global__ void __launch_bounds__(64 * S3_WAVES) k_step3(
-    int numblkC, const int *__restrict__ trowC, const int *__restrict__ tcolC,
-    const int *__restrict__ tptrA, const int *__restrict__ tcolA,
-    const int *__restrict__ offA, const uint8_t *__restrict__ rowptrA,
-    const uint8_t *__restrict__ idxA, const double *__restrict__ valA,
+__global__ void __launch_bounds__(64 * FP_WAVES) k_fill_pairs(
+    int blkm, const int *__restrict__ tptrA, const int *__restrict__ tcolA,
     const int *__restrict__ csc_ptr, const int *__restrict__ csc_row,
-    const int *__restrict__ csc_tile, const int *__restrict__ offB,
-    const uint8_t *__restrict__ rowptrB, const uint8_t *__restrict__ idxB,
-    const double *__restrict__ valB, const uint16_t *__restrict__ maskC,
-    const int *__restrict__ rowoffC, const int64_t *__restrict__ ic, int *__restrict__ jc,
-    double *__restrict__ c) {
-    __shared__ double s_acc[S3_TILES * 16 * ACC_LD];
+    const int *__restrict__ csc_tile, const int *__restrict__ tptrC,
+    const int *__restrict__ tcolC, const int64_t *__restrict__ tpair, int2 *__restrict__ pairs) {
+    __shared__ unsigned s_words[FP_WAVES][RT_WORDS];
+    __shared__ uint16_t s_pre[FP_WAVES][RT_WORDS];
+    const int I = blockIdx.x * FP_WAVES + (threadIdx.x >> 6);
     const int lane = __lane_id();
     const int g = lane >> 4, r = lane & 15;
-    const int slot = (threadIdx.x >> 6) * 4 + g;
-    double *acc = s_acc + (slot * 16 + r) * ACC_LD;
-    const int t = blockIdx.x * S3_TILES + slot;
-    const bool valid = t < numblkC;
-#pragma unroll
-    for (int i = 0; i < 16; i++) acc[i] = 0.0;
-    int I = 0, J = 0, cb0 = 0, cb1 = 0, pa0 = 0, pa1 = 0;
-    if (valid) {
-        I = trowC[t];
-        J = tcolC[t];
-        cb0 = csc_ptr[J];
-        cb1 = csc_ptr[J + 1];
-        pa0 = tptrA[I];
-        pa1 = tptrA[I + 1];
+    if (I >= blkm) return;
+    const int pa0 = tptrA[I], pa1 = tptrA[I + 1];
+    const int t0 = tptrC[I], t1 = tptrC[I + 1];
+    RowTiles rt{s_words[threadIdx.x >> 6], s_pre[threadIdx.x >> 6]};
+    rt.build(tcolA, pa0, pa1);
+    for (int base = t0; base < t1; base += 4) {
+        const int t = base + g;
+        const bool valid = t < t1;
+        const int J = valid ? tcolC[t] : 0;
+        const int cb0 = valid ? csc_ptr[J] : 0, cb1 = valid ? csc_ptr[J + 1] : 0;
+        int64_t pos = valid ? tpair[t] : 0;
+        for_pairs(valid, pa0, pa1, cb0, cb1, tcolA, csc_row, csc_tile, [&](int pa, int pb) {
+            if (r == 0) pairs[pos] = make_int2(pa, pb);
+            pos++;
+        }, &rt);
     }
-    for_pairs(valid, pa0, pa1, cb0, cb1, tcolA, csc_row, csc_tile, [&](int pa, int pb) {
-        const int abase = offA[pa];
-        const int alo = abase + rowptrA[(size_t)pa * 16 + r];
-        const int ahi = r == 15 ? offA[pa + 1] : abase + rowptrA[(size_t)pa * 16 + r + 1];
-        const int bbase = offB[pb];
-        const int bend = offB[pb + 1];
-        const uint8_t *rpb = rowptrB + (size_t)pb * 16;
-        for (int q = alo; q < ahi; q++) {
-            const int cc = idxA[q] & 15;
-            const double av = valA[q];
-            const int blo = bbase + rpb[cc];
-            const int bhi = cc == 15 ? bend : bbase + rpb[cc + 1];
-            for (int u = blo; u < bhi; u++) {
-                const int col = idxB[u] & 15;
-                acc[col] = acc_step<FUSED>(acc[col], av, valB[u]);
-            }
-        }
-    });
-    if (!valid) return;
-    const unsigned mk = maskC[(size_t)t * 16 + r];
-    if (!mk) return;
-    const int64_t o = ic[I * 16 + r] + rowoffC[(size_t)t * 16 + r];
-    int i = 0;
+}
+
+// Work units of the numeric phase by C tile nnz: bin 0 holds tiles of <= 16
+// outputs (four per wave), bin 1 <= 32 (two per wave), bin 2 <= 64 (one per
+// wave), bin 3 the 64-output parts of larger tiles (a wave per part); every
+// lane owns one output.  Tiles without outputs are left out.  A unit is
+// t * 4 + part (int64).  cnt[0..3] counts units, then the scatter places
+// them at their bin's start (bins back to back) plus a slot from fill.
+// (Measured on the 27-point stencil 64^3, whose larger C tiles hold ~74
+// outputs: a wave per 64-output part beats one wave with two outputs per
+// lane, 1.43 vs 1.78 ms, and staging pair values in LDS as well as the
+// metadata, 1.77 ms: the kernel is bound by its VALU instructions per
+// product, not by load latency.)
+constexpr int TBINS = 4;
+
+__device__ __forceinline__ int tile_bin(int64_t nnz) {
+    return nnz == 0 ? -1 : nnz <= 16 ? 0 : nnz <= 32 ? 1 : nnz <= 64 ? 2 : 3;
+}
+
+__device__ __forceinline__ int tile_units(int64_t nnz) { return (int)((nnz + 63) >> 6); }
+
+__global__ void __launch_bounds__(256) k_tbin_count(int n, const int64_t *__restrict__ off,
+                                                    int *__restrict__ cnt) {
+    __shared__ int s_cnt[TBINS];
+    if (threadIdx.x < TBINS) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nz = t < n ? off[t + 1] - off[t] : 0;
+    const int b = tile_bin(nz);
+    if (b >= 0) atomicAdd(&s_cnt[b], b == 3 ? tile_units(nz) : 1);
+    __syncthreads();
+    if (threadIdx.x < TBINS && s_cnt[threadIdx.x]) atomicAdd(&cnt[threadIdx.x], s_cnt[threadIdx.x]);
+}
+
+// Slots come from a wave scan of the unit counts per bin, one LDS atomic
+// per wave and bin, and one global atomic per block and bin (a global
+// atomic per tile on four counters serialised the scatter).
+__global__ void __launch_bounds__(256) k_tbin_scatter(int n, const int64_t *__restrict__ off,
+                                                      const int *__restrict__ cnt,
+                                                      int *__restrict__ fill,
+                                                      int64_t *__restrict__ units) {
+    __shared__ int s_cnt[TBINS], s_base[TBINS];
+    if (threadIdx.x < TBINS) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int64_t nz = t < n ? off[t + 1] - off[t] : 0;
+    const int b = tile_bin(nz);
+    const int u = b == 3 ? tile_units(nz) : (b >= 0 ? 1 : 0);   // units of this tile
+    int pos = 0;
+    for (int q = 0; q < TBINS; q++) {
+        const uint64_t mask = __ballot(b == q);
+        if (!mask) continue;
+        const int v = b == q ? u : 0;
+        const int incl = wave_incl_scan(v);
+        const int wtot = __builtin_amdgcn_readlane(incl, 63);
+        int wbase = 0;
+        if (lane == 0) wbase = atomicAdd(&s_cnt[q], wtot);
+        wbase = __builtin_amdgcn_readlane(wbase, 0);
+        if (b == q) pos = wbase + incl - v;
+    }
+    __syncthreads();
+    if (threadIdx.x < TBINS && s_cnt[threadIdx.x]) {
+        int start = 0;
+        for (int i = 0; i < (int)threadIdx.x; i++) start += cnt[i];
+        s_base[threadIdx.x] = start + atomicAdd(&fill[threadIdx.x], s_cnt[threadIdx.x]);
+    }
+    __syncthreads();
+    for (int i = 0; i < u; i++) units[s_base[b] + pos + i] = (int64_t)t * 4 + i;
+}
+
+// Column masks of 16x16 tiles: cmask[16 t + c] has bit 15 - k set when the
+// tile holds entry (k, c) -- the transpose of the row masks, in the same
+// bit order.  16 lanes per tile, lane c builds column c.
+__global__ void __launch_bounds__(256) k_col_masks(int numblk, const uint16_t *__restrict__ mask,
+                                                   uint16_t *__restrict__ cmask) {
+    const int64_t t = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+    const int c = threadIdx.x & 15;
+    if (t >= numblk) return;
+    const uint4 *m4 = reinterpret_cast<const uint4 *>(mask + t * 16);
+    const uint4 lo = m4[0], hi = m4[1];
+    const unsigned w[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+    unsigned cm = 0;
 #pragma unroll
-    for (int cc = 0; cc < 16; cc++) {
-        if (mk & (1u << (15 - cc))) {
-            jc[o + i] = J * 16 + cc;
-            c[o + i] = acc[cc];
-            i++;
+    for (int k = 0; k < 16; k++) {
+        const unsigned row = (w[k >> 1] >> ((k & 1) * 16)) & 0xffffu;
+        cm |= ((row >> (15 - c)) & 1u) << (15 - k);
+    }
+    cmask[t * 16 + c] = (uint16_t)cm;
+}
+
+// ------------------------------------------------------------------ step3
+// FP64 numeric phase.  Every output's accumulator stays in a register of
+// the lane that owns the output and takes the output's products in A-row
+// order: matched pairs in ascending K (stored by k_fill_pairs), then the
+// columns k of A tile row r in ascending order.  The products of output
+// (r, c) of a pair are the k set in maskA[r] & cmaskB[c] (row r of A_IK
+// and column c of B_KJ both hold k); their values sit at A row r's start
+// plus the rank of k in the row, and B row k's start plus the rank of c.
+// A chunk of the tile's pairs has its fixed-size metadata (payload offsets,
+// row starts, row masks of A, row and column masks of B: 136 bytes a pair)
+// staged in LDS by one lane per pair with coalesced loads, so a product
+// costs two value loads; no atomics.
+struct TileArgs {
+    const int *trowC, *tcolC;
+    const uint16_t *maskC;
+    const uint8_t *rowptrC;
+    const int *rowoffC;
+    const int64_t *tpair;
+    const int2 *pairs;
+    const int *offA;
+    const uint8_t *rowptrA;
+    const uint16_t *maskA;
+    const double *valA;
+    const int *offB;
+    const uint8_t *rowptrB;
+    const uint16_t *maskB, *cmaskB;
+    const double *valB;
+    const int64_t *ic;
+    int *jc;
+    double *c;
+};
+
+__device__ __forceinline__ unsigned byte_at(const uint4 &v, int i) {
+    const unsigned w = i < 8 ? (i < 4 ? v.x : v.y) : (i < 12 ? v.z : v.w);
+    return (w >> ((i & 3) * 8)) & 255u;
+}
+
+struct PairMeta {
+    uint8_t rowptrA[16], rowptrB[16];   // row starts (bytes 0..31)
+    uint16_t maskA[16], maskB[16];      // row masks (bytes 32..95)
+    uint16_t cmaskB[16];                // column masks of B (bytes 96..127)
+    int offA, offB;                     // payload offsets (bytes 128..135)
+};
+
+// Stage pair p's metadata (called by one lane per pair): nine loads, one
+// level deep after the pair itself.
+__device__ __forceinline__ void stage_pair(const TileArgs &T, int64_t p, PairMeta &M) {
+    const int2 pr = T.pairs[p];
+    const int oa = T.offA[pr.x], ob = T.offB[pr.y];
+    const uint4 ra = *reinterpret_cast<const uint4 *>(T.rowptrA + (size_t)pr.x * 16);
+    const uint4 rb = *reinterpret_cast<const uint4 *>(T.rowptrB + (size_t)pr.y * 16);
+    const uint4 *ma = reinterpret_cast<const uint4 *>(T.maskA + (size_t)pr.x * 16);
+    const uint4 *mb = reinterpret_cast<const uint4 *>(T.maskB + (size_t)pr.y * 16);
+    const uint4 *cb = reinterpret_cast<const uint4 *>(T.cmaskB + (size_t)pr.y * 16);
+    const uint4 ma0 = ma[0], ma1 = ma[1], mb0 = mb[0], mb1 = mb[1], cb0 = cb[0], cb1 = cb[1];
+    uint4 *d = reinterpret_cast<uint4 *>(&M);
+    d[0] = ra;
+    d[1] = rb;
+    d[2] = ma0;
+    d[3] = ma1;
+    d[4] = mb0;
+    d[5] = mb1;
+    d[6] = cb0;
+    d[7] = cb1;
+    M.offA = oa;
+    M.offB = ob;
+}
+
+// A lane's products over the staged pairs meta[0..n), in order: pair by
+// pair, and inside a pair the k set in maskA[r] & cmaskB[c] from the
+// smallest.  next() yields the A and B value positions of the next product
+// (false when none is left), so a lane's loop runs as many times as it has
+// products, not as many as the busiest lane has in every pair.
+struct ProductWalk {
+    const PairMeta *meta;
+    int n, r, c;
+    int j = -1;
+    unsigned hits = 0, ma = 0;
+    int abase = 0;
+    __device__ __forceinline__ bool next(int &ai, int &bi) {
+        while (hits == 0) {
+            if (++j >= n) return false;
+            const PairMeta &M = meta[j];
+            ma = M.maskA[r];
+            hits = ma & M.cmaskB[c];
+            abase = M.offA + M.rowptrA[r];
+        }
+        const PairMeta &M = meta[j];
+        const int bit = 31 - __clz(hits);          // highest bit: smallest k
+        hits &= ~(1u << bit);
+        const int k = 15 - bit;
+        ai = abase + __popc(ma >> (bit + 1));
+        bi = M.offB + M.rowptrB[k] + __popc((unsigned)M.maskB[k] >> (16 - c));
+        return true;
+    }
+};
+
+// Accumulates a lane's products over the staged pairs into acc, two at a
+// time (four value loads in flight), in order.
+template <bool FUSED>
+__device__ __forceinline__ double walk_products(const TileArgs &T, const PairMeta *meta, int n,
+                                                int r, int c, double acc) {
+    ProductWalk w{meta, n, r, c};
+    while (true) {
+        int a0 = 0, b0 = 0, a1 = 0, b1 = 0;
+        if (!w.next(a0, b0)) break;
+        const bool two = w.next(a1, b1);
+        const double x0 = T.valA[a0], y0 = T.valB[b0];
+        const double x1 = two ? T.valA[a1] : 0.0, y1 = two ? T.valB[b1] : 0.0;
+        acc = acc_step<FUSED>(acc, x0, y0);
+        if (!two) break;
+        acc = acc_step<FUSED>(acc, x1, y1);
+    }
+    return acc;
+}
+
+// Work units of <= 64 / TPW outputs, TPW per wave (G = 64 / TPW lanes each);
+// a unit is a tile and its 64-output part, and lane lq owns the part's lq-th
+// output in row-major order.  Pairs are staged CP = 32 / TPW at a time per
+// unit (4352 bytes of LDS a wave).
+template <bool FUSED, int TPW>
+__global__ void __launch_bounds__(256) k_step3_out(int nunits, const int64_t *__restrict__ units,
+                                                   TileArgs T) {
+    constexpr int G = 64 / TPW;
+    constexpr int CP = 32 / TPW;
+    __shared__ PairMeta s_meta[4][TPW][CP];
+    const int lane = __lane_id();
+    const int lq = lane % G, g = lane / G;
+    PairMeta *meta = s_meta[threadIdx.x >> 6][g];
+    const int64_t slot = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * TPW + g;
+    if (slot >= nunits) return;
+    const int64_t unit = units[slot];
+    const int t = (int)(unit >> 2);
+    const int q = (int)(unit & 3) * 64 + lq;     // the tile's output this lane owns
+    // row of output q: the last row whose first output is <= q (rows hold
+    // their tile-local start in rowptrC), then its (q - start)-th column;
+    // lanes past the tile's outputs stage pairs but own no output
+    const uint4 rp = *reinterpret_cast<const uint4 *>(T.rowptrC + (size_t)t * 16);
+    int r = 0, start = 0;
+#pragma unroll
+    for (int i = 1; i < 16; i++) {
+        const int st = (int)byte_at(rp, i);
+        if (st <= q) {
+            r = i;
+            start = st;
         }
     }
+    // rows after r with the same start are empty, so r holds q exactly when
+    // q is below r's end
+    const unsigned mrow = T.maskC[(size_t)t * 16 + r];
+    const bool owner = q - start < __popc(mrow);
+    int c = 0;
+    if (owner) {
+        unsigned mm = mrow;
+        for (int j = 0; j < q - start; j++) mm &= ~(1u << (31 - __clz(mm)));
+        c = 15 - (31 - __clz(mm));
+    }
+    double acc = 0.0;
+    const int64_t p0 = T.tpair[t], p1 = T.tpair[t + 1];
+    for (int64_t pc = p0; pc < p1; pc += CP) {
+        const int n = (int)min((int64_t)CP, p1 - pc);
+        wave_lds_fence();
+        if (lq < n) stage_pair(T, pc + lq, meta[lq]);
+        wave_lds_fence();
+        if (owner) acc = walk_products<FUSED>(T, meta, n, r, c, acc);
+    }
+    if (!owner) return;
+    const int I = T.trowC[t], J = T.tcolC[t];
+    const int64_t o = T.ic[I * 16 + r] + T.rowoffC[(size_t)t * 16 + r] + (q - start);
+    T.jc[o] = J * 16 + c;
+    T.c[o] = acc;
 }
 
 void convert(TileMat &T, const CsrIn &X, bool fill_values, Arena &ws, hipStream_t s) {
@@ -543,10 +839,17 @@ void tile_symbolic(TilePlan &p, const CsrIn &A, const CsrIn &B, bool b_is_a, int
     TB.csc_tile = ws.alloc_n<int>((size_t)TB.numblk + 1);
     csr_transpose(TB.blkm, TB.blkn, TB.tile_ptr, TB.tile_col, nullptr, TB.numblk, TB.csc_ptr,
                   TB.csc_row, nullptr, ws, s, TB.csc_tile);
+    TB.cmask = ws.alloc_n<uint16_t>((size_t)TB.numblk * 16 + 16);
+    if (TB.numblk > 0) {
+        hipLaunchKernelGGL(k_col_masks, dim3(grid_for((size_t)TB.numblk * 16, 256)), dim3(256), 0,
+                           s, TB.numblk, TB.mask, TB.cmask);
+        TSG_LAUNCH_CHECK();
+    }
     if (b_is_a) {
         p.A.csc_ptr = TB.csc_ptr;
         p.A.csc_row = TB.csc_row;
         p.A.csc_tile = TB.csc_tile;
+        p.A.cmask = TB.cmask;
     }
     tm.end();
 
@@ -587,25 +890,66 @@ void tile_symbolic(TilePlan &p, const CsrIn &A, const CsrIn &B, bool b_is_a, int
     p.rowptrC = ws.alloc_n<uint8_t>((size_t)nC * 16 + 16);
     p.offC = ws.alloc_n<int64_t>((size_t)nC + 1);
     p.rowoffC = ws.alloc_n<int>((size_t)nC * 16 + 16);
+    p.tpair = ws.alloc_n<int64_t>((size_t)nC + 1);
     p.rownnz = ws.alloc_n<int64_t>((size_t)m + 1);
     TSG_HIP_TRY(hipMemsetAsync(p.rownnz, 0, sizeof(int64_t) * ((size_t)m + 1), s));
     if (blkm > 0) {
         hipLaunchKernelGGL(k_step2, dim3(grid_for(blkm, S2_WAVES)), dim3(64 * S2_WAVES), 0, s, m,
                            blkm, p.A.tile_ptr, p.A.tile_col, p.A.mask, TB.csc_ptr, TB.csc_row,
                            TB.csc_tile, TB.mask, p.tptrC, p.tcolC, p.maskC, p.rowptrC, p.offC,
-                           p.rowoffC, p.rownnz);
+                           p.rowoffC, p.rownnz, p.tpair);
         TSG_LAUNCH_CHECK();
     }
     int64_t *tmpC = ws.alloc_n<int64_t>(scan_tmp_elems((size_t)nC + 1));
     exclusive_scan_i64(p.offC, p.offC, (size_t)nC, tmpC, s);
+    exclusive_scan_i64(p.tpair, p.tpair, (size_t)nC, tmpC, s);
     int64_t *tmpm = ws.alloc_n<int64_t>(scan_tmp_elems((size_t)m + 1));
     exclusive_scan_i64(p.rownnz, d_ic, (size_t)m, tmpm, s);
+    // C tiles by output count for the numeric kernels
+    int *bcnt = ws.alloc_n<int>(2 * TBINS);
+    TSG_HIP_TRY(hipMemsetAsync(bcnt, 0, 2 * TBINS * sizeof(int), s));
+    if (nC > 0) {
+        hipLaunchKernelGGL(k_tbin_count, dim3(grid_for((size_t)nC, 256)), dim3(256), 0, s, nC,
+                           p.offC, bcnt);
+        TSG_LAUNCH_CHECK();
+    }
     tm.end();
-    int64_t tot = 0;
-    TSG_HIP_TRY(hipMemcpyAsync(&tot, d_ic + m, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    // nnz(C), the matched-pair total and the bin counts in one round trip
+    int64_t h64[2] = {0, 0};
+    int hb[TBINS] = {0, 0, 0, 0};
+    TSG_HIP_TRY(hipMemcpyAsync(&h64[0], d_ic + m, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    TSG_HIP_TRY(hipMemcpyAsync(&h64[1], p.tpair + nC, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    TSG_HIP_TRY(hipMemcpyAsync(hb, bcnt, sizeof(hb), hipMemcpyDeviceToHost, s));
     TSG_HIP_TRY(hipStreamSynchronize(s));
-    p.nnzC = tot;
-    *nnz_c = tot;
+    // the numeric phase's work units, sized by the counts just read
+    int nunits = 0;
+    for (int b = 0; b < TBINS; b++) nunits += hb[b];
+    p.tbin_units = ws.alloc_n<int64_t>((size_t)nunits + 1);
+    tm.begin("tile.pairs");
+    if (nunits > 0) {
+        hipLaunchKernelGGL(k_tbin_scatter, dim3(grid_for((size_t)nC, 256)), dim3(256), 0, s, nC,
+                           p.offC, bcnt, bcnt + TBINS, p.tbin_units);
+        TSG_LAUNCH_CHECK();
+    }
+    // the matched pairs themselves (the numeric phase reads them instead of
+    // intersecting tile lists again)
+    p.npairs = h64[1];
+    p.pairs = ws.alloc_n<int2>((size_t)p.npairs + 1);
+    if (blkm > 0 && p.npairs > 0) {
+        hipLaunchKernelGGL(k_fill_pairs, dim3(grid_for(blkm, FP_WAVES)), dim3(64 * FP_WAVES), 0,
+                           s, blkm, p.A.tile_ptr, p.A.tile_col, TB.csc_ptr, TB.csc_row,
+                           TB.csc_tile, p.tptrC, p.tcolC, p.tpair, p.pairs);
+        TSG_LAUNCH_CHECK();
+    }
+    tm.end();
+    int start = 0;
+    for (int b = 0; b < TBINS; b++) {
+        p.tbin_start[b] = start;
+        start += hb[b];
+    }
+    p.tbin_start[TBINS] = start;
+    p.nnzC = h64[0];
+    *nnz_c = h64[0];
     p.valid = true;
 }
 
@@ -613,12 +957,24 @@ template <bool FUSED>
 static void tile_numeric_t(TilePlan &p, const double *valB, const int64_t *d_ic, int *d_jc,
                            double *d_c, hipStream_t s) {
     const TileMat &TA = p.A, &TB = p.B;
-    if (p.numblkC > 0) {
-        hipLaunchKernelGGL(k_step3<FUSED>, dim3(grid_for(p.numblkC, S3_TILES)),
-                           dim3(64 * S3_WAVES), 0, s, p.numblkC, p.trowC, p.tcolC, TA.tile_ptr,
-                           TA.tile_col, TA.tile_off, TA.rowptr, TA.idx, TA.val, TB.csc_ptr,
-                           TB.csc_row, TB.csc_tile, TB.tile_off, TB.rowptr, TB.idx, valB,
-                           p.maskC, p.rowoffC, d_ic, d_jc, d_c);
+    const TileArgs T{p.trowC,  p.tcolC,    p.maskC,     p.rowptrC,   p.rowoffC, p.tpair,
+                     p.pairs,  TA.tile_off, TA.rowptr,  TA.mask,     TA.val,    TB.tile_off,
+                     TB.rowptr, TB.mask,    TB.cmask,   valB,        d_ic,      d_jc,
+                     d_c};
+    // four waves per block; units per wave by bin (bins 2 and 3 share the
+    // one-unit-per-wave kernel and are launched together)
+    const int tpw[3] = {4, 2, 1};
+    for (int b = 0; b < 3; b++) {
+        const int n = (b < 2 ? p.tbin_start[b + 1] : p.tbin_start[TBINS]) - p.tbin_start[b];
+        if (n == 0) continue;
+        const int64_t *units = p.tbin_units + p.tbin_start[b];
+        const dim3 grid(grid_for((size_t)n, 4 * tpw[b]));
+        if (b == 0)
+            hipLaunchKernelGGL((k_step3_out<FUSED, 4>), grid, dim3(256), 0, s, n, units, T);
+        else if (b == 1)
+            hipLaunchKernelGGL((k_step3_out<FUSED, 2>), grid, dim3(256), 0, s, n, units, T);
+        else
+            hipLaunchKernelGGL((k_step3_out<FUSED, 1>), grid, dim3(256), 0, s, n, units, T);
         TSG_LAUNCH_CHECK();
     }
 }
