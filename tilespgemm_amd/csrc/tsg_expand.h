@@ -36,11 +36,15 @@ __device__ __forceinline__ int brange_lo(const BlockPtrs &bp, const int *__restr
 // f in product order (see for_products); bofs/bstep select every bstep-th
 // batch starting at bofs.  s_mark: the group's G ints of LDS, all -1 on entry
 // and on return.
-template <int G, int U, bool VALUES, class F>
+// PACKED: B's entries are also given as 16-byte {column, -, value} records
+// (bpk), so a product's column and value come from one load and, for a short
+// range, one cache line.
+template <int G, int U, bool VALUES, bool PACKED = false, class F>
 __device__ __forceinline__ void expand_group(int lo, int cnt, double a,
                                              const int *__restrict__ jb,
                                              const double *__restrict__ bval, int *s_mark,
-                                             F &&f, int bofs = 0, int bstep = 1) {
+                                             F &&f, int bofs = 0, int bstep = 1,
+                                             const int4 *__restrict__ bpk = nullptr) {
     static_assert(G >= 16, "segment lookup scans 16, 32 or 64 lanes");
     const int l = __lane_id() & (G - 1);
     const int gbase = __lane_id() & ~(G - 1);
@@ -83,8 +87,14 @@ __device__ __forceinline__ void expand_group(int lo, int cnt, double a,
         double bv[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            col[u] = act[u] ? jb[e[u]] : 0;
-            bv[u] = (VALUES && act[u]) ? bval[e[u]] : 0.0;
+            if (PACKED) {
+                const int4 rec = act[u] ? bpk[e[u]] : make_int4(0, 0, 0, 0);
+                col[u] = rec.x;
+                bv[u] = __hiloint2double(rec.w, rec.z);
+            } else {
+                col[u] = act[u] ? jb[e[u]] : 0;
+                bv[u] = (VALUES && act[u]) ? bval[e[u]] : 0.0;
+            }
         }
 #pragma unroll
         for (int u = 0; u < U; u++) {

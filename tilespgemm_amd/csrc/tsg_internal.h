@@ -162,6 +162,7 @@ struct RowPlan {
     // per big row span bitmaps kept from the symbolic phase (rank mode)
     bool use_rank = false;
     unsigned *rowbits = nullptr;     // [nsb_total * 32] words
+    int4 *bpk = nullptr;             // [nnz(B)] {column, 0, value} records (rank mode)
     int64_t *item_off = nullptr;     // output offset (within C) per item
     // B with rows sorted, when the input's were not (set after row_symbolic):
     // the plan ran on (B.ptr, b_idx); numeric phases gather values by b_perm

@@ -721,8 +721,11 @@ __global__ void __launch_bounds__(BAND_T) k_big_symbolic(
             wave, NW);
         __syncthreads();
         if (rowbits) {
+            // written once, read once by the numeric phase much later:
+            // streaming stores keep the L2 for B's rows
             unsigned *dst = rowbits + so * (1 << (SUB_SHIFT - 5));
-            for (int i = threadIdx.x; i < words; i += BAND_T) dst[i] = bits[i];
+            for (int i = threadIdx.x; i < words; i += BAND_T)
+                __builtin_nontemporal_store(bits[i], dst + i);
         }
         // sub-block counts: a sub-block's 32 words are one half-wave, read
         // consecutively (no bank conflicts) and summed across the half
@@ -1025,7 +1028,8 @@ __device__ __forceinline__ void item_words(const unsigned *__restrict__ src, int
 #pragma unroll
     for (int j = 0; j < WCH; j++) {
         const int w = j * 64 + l;
-        wv[j] = w < nwords ? src[w] : 0u;
+        // read once: a streaming load keeps the L2 for B's rows
+        wv[j] = w < nwords ? __builtin_nontemporal_load(src + w) : 0u;
     }
 }
 
@@ -1061,7 +1065,7 @@ __device__ __forceinline__ int item_rank_build(RankSmem<WORDS> &S, unsigned (&wv
         cnt += tot[j];
     }
     wave_lds_fence();
-    for (int i = l; i < cnt; i += 64) jc[o + i] = cols[i];
+    for (int i = l; i < cnt; i += 64) __builtin_nontemporal_store(cols[i], jc + o + i);
     wave_lds_fence();
     for (int i = l; i < cnt; i += 64) S.vals[i] = 0.0;
     wave_lds_fence();
@@ -1074,7 +1078,7 @@ __device__ __forceinline__ void item_store_values(RankSmem<WORDS> &S, int cnt, i
                                                   double *__restrict__ c) {
     const int l = __lane_id();
     wave_lds_fence();
-    for (int i = l; i < cnt; i += 64) c[o + i] = S.vals[i];
+    for (int i = l; i < cnt; i += 64) __builtin_nontemporal_store(S.vals[i], c + o + i);
     wave_lds_fence();
 }
 
@@ -1105,7 +1109,8 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric_rank(
     const unsigned *__restrict__ rowbits, const int *__restrict__ span_sb0,
     const int64_t *__restrict__ sb_off, const int *__restrict__ ia, const int *__restrict__ ja,
     const double *__restrict__ a, const int *__restrict__ jb, const double *__restrict__ b,
-    BlockPtrs bp, int *__restrict__ jc, double *__restrict__ c, int b_strict) {
+    const int4 *__restrict__ bpk, BlockPtrs bp, int *__restrict__ jc, double *__restrict__ c,
+    int b_strict) {
     extern __shared__ __align__(16) unsigned char smem[];
     RankSmem<WORDS> &S = reinterpret_cast<RankSmem<WORDS> *>(smem)[threadIdx.x >> 6];
     const int l = __lane_id();
@@ -1125,6 +1130,8 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric_rank(
         src = rowbits + sb_off[item.w] * (1 << (SUB_SHIFT - 5)) +
               ((item.y >> 5) - (span_sb0[item.w] << (SUB_SHIFT - 5)));
     };
+    // (streaming loads of the item descriptors and A entries, read once per
+    // item, were measured slower: 158 -> 166 ms on R-MAT 2^20 A*A^T)
     if (wave_id < nitems) {
         item = items[wave_id];
         locate();
@@ -1173,7 +1180,7 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric_rank(
             double a2;
             load_range(t1, lo_n, hi_n);
             load_t(g + 128, t2, a2);
-            expand_group<64, RANK_U, true>(lo, hi - lo, ag, jb, b, S.mark, f);
+            expand_group<64, RANK_U, true, true>(lo, hi - lo, ag, jb, b, S.mark, f, 0, 1, bpk);
             lo = lo_n;
             hi = hi_n;
             t1 = t2;
@@ -1183,6 +1190,17 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric_rank(
         if (next < nitems) locate();
         item_store_values(S, cnt, o, c);
     }
+}
+
+// B's entries as 16-byte {column, 0, value lo, value hi} records for the
+// big-row numeric kernel's gathers.
+__global__ void __launch_bounds__(256) k_pack_b(int64_t nnz, const int *__restrict__ jb,
+                                                const double *__restrict__ b,
+                                                int4 *__restrict__ bpk) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nnz) return;
+    const double v = b[q];
+    bpk[q] = make_int4(jb[q], 0, __double2loint(v), __double2hiint(v));
 }
 
 // The plan's block pointers in the layout they were built in; without a
@@ -1357,6 +1375,8 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
                                   : std::min<size_t>(size_t(48) << 30, free_b / 10 * 4);
         p.use_rank = bits_bytes <= budget;
         p.rowbits = p.use_rank ? ws.alloc_n<unsigned>(bits_bytes / 4 + 1) : nullptr;
+        // B as {column, value} records for the rank kernel's gathers
+        p.bpk = p.use_rank ? ws.alloc_n<int4>((size_t)B.nnz + 1) : nullptr;
         // Spans this wide (at least item_sort_min windows of 16 sub-blocks)
         // make enough bands and items for them to run in column order, which
         // a radix sort arranges; the block pointers are then laid out block
@@ -1550,12 +1570,17 @@ static void row_numeric_t(RowPlan &p, const CsrIn &A, const CsrIn &B, const int6
     tm.end();
     if (p.nitems > 0 && p.use_rank) {
         tm.begin("row.big_numeric");
+        if (B.nnz > 0) {
+            hipLaunchKernelGGL(k_pack_b, dim3(grid_for((size_t)B.nnz, 256)), dim3(256), 0, s,
+                               B.nnz, B.idx, B.val, p.bpk);
+            TSG_LAUNCH_CHECK();
+        }
         auto launch_rank = [&](auto kern, size_t sm) {
             set_smem(kern, sm);
             const unsigned grid = item_grid(p.nitems);
             hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * ITEM_WAVES), sm, s, p.nitems, p.items,
                                p.item_off, p.rowbits, p.span_sb0, p.sb_off, A.ptr, A.idx, A.val,
-                               B.idx, B.val, block_ptrs(p, B.ptr), d_jc, d_c,
+                               B.idx, B.val, p.bpk, block_ptrs(p, B.ptr), d_jc, d_c,
                                (int)p.b_strict);
             TSG_LAUNCH_CHECK();
         };
