@@ -435,6 +435,35 @@ def served_section(args, served_sweep=(0.5, 0.8)):
     run(*percall, None)
     sat_s = run(*served, None)
     sat_p = run(*percall, None)
+
+    # Where the time goes: the same requests split into the R-MAT ones and
+    # the rest, each stream alone at saturation in both modes, and the whole
+    # mix as plain device-API calls in arrival order (no server).  Served
+    # time is at least the R-MAT stream's, which batching cannot shorten
+    # (one product each, already filling the GPU), so the served speed-up is
+    # at most (per-call R-MAT + per-call rest) / served R-MAT.
+    all_ops = ops
+    split = {}
+    for label, pick_kind in (("rmat", lambda kd: kd == 2), ("rest", lambda kd: kd != 2)):
+        ops = [op for op, kd in zip(all_ops, kinds) if pick_kind(kd)]
+        nreq = len(ops)
+        split[label] = {"requests": nreq, "served_s": run(*served, None)["span_s"],
+                        "per_call_s": run(*percall, None)["span_s"]}
+    ops = all_ops
+    nreq = len(ops)
+    hp = tsg.Handle(0)
+    for A, B in ops[:8]:
+        hp.spgemm(A, B)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for A, B in ops:
+        ic_, jc_, c_ = hp.spgemm(A, B)
+        torch.cuda.synchronize()
+        del ic_, jc_, c_
+    plain_s = time.perf_counter() - t1
+    hp.close()
+    bound = (split["rmat"]["per_call_s"] + split["rest"]["per_call_s"]) / split["rmat"]["served_s"]
+    log(f"[bench] split: {split}, plain per-call {plain_s:.3f} s, speed-up bound {bound:.2f}")
     log(f"[bench] saturation: served {sat_s['req_s']:.0f} req/s (mean batch "
         f"{sat_s['mean_batch']:.1f}), per-call {sat_p['req_s']:.0f} req/s")
     sweep = []
@@ -471,6 +500,15 @@ def served_section(args, served_sweep=(0.5, 0.8)):
         "saturation_per_call_p50_ms": round(sat_p["p50_ms"], 2),
         "saturation_per_call_p99_ms": round(sat_p["p99_ms"], 2),
         "mean_batch": round(sat_s["mean_batch"], 2),
+        "plain_per_call_req_s": round(nreq / plain_s, 1),
+        "breakdown": {"what": "each sub-stream alone at saturation (s); served time >= the "
+                              "R-MAT stream's, so speed-up <= (per-call R-MAT + per-call rest) "
+                              "/ served R-MAT",
+                      "rmat": {k: round(v, 4) if isinstance(v, float) else v
+                               for k, v in split["rmat"].items()},
+                      "rest": {k: round(v, 4) if isinstance(v, float) else v
+                               for k, v in split["rest"].items()},
+                      "speedup_bound": round(bound, 2)},
         "sweep": sweep,
     }
 

@@ -267,6 +267,9 @@ typedef struct {
     double queue_ms;             /* submission -> batch start               */
     double total_ms;             /* submission -> result ready              */
     int batch_size;              /* requests in the batch that served it    */
+    int64_t products;            /* intermediate products of A * B (as
+                                    tsg_intermediate_products), the basis
+                                    of the admission estimate              */
 } tsg_result_t;
 
 int tsg_server_create(tsg_server_t *server, int device, const tsg_server_config_t *config);

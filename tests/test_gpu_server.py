@@ -63,6 +63,8 @@ def test_served_results_equal_alone(workers, max_batch):
         ic1, jc1, c1 = h.spgemm(dA, dB)
         alone = (ic1.cpu().numpy(), jc1.cpu().numpy(), c1.cpu().numpy())
         assert_csr_equal((ic, jc, c), alone)
+        # the estimator's products, summed for many requests at once
+        assert res.products == h.intermediate_products(dA, dB)
         sizes.append(res.batch_size)
     # the last request against the oracle too (structure and bits)
     m, n, k, A, B, same = mix[-1]
@@ -99,19 +101,47 @@ def test_concurrent_submitters_and_budget():
         assert_csr_equal(g[:3], O.csr_spgemm(m, k, A, B))
 
 
-def test_malformed_request_fails_alone():
-    """A request whose A has a column outside B's rows is rejected by the
-    estimator with an error of its own; the requests around it are served."""
+def _malformed(kind):
+    """(A, B) host operands, 2000 x 2000, malformed as `kind` says."""
+    A = gen.erdos_renyi(2000, 6.0, seed=3)
+    ia, ja, a = (x.copy() for x in A)
+    B = gen.transpose(2000, 2000, *A)
+    ib, jb, b = (x.copy() for x in B)
+    if kind == "a_col_high":
+        ja[7] = 5000
+    elif kind == "a_col_negative":
+        ja[ja.size - 3] = -1
+    elif kind == "a_ptr_descent":
+        ia[5] = ia[7] + 1
+    elif kind == "a_ptr_end":
+        ia[-1] -= 1
+    elif kind == "b_col_high":
+        jb[11] = 2000
+    elif kind == "b_ptr_start":
+        ib[0] = 1
+    return (ia, ja, a), (ib, jb, b), kind.startswith("a_")
+
+
+@pytest.mark.parametrize("kind", ["a_col_high", "a_col_negative", "a_ptr_descent", "a_ptr_end",
+                                  "b_col_high", "b_ptr_start"])
+def test_malformed_request_fails_alone(kind):
+    """A request with a malformed operand is rejected by the estimator with
+    an error of its own; the requests around it (submitted with it, so
+    estimated in the same group) are served.  A-side faults are submitted as
+    A*A (B is A), B-side faults as A*B with B malformed."""
     import torch
     assert torch.cuda.is_available()
     A = gen.erdos_renyi(2000, 6.0, seed=3)
-    bad = (A[0], A[1].copy(), A[2])
-    bad[1][7] = 5000
+    badA, badB, a_side = _malformed(kind)
     dA = tsg.DeviceCsr.from_host(2000, 2000, *A)
-    dbad = tsg.DeviceCsr.from_host(2000, 2000, *bad)
+    if a_side:
+        dbad = tsg.DeviceCsr.from_host(2000, 2000, *badA)
+        pair = (dbad, dbad)
+    else:
+        pair = (dA, tsg.DeviceCsr.from_host(2000, 2000, *badB))
     srv = tsg.Server(0, workers=2, max_batch=8)
     try:
-        t = [srv.submit(dA, dA), srv.submit(dbad, dA), srv.submit(dA, dA)]
+        t = [srv.submit(dA, dA), srv.submit(*pair), srv.submit(dA, dA)]
         with pytest.raises(tsg.TsgError):
             srv.wait(t[1])
         srv.release(t[1])

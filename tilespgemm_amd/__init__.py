@@ -224,7 +224,8 @@ class Result(C.Structure):
     """tsg_result_t (include/tsg.h)."""
     _fields_ = [("status", C.c_int), ("m", C.c_int), ("nnz", C.c_int64),
                 ("ic", C.c_void_p), ("jc", C.c_void_p), ("c", C.c_void_p),
-                ("queue_ms", C.c_double), ("total_ms", C.c_double), ("batch_size", C.c_int)]
+                ("queue_ms", C.c_double), ("total_ms", C.c_double), ("batch_size", C.c_int),
+                ("products", C.c_int64)]
 
 
 class Server:

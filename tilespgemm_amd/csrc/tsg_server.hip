@@ -11,6 +11,7 @@
 // on the device.  Output buffers come from a best-fit device cache and are
 // returned when the caller releases the result.
 #include "tsg_internal.h"
+#include "tsg_device.h"
 
 #include <chrono>
 #include <condition_variable>
@@ -85,6 +86,109 @@ private:
     std::unordered_map<void *, size_t> size_;
 };
 
+// ------------------------------------------------------ admission estimates
+// The estimates of every request taken from the submission queue come from
+// one host round trip: each request's A (row pointers; columns in [0, n))
+// and, unless B is A itself, its B (columns in [0, k)) are checked, and A's
+// intermediate products summed (main_spgemm-col.cu:4950-4955), over the
+// requests' arrays laid end to end.  One list entry per matrix checked;
+// ib is set on A's entries only, whose products are counted.
+struct EstDesc {
+    const int *ptr, *idx;
+    const int *ib;
+    int rows, cols;
+    int64_t nnz;
+    int req;
+};
+
+constexpr int EST_PER = 8;
+
+// Largest p in [0, count) with off[p] <= x.
+__device__ __forceinline__ int est_find(const int64_t *__restrict__ off, int count, int64_t x) {
+    int lo = 0, hi = count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= x) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Row pointers: ptr[0] = 0, ptr[rows] = nnz, non-decreasing, inside
+// [0, nnz].  Bit 0 of flags[req] marks a failure.  A block takes
+// EST_PER * 256 consecutive rows of the list; the entry of its first row is
+// searched once, each thread steps to the next entry at a boundary.  Reads
+// stay inside ptr[0..rows].
+__global__ void __launch_bounds__(256) k_est_rows(int count, const EstDesc *__restrict__ desc,
+                                                  const int64_t *__restrict__ roff, int64_t rows,
+                                                  int *__restrict__ flags) {
+    const int64_t r0 = (int64_t)blockIdx.x * (256 * EST_PER);
+    if (r0 >= rows) return;
+    __shared__ int s_p;
+    if (threadIdx.x == 0) s_p = est_find(roff, count, r0);
+    __syncthreads();
+    int p = s_p;
+#pragma unroll
+    for (int j = 0; j < EST_PER; j++) {
+        const int64_t r = r0 + j * 256 + threadIdx.x;
+        if (r >= rows) break;
+        while (r >= roff[p + 1]) p++;   // stops below count: r < roff[count]
+        const EstDesc &d = desc[p];
+        const int64_t i = r - roff[p];
+        const int64_t lo = d.ptr[i], hi = d.ptr[i + 1];
+        const bool bad = lo > hi || lo < 0 || hi > d.nnz || (i == 0 && lo != 0) ||
+                         (i == d.rows - 1 && hi != d.nnz);
+        if (bad) atomicOr(&flags[d.req], 1);
+    }
+}
+
+// Columns in [0, cols) (bit 1 of flags[req]); with PRODUCTS, the length of
+// B's row at each valid column added to prods[req].  Laid out as
+// k_est_rows; a lane's sums are flushed when its entry changes, and at the
+// end a wave whose lanes all hold the same entry adds one sum.
+template <bool PRODUCTS>
+__global__ void __launch_bounds__(256) k_est_entries(int count, const EstDesc *__restrict__ desc,
+                                                     const int64_t *__restrict__ eoff, int64_t nnz,
+                                                     int *__restrict__ flags,
+                                                     unsigned long long *__restrict__ prods) {
+    const int64_t q0 = (int64_t)blockIdx.x * (256 * EST_PER);
+    if (q0 >= nnz) return;
+    __shared__ int s_p;
+    if (threadIdx.x == 0) s_p = est_find(eoff, count, q0);
+    __syncthreads();
+    int p = s_p;
+    int cur = -1;
+    unsigned long long acc = 0;
+#pragma unroll
+    for (int j = 0; j < EST_PER; j++) {
+        const int64_t q = q0 + j * 256 + threadIdx.x;
+        if (q >= nnz) break;
+        while (q >= eoff[p + 1]) p++;   // stops below count: q < eoff[count]
+        const EstDesc &d = desc[p];
+        const int c = d.idx[q - eoff[p]];
+        if (c < 0 || c >= d.cols) {
+            atomicOr(&flags[d.req], 2);
+            continue;
+        }
+        if (PRODUCTS) {
+            if (p != cur) {
+                if (cur >= 0 && acc) atomicAdd(&prods[desc[cur].req], acc);
+                cur = p;
+                acc = 0;
+            }
+            acc += (unsigned long long)((int64_t)d.ib[c + 1] - (int64_t)d.ib[c]);
+        }
+    }
+    if (PRODUCTS) {
+        const int c0 = __shfl(cur, 0, 64);
+        if (__ballot(cur != c0) == 0) {
+            const unsigned long long sum = group_sum<64>(acc);
+            if (lane_id() == 0 && c0 >= 0 && sum) atomicAdd(&prods[desc[c0].req], sum);
+        } else if (cur >= 0 && acc) {
+            atomicAdd(&prods[desc[cur].req], acc);
+        }
+    }
+}
+
 struct BatchOut {
     void *ic = nullptr, *jc = nullptr, *c = nullptr;
     int refs = 0;
@@ -94,6 +198,7 @@ struct Req {
     uint64_t ticket = 0;
     tsg_product_t prod{};
     size_t est = 0;
+    int64_t products = 0;
     Clock::time_point t_submit, t_start, t_done;
     bool done = false;
     tsg_result_t res{};
@@ -120,6 +225,7 @@ struct tsg_server_s {
     tsg_handle_t est = nullptr;        // owned by the estimator thread
     DevicePool pool;
 
+    void estimate(std::deque<Req *> &take, std::vector<int> &status);
     void run_batch(tsg_handle_t h, std::vector<Req *> &batch);
     void worker_main();
     void estimator_main();
@@ -140,31 +246,10 @@ void tsg_server_s::estimator_main() {
             take.swap(pending);
         }
         std::vector<int> status(take.size(), TSG_OK);
-        for (size_t i = 0; i < take.size(); i++) {
-            Req *r = take[i];
-            uint64_t products = 0;
-            int rc = dev_ok ? tsg_intermediate_products(est, r->prod.m, r->prod.n, r->prod.ia,
-                                                        r->prod.ja, r->prod.nnz_a, r->prod.ib,
-                                                        &products)
-                            : TSG_ERR_NODEVICE;
-            if (rc == TSG_OK) {
-                // B checked here too, so a malformed request fails alone
-                // instead of failing the batch it would join
-                try {
-                    TSG_CHECK(r->prod.nnz_b < (int64_t(1) << 31), TSG_ERR_OVERFLOW,
-                              "B: inputs are limited to 2^31-1 entries");
-                    CsrIn B{r->prod.n, r->prod.k, r->prod.ib, r->prod.jb, nullptr, r->prod.nnz_b};
-                    if (validate_csr_pair(B, B, est->aux, est->stream) & 0x3)
-                        rc = TSG_ERR_INVALID;
-                } catch (const TsgError &e) {
-                    rc = e.code;
-                }
-            }
-            status[i] = rc;
-            const uint64_t cap =
-                std::min<uint64_t>(products, (uint64_t)r->prod.m * (uint64_t)r->prod.k);
-            r->est = rc == TSG_OK ? 16 * ((size_t)r->prod.m + 2) + 12 * (size_t)cap : 0;
-        }
+        if (dev_ok)
+            estimate(take, status);
+        else
+            std::fill(status.begin(), status.end(), (int)TSG_ERR_NODEVICE);
         std::lock_guard<std::mutex> lk(mu);
         bool failed = false;
         for (size_t i = 0; i < take.size(); i++) {
@@ -181,6 +266,96 @@ void tsg_server_s::estimator_main() {
         }
         cv_work.notify_all();
         if (failed) cv_done.notify_all();
+    }
+}
+
+void tsg_server_s::estimate(std::deque<Req *> &take, std::vector<int> &status) {
+    const int n = (int)take.size();
+    std::vector<EstDesc> la, lb;          // A of every request, B where it is not A
+    std::vector<int64_t> ra(1, 0), ea(1, 0), rb(1, 0), eb(1, 0);
+    for (int i = 0; i < n; i++) {
+        const tsg_product_t &x = take[i]->prod;
+        int rc = TSG_OK;
+        if (x.m < 0 || x.n < 0 || x.k < 0 || x.nnz_a < 0 || x.nnz_b < 0)
+            rc = TSG_ERR_INVALID;
+        else if (!x.ia || !x.ib || (x.nnz_a > 0 && !x.ja) || (x.nnz_b > 0 && !x.jb))
+            rc = TSG_ERR_INVALID;
+        else if (x.nnz_a >= (int64_t(1) << 31) || x.nnz_b >= (int64_t(1) << 31))
+            rc = TSG_ERR_OVERFLOW;
+        else if ((x.m == 0 && x.nnz_a != 0) || (x.n == 0 && x.nnz_b != 0))
+            rc = TSG_ERR_INVALID;
+        status[i] = rc;
+        if (rc != TSG_OK) continue;
+        la.push_back(EstDesc{x.ia, x.ja, x.ib, x.m, x.n, x.nnz_a, i});
+        ra.push_back(ra.back() + x.m);
+        ea.push_back(ea.back() + x.nnz_a);
+        const bool b_is_a = x.ib == x.ia && x.jb == x.ja && x.n == x.m && x.k == x.n &&
+                            x.nnz_b == x.nnz_a;
+        if (!b_is_a) {
+            lb.push_back(EstDesc{x.ib, x.jb, nullptr, x.n, x.k, x.nnz_b, i});
+            rb.push_back(rb.back() + x.n);
+            eb.push_back(eb.back() + x.nnz_b);
+        }
+    }
+    std::vector<unsigned long long> prods(n, 0);
+    if (!la.empty()) {
+        try {
+            hipStream_t st = est->stream;
+            Arena &ws = est->aux;
+            ws.reset();
+            const int na = (int)la.size(), nb = (int)lb.size();
+            EstDesc *d_la = ws.alloc_n<EstDesc>(na + nb + 1);
+            EstDesc *d_lb = d_la + na;
+            int64_t *d_off = ws.alloc_n<int64_t>(2 * ((size_t)na + 1) + 2 * ((size_t)nb + 1));
+            int64_t *d_ra = d_off, *d_ea = d_ra + na + 1, *d_rb = d_ea + na + 1,
+                    *d_eb = d_rb + nb + 1;
+            // flags (int) then sums (u64) in one block, cleared and read back together
+            unsigned long long *d_sum = ws.alloc_n<unsigned long long>(2 * (size_t)n);
+            int *d_flag = reinterpret_cast<int *>(d_sum + n);
+            std::vector<EstDesc> dl(la);
+            dl.insert(dl.end(), lb.begin(), lb.end());
+            std::vector<int64_t> offs(ra);
+            offs.insert(offs.end(), ea.begin(), ea.end());
+            offs.insert(offs.end(), rb.begin(), rb.end());
+            offs.insert(offs.end(), eb.begin(), eb.end());
+            TSG_HIP_TRY(hipMemcpyAsync(d_la, dl.data(), sizeof(EstDesc) * dl.size(),
+                                       hipMemcpyHostToDevice, st));
+            TSG_HIP_TRY(hipMemcpyAsync(d_off, offs.data(), sizeof(int64_t) * offs.size(),
+                                       hipMemcpyHostToDevice, st));
+            TSG_HIP_TRY(hipMemsetAsync(d_sum, 0, 2 * sizeof(unsigned long long) * n, st));
+            auto grid = [](int64_t w) { return dim3(grid_for((size_t)w, 256 * EST_PER)); };
+            if (ra[na] > 0)
+                hipLaunchKernelGGL(k_est_rows, grid(ra[na]), dim3(256), 0, st, na, d_la, d_ra,
+                                   ra[na], d_flag);
+            if (ea[na] > 0)
+                hipLaunchKernelGGL(k_est_entries<true>, grid(ea[na]), dim3(256), 0, st, na, d_la,
+                                   d_ea, ea[na], d_flag, d_sum);
+            if (nb > 0 && rb[nb] > 0)
+                hipLaunchKernelGGL(k_est_rows, grid(rb[nb]), dim3(256), 0, st, nb, d_lb, d_rb,
+                                   rb[nb], d_flag);
+            if (nb > 0 && eb[nb] > 0)
+                hipLaunchKernelGGL((k_est_entries<false>), grid(eb[nb]), dim3(256), 0, st, nb,
+                                   d_lb, d_eb, eb[nb], d_flag, nullptr);
+            TSG_LAUNCH_CHECK();
+            std::vector<unsigned long long> back(2 * (size_t)n);
+            TSG_HIP_TRY(hipMemcpyAsync(back.data(), d_sum, sizeof(unsigned long long) * back.size(),
+                                       hipMemcpyDeviceToHost, st));
+            TSG_HIP_TRY(hipStreamSynchronize(st));
+            const int *flags = reinterpret_cast<const int *>(back.data() + n);
+            for (int i = 0; i < n; i++) {
+                prods[i] = back[i];
+                if (status[i] == TSG_OK && flags[i]) status[i] = TSG_ERR_INVALID;
+            }
+        } catch (const TsgError &e) {
+            for (int i = 0; i < n; i++)
+                if (status[i] == TSG_OK) status[i] = e.code;
+        }
+    }
+    for (int i = 0; i < n; i++) {
+        Req *r = take[i];
+        const uint64_t cap = std::min<uint64_t>(prods[i], (uint64_t)r->prod.m * (uint64_t)r->prod.k);
+        r->est = status[i] == TSG_OK ? 16 * ((size_t)r->prod.m + 2) + 12 * (size_t)cap : 0;
+        r->products = status[i] == TSG_OK ? (int64_t)prods[i] : 0;
     }
 }
 
@@ -235,6 +410,7 @@ void tsg_server_s::run_batch(tsg_handle_t h, std::vector<Req *> &batch) {
         r->res.queue_ms = ms_between(r->t_submit, r->t_start);
         r->res.total_ms = ms_between(r->t_submit, t1);
         r->res.batch_size = n;
+        r->res.products = r->products;
         e0 += per[i];
         r0 += r->prod.m;
     }

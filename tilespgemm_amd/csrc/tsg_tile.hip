@@ -19,12 +19,17 @@
 //            owns row r; matched (A_IK, B_KJ) pairs come from a ballot-driven
 //            intersection; maskC[r] = OR of B_KJ row masks selected by the set
 //            bits of A_IK row r.  Also produces every (tile, row) segment's
-//            position inside its CSR row, so step3 writes CSR directly.
+//            position inside its CSR row, so step3 writes CSR directly, and
+//            each C tile's count of matched pairs.
+//   pairs    the matched pairs themselves, stored per C tile in ascending K,
+//            and the C tiles binned by their output count (<= 16, <= 32,
+//            <= 64, larger tiles split into 64-output parts).
 //   step3    FP64 numeric [step4, main_spgemm-col.cu:2094-2397,3386-3754]:
-//            lane r of a tile walks A_IK row r and the selected B_KJ rows in
-//            ascending K, accumulating a padded 16x17 LDS tile with one FMA
-//            per product (no atomics; products of every output in A-row
-//            order), then writes its row's entries straight into C's CSR.
+//            one lane per output of a work unit keeps the output's sum in a
+//            register and takes its products -- the set bits of
+//            maskA[r] & cmaskB[c] of every pair, pairs in ascending K -- with
+//            one FMA each, in A-row order (no atomics, no LDS accumulator),
+//            then writes the entry straight into C's CSR.
 #include "tsg_internal.h"
 #include "tsg_device.h"
 #include "tsg_expand.h"
