@@ -323,7 +323,11 @@ enum {
     TSG_B_MASK,
     TSG_B_ROWPTR,
     TSG_B_IDX,
-    TSG_B_VAL,
+    TSG_B_VAL,            /* double[nnzB] column-major inside each tile:
+                             columns ascending, rows ascending in a column
+                             (the reference's transposed B sub-tiles,
+                             main_spgemm-col.cu:5255-5290); after a numeric
+                             phase                                        */
     TSG_B_CSC_PTR,        /* int32[blknB+1]                                */
     TSG_B_CSC_ROW,        /* int32[numblkB]  tile row K per tile column     */
     TSG_B_CSC_TILE,       /* int32[numblkB]  tile-CSR position of (K, J)   */

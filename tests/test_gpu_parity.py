@@ -239,6 +239,11 @@ def test_tile_plan_arrays_match_tile_oracle(handle):
         np.testing.assert_array_equal(P("A_VAL").view(np.int64), tA["val"].view(np.int64))
         np.testing.assert_array_equal(P("B_TILE_PTR"), tB["tile_ptr"])
         np.testing.assert_array_equal(P("B_MASK"), tB["mask"])
+        # B's values column-major inside each tile (columns, then rows)
+        tiles = np.repeat(np.arange(tB["tile_col"].size), np.diff(tB["tile_off"]))
+        order = np.lexsort((tB["idx"] >> 4, tB["idx"] & 15, tiles))
+        np.testing.assert_array_equal(P("B_VAL").view(np.int64),
+                                      tB["val"][order].view(np.int64))
         np.testing.assert_array_equal(P("B_CSC_PTR"), csc["col_ptr"])
         np.testing.assert_array_equal(P("B_CSC_ROW"), csc["col_row"])
         np.testing.assert_array_equal(P("B_CSC_TILE"), csc["col_tile"])

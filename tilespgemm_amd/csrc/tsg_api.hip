@@ -523,7 +523,7 @@ static bool plan_array(tsg_handle_t h, int which, const void **ptr, size_t *byte
     case TSG_B_MASK: *ptr = B.mask; *bytes = 2 * 16 * (size_t)B.numblk; return true;
     case TSG_B_ROWPTR: *ptr = B.rowptr; *bytes = 16 * (size_t)B.numblk; return true;
     case TSG_B_IDX: *ptr = B.idx; *bytes = (size_t)B.nnz; return true;
-    case TSG_B_VAL: *ptr = p.b_val_shared ? A.val : B.val; *bytes = 8 * (size_t)B.nnz; return true;
+    case TSG_B_VAL: *ptr = B.valc; *bytes = 8 * (size_t)B.nnz; return true;
     case TSG_B_CSC_PTR: *ptr = B.csc_ptr; *bytes = 4 * ((size_t)B.blkn + 1); return true;
     case TSG_B_CSC_ROW: *ptr = B.csc_row; *bytes = 4 * (size_t)B.numblk; return true;
     case TSG_B_CSC_TILE: *ptr = B.csc_tile; *bytes = 4 * (size_t)B.numblk; return true;

@@ -195,14 +195,16 @@ struct TileMat {
     int *csc_row = nullptr;      // [numblk]
     int *csc_tile = nullptr;     // [numblk]
     uint16_t *cmask = nullptr;   // [16*numblk] column masks, bit 15-k for row k (B only)
+    uint8_t *colptr = nullptr;   // [16*numblk] in-tile column starts, column-major (B only)
     // entry -> payload position permutation (values refreshed per numeric)
     int *perm = nullptr;         // [nnz] CSR position -> tile payload position
+    int *permc = nullptr;        // [nnz] CSR position -> column-major payload position (B only)
+    double *valc = nullptr;      // [nnz] values column-major inside each tile (B only)
 };
 
 struct TilePlan {
     bool valid = false;
     bool b_is_a = false;         // B has A's structure (same index arrays)
-    bool b_val_shared = false;   // last numeric phase: B's values were A's
     TileMat A, B;
     int numblkC = 0;
     int *tptrC = nullptr;        // [blkm+1]

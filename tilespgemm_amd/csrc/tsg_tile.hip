@@ -22,8 +22,8 @@
 //            position inside its CSR row, so step3 writes CSR directly, and
 //            each C tile's count of matched pairs.
 //   pairs    the matched pairs themselves, stored per C tile in ascending K,
-//            and the C tiles binned by their output count (<= 16, <= 32,
-//            <= 64, larger tiles split into 64-output parts).
+//            and the numeric phase's work units: each C tile's 64-output
+//            parts and its remainder, binned by size (<= 16, <= 32, more).
 //   step3    FP64 numeric [step4, main_spgemm-col.cu:2094-2397,3386-3754]:
 //            one lane per output of a work unit keeps the output's sum in a
 //            register and takes its products -- the set bits of
@@ -510,24 +510,31 @@ __global__ void __launch_bounds__(64 * FP_WAVES) k_fill_pairs(
     }
 }
 
-// Work units of the numeric phase by C tile nnz: bin 0 holds tiles of <= 16
-// outputs (four per wave), bin 1 <= 32 (two per wave), bin 2 <= 64 (one per
-// wave), bin 3 the 64-output parts of larger tiles (a wave per part); every
-// lane owns one output.  Tiles without outputs are left out.  A unit is
-// t * 4 + part (int64).  cnt[0..3] counts units, then the scatter places
-// them at their bin's start (bins back to back) plus a slot from fill.
-// (Measured on the 27-point stencil 64^3, whose larger C tiles hold ~74
-// outputs: a wave per 64-output part beats one wave with two outputs per
-// lane, 1.43 vs 1.78 ms, and staging pair values in LDS as well as the
-// metadata, 1.77 ms: the kernel is bound by its VALU instructions per
-// product, not by load latency.)
-constexpr int TBINS = 4;
+// Work units of the numeric phase.  A C tile of n outputs gives n / 64
+// units of 64 outputs and, when n % 64 is not 0, one unit of the remaining
+// outputs; every lane owns one output.  Units go to bins by their size:
+// bin 0 <= 16 outputs (four units per wave), bin 1 <= 32 (two per wave),
+// bin 2 more (one per wave).  A unit is t * 4 + part, its outputs the tile's
+// 64 * part onwards (int64).  cnt[0..2] counts units, then the scatter
+// places them at their bin's start (bins back to back) plus a slot from
+// fill.  (The 27-point stencil's larger C tiles hold 74 outputs; their
+// 10-output remainders, a wave each, had cost as much as the 64-output
+// parts: 1.44 ms of one-unit waves before, with half of them remainders.
+// A wave per 64-output part beats one wave with two outputs per lane, 1.43
+// vs 1.78 ms, and staging pair values in LDS as well as the metadata, 1.77
+// ms: the kernel is bound by its VALU instructions per product and by the
+// busiest lane of each wave, not by load latency.)
+constexpr int TBINS = 3;
 
-__device__ __forceinline__ int tile_bin(int64_t nnz) {
-    return nnz == 0 ? -1 : nnz <= 16 ? 0 : nnz <= 32 ? 1 : nnz <= 64 ? 2 : 3;
+__device__ __forceinline__ int rem_bin(int64_t nnz) {
+    const int rem = (int)(nnz & 63);
+    return rem == 0 ? -1 : rem <= 16 ? 0 : rem <= 32 ? 1 : 2;
 }
 
-__device__ __forceinline__ int tile_units(int64_t nnz) { return (int)((nnz + 63) >> 6); }
+// units of a tile with nnz outputs in bin q
+__device__ __forceinline__ int tile_units_in(int64_t nnz, int q) {
+    return (q == 2 ? (int)(nnz >> 6) : 0) + (rem_bin(nnz) == q ? 1 : 0);
+}
 
 __global__ void __launch_bounds__(256) k_tbin_count(int n, const int64_t *__restrict__ off,
                                                     int *__restrict__ cnt) {
@@ -536,15 +543,20 @@ __global__ void __launch_bounds__(256) k_tbin_count(int n, const int64_t *__rest
     __syncthreads();
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t nz = t < n ? off[t + 1] - off[t] : 0;
-    const int b = tile_bin(nz);
-    if (b >= 0) atomicAdd(&s_cnt[b], b == 3 ? tile_units(nz) : 1);
+#pragma unroll
+    for (int q = 0; q < TBINS; q++) {
+        const int u = tile_units_in(nz, q);
+        if (u) atomicAdd(&s_cnt[q], u);
+    }
     __syncthreads();
     if (threadIdx.x < TBINS && s_cnt[threadIdx.x]) atomicAdd(&cnt[threadIdx.x], s_cnt[threadIdx.x]);
 }
 
 // Slots come from a wave scan of the unit counts per bin, one LDS atomic
 // per wave and bin, and one global atomic per block and bin (a global
-// atomic per tile on four counters serialised the scatter).
+// atomic per tile on the counters serialised the scatter).  Bin 2 takes a
+// tile's parts 0, 1, ... (the remainder last, when it is there); bins 0 and
+// 1 take only a remainder, part n / 64.
 __global__ void __launch_bounds__(256) k_tbin_scatter(int n, const int64_t *__restrict__ off,
                                                       const int *__restrict__ cnt,
                                                       int *__restrict__ fill,
@@ -555,19 +567,17 @@ __global__ void __launch_bounds__(256) k_tbin_scatter(int n, const int64_t *__re
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const int64_t nz = t < n ? off[t + 1] - off[t] : 0;
-    const int b = tile_bin(nz);
-    const int u = b == 3 ? tile_units(nz) : (b >= 0 ? 1 : 0);   // units of this tile
-    int pos = 0;
+    int pos[TBINS] = {0, 0, 0};
+#pragma unroll
     for (int q = 0; q < TBINS; q++) {
-        const uint64_t mask = __ballot(b == q);
-        if (!mask) continue;
-        const int v = b == q ? u : 0;
+        const int v = tile_units_in(nz, q);
+        if (!__ballot(v != 0)) continue;
         const int incl = wave_incl_scan(v);
         const int wtot = __builtin_amdgcn_readlane(incl, 63);
         int wbase = 0;
         if (lane == 0) wbase = atomicAdd(&s_cnt[q], wtot);
         wbase = __builtin_amdgcn_readlane(wbase, 0);
-        if (b == q) pos = wbase + incl - v;
+        pos[q] = wbase + incl - v;
     }
     __syncthreads();
     if (threadIdx.x < TBINS && s_cnt[threadIdx.x]) {
@@ -576,14 +586,27 @@ __global__ void __launch_bounds__(256) k_tbin_scatter(int n, const int64_t *__re
         s_base[threadIdx.x] = start + atomicAdd(&fill[threadIdx.x], s_cnt[threadIdx.x]);
     }
     __syncthreads();
-    for (int i = 0; i < u; i++) units[s_base[b] + pos + i] = (int64_t)t * 4 + i;
+    const int full = (int)(nz >> 6);
+    const int rb = rem_bin(nz);
+    const int u2 = tile_units_in(nz, 2);
+    for (int i = 0; i < u2; i++) units[s_base[2] + pos[2] + i] = (int64_t)t * 4 + i;
+    if (rb == 0 || rb == 1) units[s_base[rb] + pos[rb]] = (int64_t)t * 4 + full;
 }
 
-// Column masks of 16x16 tiles: cmask[16 t + c] has bit 15 - k set when the
-// tile holds entry (k, c) -- the transpose of the row masks, in the same
-// bit order.  16 lanes per tile, lane c builds column c.
-__global__ void __launch_bounds__(256) k_col_masks(int numblk, const uint16_t *__restrict__ mask,
-                                                   uint16_t *__restrict__ cmask) {
+// Column layout of 16x16 tiles (B only), the transposed sub-tiles of
+// main_spgemm-col.cu:5255-5290: cmask[16 t + c] has bit 15 - k set when the
+// tile holds entry (k, c) -- the transpose of the row masks, in the same bit
+// order; colptr[16 t + c] counts the tile's entries in columns before c; and
+// cpos[q] is the position of row-major payload entry q in the tile's
+// column-major payload (columns ascending, rows ascending in a column).
+// 16 lanes per tile, lane c builds column c (a group shares its tile, so
+// whole groups leave together).
+__global__ void __launch_bounds__(256) k_col_layout(int numblk, const int *__restrict__ tile_off,
+                                                    const uint16_t *__restrict__ mask,
+                                                    const uint8_t *__restrict__ rowptr,
+                                                    uint16_t *__restrict__ cmask,
+                                                    uint8_t *__restrict__ colptr,
+                                                    int *__restrict__ cpos) {
     const int64_t t = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
     const int c = threadIdx.x & 15;
     if (t >= numblk) return;
@@ -596,7 +619,30 @@ __global__ void __launch_bounds__(256) k_col_masks(int numblk, const uint16_t *_
         const unsigned row = (w[k >> 1] >> ((k & 1) * 16)) & 0xffffu;
         cm |= ((row >> (15 - c)) & 1u) << (15 - k);
     }
+    const int cnt = __popc(cm);
+    const int cstart = group_incl_scan<16>(cnt) - cnt;
     cmask[t * 16 + c] = (uint16_t)cm;
+    colptr[t * 16 + c] = (uint8_t)cstart;
+    const uint4 rp = *reinterpret_cast<const uint4 *>(rowptr + t * 16);
+    const int off = tile_off[t];
+    int j = 0;
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        const unsigned row = (w[k >> 1] >> ((k & 1) * 16)) & 0xffffu;
+        if ((row >> (15 - c)) & 1u) {
+            const int rw = (int)((k < 8 ? (k < 4 ? rp.x : rp.y) : (k < 12 ? rp.z : rp.w)) >>
+                                 ((k & 3) * 8)) & 255;
+            cpos[off + rw + __popc(row >> (16 - c))] = off + cstart + j;
+            j++;
+        }
+    }
+}
+
+// permc[i] = cpos[perm[i]]: CSR position -> column-major payload position.
+__global__ void k_compose_perm(int64_t nnz, const int *__restrict__ perm,
+                               const int *__restrict__ cpos, int *__restrict__ permc) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nnz) permc[i] = cpos[perm[i]];
 }
 
 // ------------------------------------------------------------------ step3
@@ -605,12 +651,15 @@ __global__ void __launch_bounds__(256) k_col_masks(int numblk, const uint16_t *_
 // order: matched pairs in ascending K (stored by k_fill_pairs), then the
 // columns k of A tile row r in ascending order.  The products of output
 // (r, c) of a pair are the k set in maskA[r] & cmaskB[c] (row r of A_IK
-// and column c of B_KJ both hold k); their values sit at A row r's start
-// plus the rank of k in the row, and B row k's start plus the rank of c.
-// A chunk of the tile's pairs has its fixed-size metadata (payload offsets,
-// row starts, row masks of A, row and column masks of B: 136 bytes a pair)
-// staged in LDS by one lane per pair with coalesced loads, so a product
-// costs two value loads; no atomics.
+// and column c of B_KJ both hold k).  A's values are row-major in its tile
+// and B's column-major, so both operands of a product sit at a base fixed
+// for the pair plus the rank of k in a mask fixed for the pair: A row r's
+// start + popc of maskA[r] above bit 15-k, B column c's start + popc of
+// cmaskB[c] above it.  A chunk of the tile's pairs has its fixed-size
+// metadata (payload offsets, A row starts and masks, B column starts and
+// masks: 104 bytes a pair) staged in LDS by one lane per pair with coalesced
+// loads, so a product costs a few bit operations and two value loads; no
+// atomics.
 struct TileArgs {
     const int *trowC, *tcolC;
     const uint16_t *maskC;
@@ -623,9 +672,9 @@ struct TileArgs {
     const uint16_t *maskA;
     const double *valA;
     const int *offB;
-    const uint8_t *rowptrB;
-    const uint16_t *maskB, *cmaskB;
-    const double *valB;
+    const uint8_t *colptrB;
+    const uint16_t *cmaskB;
+    const double *valB;          // column-major inside each tile
     const int64_t *ic;
     int *jc;
     double *c;
@@ -637,71 +686,73 @@ __device__ __forceinline__ unsigned byte_at(const uint4 &v, int i) {
 }
 
 struct PairMeta {
-    uint8_t rowptrA[16], rowptrB[16];   // row starts (bytes 0..31)
-    uint16_t maskA[16], maskB[16];      // row masks (bytes 32..95)
-    uint16_t cmaskB[16];                // column masks of B (bytes 96..127)
-    int offA, offB;                     // payload offsets (bytes 128..135)
+    uint8_t rowptrA[16], colptrB[16];   // row starts of A, column starts of B (bytes 0..31)
+    uint16_t maskA[16];                 // row masks of A (bytes 32..63)
+    uint16_t cmaskB[16];                // column masks of B (bytes 64..95)
+    int offA, offB;                     // payload offsets (bytes 96..103)
 };
 
-// Stage pair p's metadata (called by one lane per pair): nine loads, one
+// Stage pair p's metadata (called by one lane per pair): seven loads, one
 // level deep after the pair itself.
 __device__ __forceinline__ void stage_pair(const TileArgs &T, int64_t p, PairMeta &M) {
     const int2 pr = T.pairs[p];
     const int oa = T.offA[pr.x], ob = T.offB[pr.y];
     const uint4 ra = *reinterpret_cast<const uint4 *>(T.rowptrA + (size_t)pr.x * 16);
-    const uint4 rb = *reinterpret_cast<const uint4 *>(T.rowptrB + (size_t)pr.y * 16);
+    const uint4 cpb = *reinterpret_cast<const uint4 *>(T.colptrB + (size_t)pr.y * 16);
     const uint4 *ma = reinterpret_cast<const uint4 *>(T.maskA + (size_t)pr.x * 16);
-    const uint4 *mb = reinterpret_cast<const uint4 *>(T.maskB + (size_t)pr.y * 16);
     const uint4 *cb = reinterpret_cast<const uint4 *>(T.cmaskB + (size_t)pr.y * 16);
-    const uint4 ma0 = ma[0], ma1 = ma[1], mb0 = mb[0], mb1 = mb[1], cb0 = cb[0], cb1 = cb[1];
+    const uint4 ma0 = ma[0], ma1 = ma[1], cb0 = cb[0], cb1 = cb[1];
     uint4 *d = reinterpret_cast<uint4 *>(&M);
     d[0] = ra;
-    d[1] = rb;
+    d[1] = cpb;
     d[2] = ma0;
     d[3] = ma1;
-    d[4] = mb0;
-    d[5] = mb1;
-    d[6] = cb0;
-    d[7] = cb1;
+    d[4] = cb0;
+    d[5] = cb1;
     M.offA = oa;
     M.offB = ob;
 }
 
-// A lane's products over the staged pairs meta[0..n), in order: pair by
-// pair, and inside a pair the k set in maskA[r] & cmaskB[c] from the
+// A lane's products over the staged pairs, in order: pair by pair (the
+// pairs with products for this lane are the set bits of live, taken from the
+// lowest), and inside a pair the k set in maskA[r] & cmaskB[c] from the
 // smallest.  next() yields the A and B value positions of the next product
 // (false when none is left), so a lane's loop runs as many times as it has
 // products, not as many as the busiest lane has in every pair.
 struct ProductWalk {
     const PairMeta *meta;
-    int n, r, c;
-    int j = -1;
-    unsigned hits = 0, ma = 0;
-    int abase = 0;
+    int r, c;
+    unsigned live;
+    unsigned hits = 0, ma = 0, cm = 0;
+    int abase = 0, bbase = 0;
     __device__ __forceinline__ bool next(int &ai, int &bi) {
-        while (hits == 0) {
-            if (++j >= n) return false;
-            const PairMeta &M = meta[j];
+        if (hits == 0) {
+            if (live == 0) return false;
+            const PairMeta &M = meta[__builtin_ctz(live)];
+            live &= live - 1;
             ma = M.maskA[r];
-            hits = ma & M.cmaskB[c];
+            cm = M.cmaskB[c];
+            hits = ma & cm;
             abase = M.offA + M.rowptrA[r];
+            bbase = M.offB + M.colptrB[c];
         }
-        const PairMeta &M = meta[j];
         const int bit = 31 - __clz(hits);          // highest bit: smallest k
-        hits &= ~(1u << bit);
-        const int k = 15 - bit;
+        hits ^= 1u << bit;
         ai = abase + __popc(ma >> (bit + 1));
-        bi = M.offB + M.rowptrB[k] + __popc((unsigned)M.maskB[k] >> (16 - c));
+        bi = bbase + __popc(cm >> (bit + 1));
         return true;
     }
 };
 
-// Accumulates a lane's products over the staged pairs into acc, two at a
-// time (four value loads in flight), in order.
+// Accumulates a lane's products over the staged pairs meta[0..n) into acc,
+// two at a time (four value loads in flight), in order.
 template <bool FUSED>
 __device__ __forceinline__ double walk_products(const TileArgs &T, const PairMeta *meta, int n,
                                                 int r, int c, double acc) {
-    ProductWalk w{meta, n, r, c};
+    unsigned live = 0;
+    for (int j = 0; j < n; j++)
+        if (meta[j].maskA[r] & meta[j].cmaskB[c]) live |= 1u << j;
+    ProductWalk w{meta, r, c, live};
     while (true) {
         int a0 = 0, b0 = 0, a1 = 0, b1 = 0;
         if (!w.next(a0, b0)) break;
@@ -718,12 +769,13 @@ __device__ __forceinline__ double walk_products(const TileArgs &T, const PairMet
 // Work units of <= 64 / TPW outputs, TPW per wave (G = 64 / TPW lanes each);
 // a unit is a tile and its 64-output part, and lane lq owns the part's lq-th
 // output in row-major order.  Pairs are staged CP = 32 / TPW at a time per
-// unit (4352 bytes of LDS a wave).
+// unit (3328 bytes of LDS a wave).
 template <bool FUSED, int TPW>
 __global__ void __launch_bounds__(256) k_step3_out(int nunits, const int64_t *__restrict__ units,
                                                    TileArgs T) {
     constexpr int G = 64 / TPW;
     constexpr int CP = 32 / TPW;
+    static_assert(CP <= 32, "a lane's live pairs are a 32-bit mask");
     __shared__ PairMeta s_meta[4][TPW][CP];
     const int lane = __lane_id();
     const int lq = lane % G, g = lane / G;
@@ -830,10 +882,8 @@ void tile_symbolic(TilePlan &p, const CsrIn &A, const CsrIn &B, bool b_is_a, int
     tm.begin("tile.convert");
     convert(p.A, A, false, ws, s);
     if (b_is_a) {
-        // B shares A's structure; its values get a buffer of their own, used
-        // when a numeric phase passes B values other than A's
+        // B shares A's structure (its column-major values are its own, below)
         p.B = p.A;
-        p.B.val = ws.alloc_n<double>((size_t)B.nnz + 1);
     } else {
         convert(p.B, B, false, ws, s);
     }
@@ -844,10 +894,20 @@ void tile_symbolic(TilePlan &p, const CsrIn &A, const CsrIn &B, bool b_is_a, int
     TB.csc_tile = ws.alloc_n<int>((size_t)TB.numblk + 1);
     csr_transpose(TB.blkm, TB.blkn, TB.tile_ptr, TB.tile_col, nullptr, TB.numblk, TB.csc_ptr,
                   TB.csc_row, nullptr, ws, s, TB.csc_tile);
+    // B's column layout: column masks and starts, and every entry's position
+    // in the column-major payload the numeric phase reads
     TB.cmask = ws.alloc_n<uint16_t>((size_t)TB.numblk * 16 + 16);
+    TB.colptr = ws.alloc_n<uint8_t>((size_t)TB.numblk * 16 + 16);
+    TB.permc = ws.alloc_n<int>((size_t)TB.nnz + 1);
+    TB.valc = ws.alloc_n<double>((size_t)TB.nnz + 1);
     if (TB.numblk > 0) {
-        hipLaunchKernelGGL(k_col_masks, dim3(grid_for((size_t)TB.numblk * 16, 256)), dim3(256), 0,
-                           s, TB.numblk, TB.mask, TB.cmask);
+        int *cpos = ws.alloc_n<int>((size_t)TB.nnz + 1);
+        hipLaunchKernelGGL(k_col_layout, dim3(grid_for((size_t)TB.numblk * 16, 256)), dim3(256),
+                           0, s, TB.numblk, TB.tile_off, TB.mask, TB.rowptr, TB.cmask, TB.colptr,
+                           cpos);
+        TSG_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_compose_perm, dim3(grid_for((size_t)TB.nnz, 256)), dim3(256), 0, s,
+                           TB.nnz, TB.perm, cpos, TB.permc);
         TSG_LAUNCH_CHECK();
     }
     if (b_is_a) {
@@ -921,7 +981,7 @@ void tile_symbolic(TilePlan &p, const CsrIn &A, const CsrIn &B, bool b_is_a, int
     tm.end();
     // nnz(C), the matched-pair total and the bin counts in one round trip
     int64_t h64[2] = {0, 0};
-    int hb[TBINS] = {0, 0, 0, 0};
+    int hb[TBINS] = {0, 0, 0};
     TSG_HIP_TRY(hipMemcpyAsync(&h64[0], d_ic + m, sizeof(int64_t), hipMemcpyDeviceToHost, s));
     TSG_HIP_TRY(hipMemcpyAsync(&h64[1], p.tpair + nC, sizeof(int64_t), hipMemcpyDeviceToHost, s));
     TSG_HIP_TRY(hipMemcpyAsync(hb, bcnt, sizeof(hb), hipMemcpyDeviceToHost, s));
@@ -962,15 +1022,13 @@ template <bool FUSED>
 static void tile_numeric_t(TilePlan &p, const double *valB, const int64_t *d_ic, int *d_jc,
                            double *d_c, hipStream_t s) {
     const TileMat &TA = p.A, &TB = p.B;
-    const TileArgs T{p.trowC,  p.tcolC,    p.maskC,     p.rowptrC,   p.rowoffC, p.tpair,
-                     p.pairs,  TA.tile_off, TA.rowptr,  TA.mask,     TA.val,    TB.tile_off,
-                     TB.rowptr, TB.mask,    TB.cmask,   valB,        d_ic,      d_jc,
-                     d_c};
-    // four waves per block; units per wave by bin (bins 2 and 3 share the
-    // one-unit-per-wave kernel and are launched together)
-    const int tpw[3] = {4, 2, 1};
-    for (int b = 0; b < 3; b++) {
-        const int n = (b < 2 ? p.tbin_start[b + 1] : p.tbin_start[TBINS]) - p.tbin_start[b];
+    const TileArgs T{p.trowC,     p.tcolC,  p.maskC,   p.rowptrC, p.rowoffC, p.tpair,
+                     p.pairs,     TA.tile_off, TA.rowptr, TA.mask, TA.val,   TB.tile_off,
+                     TB.colptr,   TB.cmask, valB,      d_ic,      d_jc,      d_c};
+    // four waves per block; units per wave by bin
+    const int tpw[TBINS] = {4, 2, 1};
+    for (int b = 0; b < TBINS; b++) {
+        const int n = p.tbin_start[b + 1] - p.tbin_start[b];
         if (n == 0) continue;
         const int64_t *units = p.tbin_units + p.tbin_start[b];
         const dim3 grid(grid_for((size_t)n, 4 * tpw[b]));
@@ -996,16 +1054,14 @@ void tile_numeric(TilePlan &p, const CsrIn &A, const CsrIn &B, const int64_t *d_
                            A.nnz, p.A.perm, A.val, p.A.val);
         TSG_LAUNCH_CHECK();
     }
-    // B's values: A's tile payload when B is A (same structure, same value
-    // array), else scattered into B's own payload (with A's structure, by
-    // A's permutation)
-    p.b_val_shared = p.b_is_a && B.val == A.val;
-    if (!p.b_val_shared && B.nnz > 0) {
+    // B's values into its column-major payload (also when B is A: A's
+    // payload is row-major)
+    if (B.nnz > 0) {
         hipLaunchKernelGGL(k_perm_scatter, dim3(grid_for((size_t)B.nnz, 256)), dim3(256), 0, s,
-                           B.nnz, p.B.perm, B.val, p.B.val);
+                           B.nnz, p.B.permc, B.val, p.B.valc);
         TSG_LAUNCH_CHECK();
     }
-    const double *valB = p.b_val_shared ? p.A.val : p.B.val;
+    const double *valB = p.B.valc;
     tm.end();
     tm.begin("tile.step3");
     if (fused)
