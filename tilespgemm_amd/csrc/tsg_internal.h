@@ -180,6 +180,18 @@ void row_numeric(RowPlan &p, const CsrIn &A, const CsrIn &B, const int64_t *d_ic
                  StageTimer &tm, int cus);
 
 // ------------------------------------------------------------ tile path
+// One tile's numeric-phase metadata in 64 bytes (one aligned half line), so
+// a matched pair's two tiles cost two reads.  A tiles: row masks (bit 15-c)
+// and row starts of the row-major payload; B tiles: column masks (bit 15-k)
+// and column starts of the column-major payload.
+struct TileRec {
+    uint16_t mask[16];
+    uint8_t start[16];
+    int off;                     // payload offset of the tile
+    int pad[3];
+};
+static_assert(sizeof(TileRec) == 64, "tile records are 64 bytes");
+
 struct TileMat {
     int rows = 0, cols = 0, blkm = 0, blkn = 0, numblk = 0;
     int *tile_ptr = nullptr;     // [blkm+1]
@@ -194,8 +206,7 @@ struct TileMat {
     int *csc_ptr = nullptr;      // [blkn+1]
     int *csc_row = nullptr;      // [numblk]
     int *csc_tile = nullptr;     // [numblk]
-    uint16_t *cmask = nullptr;   // [16*numblk] column masks, bit 15-k for row k (B only)
-    uint8_t *colptr = nullptr;   // [16*numblk] in-tile column starts, column-major (B only)
+    TileRec *rec = nullptr;      // [numblk] A: row layout; B: column layout
     // entry -> payload position permutation (values refreshed per numeric)
     int *perm = nullptr;         // [nnz] CSR position -> tile payload position
     int *permc = nullptr;        // [nnz] CSR position -> column-major payload position (B only)
@@ -223,7 +234,7 @@ struct TilePlan {
     // numeric work units (tile * 4 + part) by C tile output count: bins 0..3
     // (<= 16, <= 32, <= 64, 64-output parts of larger tiles) back to back,
     // tiles without outputs left out
-    int64_t *tbin_units = nullptr;   // [units]
+    int4 *tbin_units = nullptr;      // [units] {tile, part | pairs << 2, first pair (lo, hi)}
     int tbin_start[5] = {0, 0, 0, 0, 0};
 };
 

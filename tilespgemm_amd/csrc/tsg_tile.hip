@@ -446,13 +446,15 @@ __global__ void __launch_bounds__(64 * S2_WAVES) k_step2(
         int np = 0;
         for_pairs(valid, pa0, pa1, cb0, cb1, tcolA, csc_row, csc_tile, [&](int pa, int pb) {
             np++;
-            unsigned ma = maskA[(size_t)pa * 16 + r];
-            const uint16_t *mb = maskB + (size_t)pb * 16;
-            while (ma) {
-                const int bit = 31 - __clz(ma);   // highest set bit = lowest column
-                ma &= ~(1u << bit);
-                mc |= mb[15 - bit];
-            }
+            // B_KJ's 16 row masks in registers (the group's lanes read the
+            // same 32 bytes), then the rows selected by A_IK's row r
+            const unsigned ma = maskA[(size_t)pa * 16 + r];
+            const uint4 *m4 = reinterpret_cast<const uint4 *>(maskB + (size_t)pb * 16);
+            const uint4 lo = m4[0], hi = m4[1];
+            const unsigned w[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+#pragma unroll
+            for (int k = 0; k < 16; k++)
+                if (ma & (0x8000u >> k)) mc |= (w[k >> 1] >> ((k & 1) * 16)) & 0xffffu;
         }, &rt);
         const int pc = __popc(mc);
         const int incl = group_incl_scan<16>(pc);
@@ -552,15 +554,18 @@ __global__ void __launch_bounds__(256) k_tbin_count(int n, const int64_t *__rest
     if (threadIdx.x < TBINS && s_cnt[threadIdx.x]) atomicAdd(&cnt[threadIdx.x], s_cnt[threadIdx.x]);
 }
 
+// A unit's record carries its tile's pairs (first one, count), so the
+// numeric kernel's first loads do not wait on each other.
 // Slots come from a wave scan of the unit counts per bin, one LDS atomic
 // per wave and bin, and one global atomic per block and bin (a global
 // atomic per tile on the counters serialised the scatter).  Bin 2 takes a
 // tile's parts 0, 1, ... (the remainder last, when it is there); bins 0 and
 // 1 take only a remainder, part n / 64.
 __global__ void __launch_bounds__(256) k_tbin_scatter(int n, const int64_t *__restrict__ off,
+                                                      const int64_t *__restrict__ tpair,
                                                       const int *__restrict__ cnt,
                                                       int *__restrict__ fill,
-                                                      int64_t *__restrict__ units) {
+                                                      int4 *__restrict__ units) {
     __shared__ int s_cnt[TBINS], s_base[TBINS];
     if (threadIdx.x < TBINS) s_cnt[threadIdx.x] = 0;
     __syncthreads();
@@ -589,23 +594,29 @@ __global__ void __launch_bounds__(256) k_tbin_scatter(int n, const int64_t *__re
     const int full = (int)(nz >> 6);
     const int rb = rem_bin(nz);
     const int u2 = tile_units_in(nz, 2);
-    for (int i = 0; i < u2; i++) units[s_base[2] + pos[2] + i] = (int64_t)t * 4 + i;
-    if (rb == 0 || rb == 1) units[s_base[rb] + pos[rb]] = (int64_t)t * 4 + full;
+    if (u2 == 0 && rb < 0) return;
+    const int64_t p0 = tpair[t];
+    const int np = (int)(tpair[t + 1] - p0);
+    auto rec = [&](int part) {
+        return make_int4(t, part | (np << 2), (int)(uint32_t)(uint64_t)p0,
+                         (int)(uint32_t)((uint64_t)p0 >> 32));
+    };
+    for (int i = 0; i < u2; i++) units[s_base[2] + pos[2] + i] = rec(i);
+    if (rb == 0 || rb == 1) units[s_base[rb] + pos[rb]] = rec(full);
 }
 
 // Column layout of 16x16 tiles (B only), the transposed sub-tiles of
-// main_spgemm-col.cu:5255-5290: cmask[16 t + c] has bit 15 - k set when the
-// tile holds entry (k, c) -- the transpose of the row masks, in the same bit
-// order; colptr[16 t + c] counts the tile's entries in columns before c; and
-// cpos[q] is the position of row-major payload entry q in the tile's
+// main_spgemm-col.cu:5255-5290, as tile records: mask[c] has bit 15 - k set
+// when the tile holds entry (k, c) -- the transpose of the row masks, in the
+// same bit order; start[c] counts the tile's entries in columns before c;
+// and cpos[q] is the position of row-major payload entry q in the tile's
 // column-major payload (columns ascending, rows ascending in a column).
 // 16 lanes per tile, lane c builds column c (a group shares its tile, so
 // whole groups leave together).
 __global__ void __launch_bounds__(256) k_col_layout(int numblk, const int *__restrict__ tile_off,
                                                     const uint16_t *__restrict__ mask,
                                                     const uint8_t *__restrict__ rowptr,
-                                                    uint16_t *__restrict__ cmask,
-                                                    uint8_t *__restrict__ colptr,
+                                                    TileRec *__restrict__ rec,
                                                     int *__restrict__ cpos) {
     const int64_t t = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
     const int c = threadIdx.x & 15;
@@ -621,10 +632,11 @@ __global__ void __launch_bounds__(256) k_col_layout(int numblk, const int *__res
     }
     const int cnt = __popc(cm);
     const int cstart = group_incl_scan<16>(cnt) - cnt;
-    cmask[t * 16 + c] = (uint16_t)cm;
-    colptr[t * 16 + c] = (uint8_t)cstart;
-    const uint4 rp = *reinterpret_cast<const uint4 *>(rowptr + t * 16);
     const int off = tile_off[t];
+    rec[t].mask[c] = (uint16_t)cm;
+    rec[t].start[c] = (uint8_t)cstart;
+    if (c == 0) rec[t].off = off;
+    const uint4 rp = *reinterpret_cast<const uint4 *>(rowptr + t * 16);
     int j = 0;
 #pragma unroll
     for (int k = 0; k < 16; k++) {
@@ -636,6 +648,25 @@ __global__ void __launch_bounds__(256) k_col_layout(int numblk, const int *__res
             j++;
         }
     }
+}
+
+// Row layout of 16x16 tiles as tile records (A): row masks, row starts and
+// payload offset, four lanes per tile writing 16 bytes each.
+__global__ void __launch_bounds__(256) k_row_records(int numblk, const int *__restrict__ tile_off,
+                                                     const uint16_t *__restrict__ mask,
+                                                     const uint8_t *__restrict__ rowptr,
+                                                     TileRec *__restrict__ rec) {
+    const int64_t t = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 2;
+    const int j = threadIdx.x & 3;
+    if (t >= numblk) return;
+    uint4 v;
+    if (j < 2)
+        v = reinterpret_cast<const uint4 *>(mask + t * 16)[j];
+    else if (j == 2)
+        v = *reinterpret_cast<const uint4 *>(rowptr + t * 16);
+    else
+        v = make_uint4((unsigned)tile_off[t], 0u, 0u, 0u);
+    reinterpret_cast<uint4 *>(rec + t)[j] = v;
 }
 
 // permc[i] = cpos[perm[i]]: CSR position -> column-major payload position.
@@ -665,15 +696,10 @@ struct TileArgs {
     const uint16_t *maskC;
     const uint8_t *rowptrC;
     const int *rowoffC;
-    const int64_t *tpair;
     const int2 *pairs;
-    const int *offA;
-    const uint8_t *rowptrA;
-    const uint16_t *maskA;
+    const TileRec *recA;         // row layout
     const double *valA;
-    const int *offB;
-    const uint8_t *colptrB;
-    const uint16_t *cmaskB;
+    const TileRec *recB;         // column layout
     const double *valB;          // column-major inside each tile
     const int64_t *ic;
     int *jc;
@@ -685,32 +711,27 @@ __device__ __forceinline__ unsigned byte_at(const uint4 &v, int i) {
     return (w >> ((i & 3) * 8)) & 255u;
 }
 
+// A matched pair's two tile records, as staged in LDS.
 struct PairMeta {
-    uint8_t rowptrA[16], colptrB[16];   // row starts of A, column starts of B (bytes 0..31)
-    uint16_t maskA[16];                 // row masks of A (bytes 32..63)
-    uint16_t cmaskB[16];                // column masks of B (bytes 64..95)
-    int offA, offB;                     // payload offsets (bytes 96..103)
+    TileRec a, b;
 };
 
-// Stage pair p's metadata (called by one lane per pair): seven loads, one
-// level deep after the pair itself.
-__device__ __forceinline__ void stage_pair(const TileArgs &T, int64_t p, PairMeta &M) {
-    const int2 pr = T.pairs[p];
-    const int oa = T.offA[pr.x], ob = T.offB[pr.y];
-    const uint4 ra = *reinterpret_cast<const uint4 *>(T.rowptrA + (size_t)pr.x * 16);
-    const uint4 cpb = *reinterpret_cast<const uint4 *>(T.colptrB + (size_t)pr.y * 16);
-    const uint4 *ma = reinterpret_cast<const uint4 *>(T.maskA + (size_t)pr.x * 16);
-    const uint4 *cb = reinterpret_cast<const uint4 *>(T.cmaskB + (size_t)pr.y * 16);
-    const uint4 ma0 = ma[0], ma1 = ma[1], cb0 = cb[0], cb1 = cb[1];
+// Stage pair pr's metadata (called by one lane per pair): two 64-byte
+// records.
+__device__ __forceinline__ void stage_pair(const TileArgs &T, int2 pr, PairMeta &M) {
+    const uint4 *ra = reinterpret_cast<const uint4 *>(T.recA + pr.x);
+    const uint4 *rb = reinterpret_cast<const uint4 *>(T.recB + pr.y);
+    const uint4 a0 = ra[0], a1 = ra[1], a2 = ra[2], a3 = ra[3];
+    const uint4 b0 = rb[0], b1 = rb[1], b2 = rb[2], b3 = rb[3];
     uint4 *d = reinterpret_cast<uint4 *>(&M);
-    d[0] = ra;
-    d[1] = cpb;
-    d[2] = ma0;
-    d[3] = ma1;
-    d[4] = cb0;
-    d[5] = cb1;
-    M.offA = oa;
-    M.offB = ob;
+    d[0] = a0;
+    d[1] = a1;
+    d[2] = a2;
+    d[3] = a3;
+    d[4] = b0;
+    d[5] = b1;
+    d[6] = b2;
+    d[7] = b3;
 }
 
 // A lane's products over the staged pairs, in order: pair by pair (the
@@ -730,11 +751,11 @@ struct ProductWalk {
             if (live == 0) return false;
             const PairMeta &M = meta[__builtin_ctz(live)];
             live &= live - 1;
-            ma = M.maskA[r];
-            cm = M.cmaskB[c];
+            ma = M.a.mask[r];
+            cm = M.b.mask[c];
             hits = ma & cm;
-            abase = M.offA + M.rowptrA[r];
-            bbase = M.offB + M.colptrB[c];
+            abase = M.a.off + M.a.start[r];
+            bbase = M.b.off + M.b.start[c];
         }
         const int bit = 31 - __clz(hits);          // highest bit: smallest k
         hits ^= 1u << bit;
@@ -745,33 +766,55 @@ struct ProductWalk {
 };
 
 // Accumulates a lane's products over the staged pairs meta[0..n) into acc,
-// two at a time (four value loads in flight), in order.
+// in order, STEP_U at a time: the positions of the next STEP_U products come
+// from the walk (LDS and bit operations only), their 2 STEP_U value loads
+// are issued together, then the FMAs run in order (positions past the
+// lane's last product repeat the first and are not accumulated).
+constexpr int STEP_U = 4;
+
 template <bool FUSED>
 __device__ __forceinline__ double walk_products(const TileArgs &T, const PairMeta *meta, int n,
                                                 int r, int c, double acc) {
     unsigned live = 0;
     for (int j = 0; j < n; j++)
-        if (meta[j].maskA[r] & meta[j].cmaskB[c]) live |= 1u << j;
+        if (meta[j].a.mask[r] & meta[j].b.mask[c]) live |= 1u << j;
     ProductWalk w{meta, r, c, live};
     while (true) {
-        int a0 = 0, b0 = 0, a1 = 0, b1 = 0;
-        if (!w.next(a0, b0)) break;
-        const bool two = w.next(a1, b1);
-        const double x0 = T.valA[a0], y0 = T.valB[b0];
-        const double x1 = two ? T.valA[a1] : 0.0, y1 = two ? T.valB[b1] : 0.0;
-        acc = acc_step<FUSED>(acc, x0, y0);
-        if (!two) break;
-        acc = acc_step<FUSED>(acc, x1, y1);
+        int a[STEP_U], b[STEP_U];
+        int cnt = 0;
+#pragma unroll
+        for (int u = 0; u < STEP_U; u++) {
+            a[u] = u ? a[0] : 0;
+            b[u] = u ? b[0] : 0;
+            cnt += w.next(a[u], b[u]) ? 1 : 0;
+        }
+        if (cnt == 0) break;
+        double x[STEP_U], y[STEP_U];
+#pragma unroll
+        for (int u = 0; u < STEP_U; u++) {
+            x[u] = T.valA[a[u]];
+            y[u] = T.valB[b[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < STEP_U; u++)
+            if (u < cnt) acc = acc_step<FUSED>(acc, x[u], y[u]);
+        if (cnt < STEP_U) break;
     }
     return acc;
 }
 
-// Work units of <= 64 / TPW outputs, TPW per wave (G = 64 / TPW lanes each);
-// a unit is a tile and its 64-output part, and lane lq owns the part's lq-th
-// output in row-major order.  Pairs are staged CP = 32 / TPW at a time per
-// unit (3328 bytes of LDS a wave).
+// Work units of <= 64 / TPW outputs, TPW at a time per wave (G = 64 / TPW
+// lanes each), UPW rounds per wave over consecutive units; a unit is a tile
+// and its 64-output part, and lane lq owns the part's lq-th output in
+// row-major order.  Pairs are staged CP = 32 / TPW at a time per unit (3328
+// bytes of LDS a wave).  The loads that need only the unit (the tile's row
+// starts and masks, its tile row and column, the first chunk of pairs) are
+// issued together, and the output's position in C right after the lane's
+// row is known, so they overlap the staging.
+constexpr int UPW = 1;
+
 template <bool FUSED, int TPW>
-__global__ void __launch_bounds__(256) k_step3_out(int nunits, const int64_t *__restrict__ units,
+__global__ void __launch_bounds__(256) k_step3_out(int nunits, const int4 *__restrict__ units,
                                                    TileArgs T) {
     constexpr int G = 64 / TPW;
     constexpr int CP = 32 / TPW;
@@ -780,48 +823,60 @@ __global__ void __launch_bounds__(256) k_step3_out(int nunits, const int64_t *__
     const int lane = __lane_id();
     const int lq = lane % G, g = lane / G;
     PairMeta *meta = s_meta[threadIdx.x >> 6][g];
-    const int64_t slot = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * TPW + g;
-    if (slot >= nunits) return;
-    const int64_t unit = units[slot];
-    const int t = (int)(unit >> 2);
-    const int q = (int)(unit & 3) * 64 + lq;     // the tile's output this lane owns
-    // row of output q: the last row whose first output is <= q (rows hold
-    // their tile-local start in rowptrC), then its (q - start)-th column;
-    // lanes past the tile's outputs stage pairs but own no output
-    const uint4 rp = *reinterpret_cast<const uint4 *>(T.rowptrC + (size_t)t * 16);
-    int r = 0, start = 0;
+    const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    for (int round = 0; round < UPW; round++) {
+        const int64_t slot = (wave * UPW + round) * TPW + g;
+        if (slot >= nunits) break;
+        const int4 u = units[slot];
+        const int t = u.x;
+        const int q = (u.y & 3) * 64 + lq;       // the tile's output this lane owns
+        const int np = (int)((unsigned)u.y >> 2);
+        const int64_t p0 = (int64_t)(((uint64_t)(uint32_t)u.w << 32) | (uint32_t)u.z);
+        const uint4 rp = *reinterpret_cast<const uint4 *>(T.rowptrC + (size_t)t * 16);
+        const uint4 mk0 = *reinterpret_cast<const uint4 *>(T.maskC + (size_t)t * 16);
+        const uint4 mk1 = *reinterpret_cast<const uint4 *>(T.maskC + (size_t)t * 16 + 8);
+        const int I = T.trowC[t], J = T.tcolC[t];
+        int2 pr0 = make_int2(0, 0);
+        if (lq < min(CP, np)) pr0 = T.pairs[p0 + lq];
+        // row of output q: the last row whose first output is <= q (rows hold
+        // their tile-local start in rowptrC), then its (q - start)-th column;
+        // lanes past the unit's outputs stage pairs but own no output
+        int r = 0, start = 0;
 #pragma unroll
-    for (int i = 1; i < 16; i++) {
-        const int st = (int)byte_at(rp, i);
-        if (st <= q) {
-            r = i;
-            start = st;
+        for (int i = 1; i < 16; i++) {
+            const int st = (int)byte_at(rp, i);
+            if (st <= q) {
+                r = i;
+                start = st;
+            }
+        }
+        // rows after r with the same start are empty, so r holds q exactly
+        // when q is below r's end
+        const unsigned mw = r < 8 ? (r < 4 ? (r < 2 ? mk0.x : mk0.y) : (r < 6 ? mk0.z : mk0.w))
+                                  : (r < 12 ? (r < 10 ? mk1.x : mk1.y) : (r < 14 ? mk1.z : mk1.w));
+        const unsigned mrow = (mw >> ((r & 1) * 16)) & 0xffffu;
+        const bool owner = q - start < __popc(mrow);
+        int64_t o = 0;
+        if (owner) o = T.ic[(int64_t)I * 16 + r] + T.rowoffC[(size_t)t * 16 + r] + (q - start);
+        int c = 0;
+        if (owner) {
+            unsigned mm = mrow;
+            for (int j = 0; j < q - start; j++) mm &= ~(1u << (31 - __clz(mm)));
+            c = 15 - (31 - __clz(mm));
+        }
+        double acc = 0.0;
+        for (int pc = 0; pc < np; pc += CP) {
+            const int n = min(CP, np - pc);
+            wave_lds_fence();
+            if (lq < n) stage_pair(T, pc == 0 ? pr0 : T.pairs[p0 + pc + lq], meta[lq]);
+            wave_lds_fence();
+            if (owner) acc = walk_products<FUSED>(T, meta, n, r, c, acc);
+        }
+        if (owner) {
+            T.jc[o] = J * 16 + c;
+            T.c[o] = acc;
         }
     }
-    // rows after r with the same start are empty, so r holds q exactly when
-    // q is below r's end
-    const unsigned mrow = T.maskC[(size_t)t * 16 + r];
-    const bool owner = q - start < __popc(mrow);
-    int c = 0;
-    if (owner) {
-        unsigned mm = mrow;
-        for (int j = 0; j < q - start; j++) mm &= ~(1u << (31 - __clz(mm)));
-        c = 15 - (31 - __clz(mm));
-    }
-    double acc = 0.0;
-    const int64_t p0 = T.tpair[t], p1 = T.tpair[t + 1];
-    for (int64_t pc = p0; pc < p1; pc += CP) {
-        const int n = (int)min((int64_t)CP, p1 - pc);
-        wave_lds_fence();
-        if (lq < n) stage_pair(T, pc + lq, meta[lq]);
-        wave_lds_fence();
-        if (owner) acc = walk_products<FUSED>(T, meta, n, r, c, acc);
-    }
-    if (!owner) return;
-    const int I = T.trowC[t], J = T.tcolC[t];
-    const int64_t o = T.ic[I * 16 + r] + T.rowoffC[(size_t)t * 16 + r] + (q - start);
-    T.jc[o] = J * 16 + c;
-    T.c[o] = acc;
 }
 
 void convert(TileMat &T, const CsrIn &X, bool fill_values, Arena &ws, hipStream_t s) {
@@ -896,25 +951,29 @@ void tile_symbolic(TilePlan &p, const CsrIn &A, const CsrIn &B, bool b_is_a, int
                   TB.csc_row, nullptr, ws, s, TB.csc_tile);
     // B's column layout: column masks and starts, and every entry's position
     // in the column-major payload the numeric phase reads
-    TB.cmask = ws.alloc_n<uint16_t>((size_t)TB.numblk * 16 + 16);
-    TB.colptr = ws.alloc_n<uint8_t>((size_t)TB.numblk * 16 + 16);
+    TB.rec = ws.alloc_n<TileRec>((size_t)TB.numblk + 1);
     TB.permc = ws.alloc_n<int>((size_t)TB.nnz + 1);
     TB.valc = ws.alloc_n<double>((size_t)TB.nnz + 1);
     if (TB.numblk > 0) {
         int *cpos = ws.alloc_n<int>((size_t)TB.nnz + 1);
         hipLaunchKernelGGL(k_col_layout, dim3(grid_for((size_t)TB.numblk * 16, 256)), dim3(256),
-                           0, s, TB.numblk, TB.tile_off, TB.mask, TB.rowptr, TB.cmask, TB.colptr,
-                           cpos);
+                           0, s, TB.numblk, TB.tile_off, TB.mask, TB.rowptr, TB.rec, cpos);
         TSG_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_compose_perm, dim3(grid_for((size_t)TB.nnz, 256)), dim3(256), 0, s,
                            TB.nnz, TB.perm, cpos, TB.permc);
+        TSG_LAUNCH_CHECK();
+    }
+    // A's row layout records
+    p.A.rec = ws.alloc_n<TileRec>((size_t)p.A.numblk + 1);
+    if (p.A.numblk > 0) {
+        hipLaunchKernelGGL(k_row_records, dim3(grid_for((size_t)p.A.numblk * 4, 256)), dim3(256),
+                           0, s, p.A.numblk, p.A.tile_off, p.A.mask, p.A.rowptr, p.A.rec);
         TSG_LAUNCH_CHECK();
     }
     if (b_is_a) {
         p.A.csc_ptr = TB.csc_ptr;
         p.A.csc_row = TB.csc_row;
         p.A.csc_tile = TB.csc_tile;
-        p.A.cmask = TB.cmask;
     }
     tm.end();
 
@@ -989,11 +1048,11 @@ void tile_symbolic(TilePlan &p, const CsrIn &A, const CsrIn &B, bool b_is_a, int
     // the numeric phase's work units, sized by the counts just read
     int nunits = 0;
     for (int b = 0; b < TBINS; b++) nunits += hb[b];
-    p.tbin_units = ws.alloc_n<int64_t>((size_t)nunits + 1);
+    p.tbin_units = ws.alloc_n<int4>((size_t)nunits + 1);
     tm.begin("tile.pairs");
     if (nunits > 0) {
         hipLaunchKernelGGL(k_tbin_scatter, dim3(grid_for((size_t)nC, 256)), dim3(256), 0, s, nC,
-                           p.offC, bcnt, bcnt + TBINS, p.tbin_units);
+                           p.offC, p.tpair, bcnt, bcnt + TBINS, p.tbin_units);
         TSG_LAUNCH_CHECK();
     }
     // the matched pairs themselves (the numeric phase reads them instead of
@@ -1022,16 +1081,15 @@ template <bool FUSED>
 static void tile_numeric_t(TilePlan &p, const double *valB, const int64_t *d_ic, int *d_jc,
                            double *d_c, hipStream_t s) {
     const TileMat &TA = p.A, &TB = p.B;
-    const TileArgs T{p.trowC,     p.tcolC,  p.maskC,   p.rowptrC, p.rowoffC, p.tpair,
-                     p.pairs,     TA.tile_off, TA.rowptr, TA.mask, TA.val,   TB.tile_off,
-                     TB.colptr,   TB.cmask, valB,      d_ic,      d_jc,      d_c};
+    const TileArgs T{p.trowC, p.tcolC, p.maskC, p.rowptrC, p.rowoffC, p.pairs, TA.rec,
+                     TA.val,  TB.rec,  valB,    d_ic,      d_jc,      d_c};
     // four waves per block; units per wave by bin
     const int tpw[TBINS] = {4, 2, 1};
     for (int b = 0; b < TBINS; b++) {
         const int n = p.tbin_start[b + 1] - p.tbin_start[b];
         if (n == 0) continue;
-        const int64_t *units = p.tbin_units + p.tbin_start[b];
-        const dim3 grid(grid_for((size_t)n, 4 * tpw[b]));
+        const int4 *units = p.tbin_units + p.tbin_start[b];
+        const dim3 grid(grid_for((size_t)n, 4 * tpw[b] * UPW));
         if (b == 0)
             hipLaunchKernelGGL((k_step3_out<FUSED, 4>), grid, dim3(256), 0, s, n, units, T);
         else if (b == 1)
