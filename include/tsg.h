@@ -62,8 +62,10 @@ typedef struct tsg_handle_s *tsg_handle_t;
  * trans, sort and nzmax are accepted and ignored, as in the reference.
  * Host pointers in, host pointers out; the device work runs on device 0 (or
  * $TSG_DEVICE).  The symbolic plan of the request=1 call is cached and reused
- * by the request=0 call when (m, n, k, ia, ja, ib, jb) match; otherwise the
- * symbolic phase is recomputed.
+ * by the request=0 call when (m, n, k, ia, ja, ib, jb) match and a checksum
+ * of the index arrays' contents (row pointers and columns of A and B) is
+ * unchanged, so arrays rewritten in place between the two calls are seen;
+ * otherwise the symbolic phase is recomputed.
  * The request=0 call keeps C whole on the device when it fits, else it
  * streams C in row slabs (tsg_set_host_slab_entries): each slab of
  * consecutive rows of A is a product of its own (symbolic, numeric, copy of
