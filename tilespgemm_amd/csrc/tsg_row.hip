@@ -821,7 +821,8 @@ __global__ void __launch_bounds__(256) k_items(int nbig, const int *__restrict__
                                                const int64_t *__restrict__ ic,
                                                int64_t *__restrict__ item_cnt,
                                                int4 *__restrict__ items,
-                                               int64_t *__restrict__ item_off) {
+                                               int64_t *__restrict__ item_off,
+                                               int *__restrict__ item_n) {
     static_assert(WSB <= (1 << (ITEM_WBITS_SHIFT - SUB_SHIFT)), "item wider than its LDS");
     const int r = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int l = threadIdx.x & 63;
@@ -871,6 +872,7 @@ __global__ void __launch_bounds__(256) k_items(int nbig, const int *__restrict__
                     items[w] = make_int4(row, (cbase + start) << SUB_SHIFT,
                                          min(k, (cbase + q) << SUB_SHIFT), r);
                     item_off[w] = out;
+                    item_n[w] = cnt;
                     w++;
                     out += cnt;
                     start = -1;
@@ -885,6 +887,7 @@ __global__ void __launch_bounds__(256) k_items(int nbig, const int *__restrict__
                 items[w] = make_int4(row, (cbase + start) << SUB_SHIFT,
                                      min(k, (cbase + len) << SUB_SHIFT), r);
                 item_off[w] = out;
+                item_n[w] = cnt;
             }
         }
         wbase += group_last<64>(it_incl);
@@ -898,10 +901,26 @@ __global__ void __launch_bounds__(256) k_items(int nbig, const int *__restrict__
 // items in this order, so the items in flight at any time gather B entries
 // of one narrow column band, which the L2 caches keep, instead of bands
 // spread over all of B.
+// Items of at most SMALL_ITEM outputs sort first (the numeric phase runs
+// them in a kernel with a quarter of the LDS, twice the items resident),
+// each class in first-column order: key = class << class_bit | first
+// sub-block (class_bit < 0: no classes).  block_small[b] counts block b's
+// items of the first class.
+constexpr int SMALL_ITEM = 256;
+constexpr int SMALL_RANK_U = RANK_U;
+
 __global__ void __launch_bounds__(256) k_item_keys(int64_t n, const int4 *__restrict__ items,
-                                                   int *__restrict__ keys) {
+                                                   const int *__restrict__ item_n, int class_bit,
+                                                   int *__restrict__ keys,
+                                                   int *__restrict__ block_small) {
+    __shared__ int s_cnt[4];
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) keys[i] = items[i].y >> SUB_SHIFT;
+    const bool sm = i < n && class_bit >= 0 && item_n[i] <= SMALL_ITEM;
+    if (i < n) keys[i] = (sm || class_bit < 0 ? 0 : (1 << class_bit)) | (items[i].y >> SUB_SHIFT);
+    const unsigned long long b = __ballot(sm);
+    if (__lane_id() == 0) s_cnt[threadIdx.x >> 6] = __popcll(b);
+    __syncthreads();
+    if (threadIdx.x == 0) block_small[blockIdx.x] = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
 }
 
 __global__ void __launch_bounds__(256) k_gather_items(int64_t n, const int *__restrict__ order,
@@ -1005,10 +1024,10 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric(
 // dense rank-ordered array that is written out coalesced.
 // WORDS: bitmap words of the widest item (256 or 512: items of 8192 or
 // 16384 columns).
-template <int WORDS>
+template <int WORDS, int CAP = ITEM_CAP>
 struct RankSmem {
     uint2 words[WORDS];           // {bitmap word, outputs before it}
-    double vals[ITEM_CAP];
+    double vals[CAP];
     unsigned tags[ITEM_TAGS];     // bucketed by rank
     int mark[64];                 // segment marks of the product expansion
 };
@@ -1033,8 +1052,8 @@ __device__ __forceinline__ void item_words(const unsigned *__restrict__ src, int
     }
 }
 
-template <int WORDS, int WCH = WORDS / 64>
-__device__ __forceinline__ int item_rank_build(RankSmem<WORDS> &S, unsigned (&wv)[WCH], int c0,
+template <int WORDS, int CAP, int WCH = WORDS / 64>
+__device__ __forceinline__ int item_rank_build(RankSmem<WORDS, CAP> &S, unsigned (&wv)[WCH], int c0,
                                                int c1, int64_t o, int *__restrict__ jc) {
     const int l = __lane_id();
     const int nwords = (c1 - c0 + 31) >> 5;
@@ -1073,8 +1092,8 @@ __device__ __forceinline__ int item_rank_build(RankSmem<WORDS> &S, unsigned (&wv
 }
 
 // Writes an item's cnt values to C at offset o, straight from LDS.
-template <int WORDS>
-__device__ __forceinline__ void item_store_values(RankSmem<WORDS> &S, int cnt, int64_t o,
+template <int WORDS, int CAP>
+__device__ __forceinline__ void item_store_values(RankSmem<WORDS, CAP> &S, int cnt, int64_t o,
                                                   double *__restrict__ c) {
     const int l = __lane_id();
     wave_lds_fence();
@@ -1085,8 +1104,8 @@ __device__ __forceinline__ void item_store_values(RankSmem<WORDS> &S, int cnt, i
 // Accumulates one batch of products into the item's rank-ordered values:
 // a batch from one A entry (distinct columns when B's rows are strictly
 // ascending) adds directly, any other batch by tag rounds in lane order.
-template <bool FUSED, int WORDS>
-__device__ __forceinline__ void item_accumulate(RankSmem<WORDS> &S, int c0, bool act, int col,
+template <bool FUSED, int WORDS, int CAP>
+__device__ __forceinline__ void item_accumulate(RankSmem<WORDS, CAP> &S, int c0, bool act, int col,
                                                 double av, double bv, bool single, int b_strict) {
     int rank = -1;
     if (act) {
@@ -1103,7 +1122,7 @@ __device__ __forceinline__ void item_accumulate(RankSmem<WORDS> &S, int c0, bool
                                          (unsigned)__lane_id(), false);
 }
 
-template <bool FUSED, int WORDS>
+template <bool FUSED, int WORDS, int CAP = ITEM_CAP, int U = RANK_U>
 __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric_rank(
     int64_t nitems, const int4 *__restrict__ items, const int64_t *__restrict__ item_off,
     const unsigned *__restrict__ rowbits, const int *__restrict__ span_sb0,
@@ -1112,7 +1131,7 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric_rank(
     const int4 *__restrict__ bpk, BlockPtrs bp, int *__restrict__ jc, double *__restrict__ c,
     int b_strict) {
     extern __shared__ __align__(16) unsigned char smem[];
-    RankSmem<WORDS> &S = reinterpret_cast<RankSmem<WORDS> *>(smem)[threadIdx.x >> 6];
+    RankSmem<WORDS, CAP> &S = reinterpret_cast<RankSmem<WORDS, CAP> *>(smem)[threadIdx.x >> 6];
     const int l = __lane_id();
     for (int i = l; i < ITEM_TAGS; i += 64) S.tags[i] = 0xffffffffu;
     S.mark[l] = -1;
@@ -1170,9 +1189,9 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric_rank(
         item_words(isrc, c0, c1, wv);
         int lo, hi;
         load_range(t0, lo, hi);
-        const int cnt = item_rank_build(S, wv, c0, c1, o, jc);
+        const int cnt = item_rank_build<WORDS, CAP>(S, wv, c0, c1, o, jc);
         auto f = [&](bool act, int col, double av, double bv, bool single) {
-            item_accumulate<FUSED, WORDS>(S, c0, act, col, av, bv, single, b_strict);
+            item_accumulate<FUSED, WORDS, CAP>(S, c0, act, col, av, bv, single, b_strict);
         };
         for (int g = ipa; g < ipe; g += 64) {
             const double ag = a0;
@@ -1180,7 +1199,7 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric_rank(
             double a2;
             load_range(t1, lo_n, hi_n);
             load_t(g + 128, t2, a2);
-            expand_group<64, RANK_U, true, true>(lo, hi - lo, ag, jb, b, S.mark, f, 0, 1, bpk);
+            expand_group<64, U, true, true>(lo, hi - lo, ag, jb, b, S.mark, f, 0, 1, bpk);
             lo = lo_n;
             hi = hi_n;
             t1 = t2;
@@ -1473,16 +1492,16 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
         // windows otherwise (fewer items, fewer (A entry, item) pairs).
         const bool narrow = p.column_order;
         p.narrow_items = narrow;
-        auto items_kernel = [&](auto kern, int64_t *cnt, int4 *it, int64_t *off) {
+        auto items_kernel = [&](auto kern, int64_t *cnt, int4 *it, int64_t *off, int *itn) {
             hipLaunchKernelGGL(kern, dim3(grid_for((size_t)p.nbig * 64, 256)), dim3(256), 0, s,
                                p.nbig, rowsb, k, p.span_sb0, p.sb_off, p.subcnt, d_ic, cnt, it,
-                               off);
+                               off, itn);
             TSG_LAUNCH_CHECK();
         };
         if (narrow)
-            items_kernel(k_items<false, 8>, p.item_cnt, nullptr, nullptr);
+            items_kernel(k_items<false, 8>, p.item_cnt, nullptr, nullptr, nullptr);
         else
-            items_kernel(k_items<false, 16>, p.item_cnt, nullptr, nullptr);
+            items_kernel(k_items<false, 16>, p.item_cnt, nullptr, nullptr, nullptr);
         exclusive_scan_i64(p.item_cnt, p.item_cnt, (size_t)p.nbig, tmp2, s);
         TSG_HIP_TRY(hipMemcpyAsync(&h_tot[1], p.item_cnt + p.nbig, sizeof(int64_t),
                                    hipMemcpyDeviceToHost, s));
@@ -1491,20 +1510,28 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
         p.nitems = h_tot[1];
         p.items = ws.alloc_n<int4>((size_t)p.nitems + 1);
         p.item_off = ws.alloc_n<int64_t>((size_t)p.nitems + 1);
+        int *item_n = ws.alloc_n<int>((size_t)p.nitems + 1);
         if (narrow)
-            items_kernel(k_items<true, 8>, p.item_cnt, p.items, p.item_off);
+            items_kernel(k_items<true, 8>, p.item_cnt, p.items, p.item_off, item_n);
         else
-            items_kernel(k_items<true, 16>, p.item_cnt, p.items, p.item_off);
+            items_kernel(k_items<true, 16>, p.item_cnt, p.items, p.item_off, item_n);
+        p.nitems_small = 0;
         if (p.column_order && p.nitems > 1 && p.nitems < (int64_t(1) << 31)) {
             const size_t ni = (size_t)p.nitems;
             int *keys = ws.alloc_n<int>(ni);
             int *order = ws.alloc_n<int>(ni);
-            hipLaunchKernelGGL(k_item_keys, dim3(grid_for(ni, 256)), dim3(256), 0, s, p.nitems,
-                               p.items, keys);
-            TSG_LAUNCH_CHECK();
+            const unsigned kblocks = grid_for(ni, 256);
+            int *bsmall = ws.alloc_n<int>((size_t)kblocks + 1);
+            int64_t *tmpk = ws.alloc_n<int64_t>(scan_tmp_elems((size_t)kblocks + 1));
+            int64_t *d_small = ws.alloc_n<int64_t>(1);
             int end_bit = 1;
-            while (end_bit < 31 && (int64_t(1) << end_bit) <= ((int64_t)k >> SUB_SHIFT)) end_bit++;
-            sort_order_i32(keys, p.nitems, end_bit, order, ws, s);
+            while (end_bit < 30 && (int64_t(1) << end_bit) <= ((int64_t)k >> SUB_SHIFT)) end_bit++;
+            // the size class splits the rank path's items only
+            hipLaunchKernelGGL(k_item_keys, dim3(kblocks), dim3(256), 0, s, p.nitems,
+                               p.items, item_n, p.use_rank ? end_bit : -1, keys, bsmall);
+            TSG_LAUNCH_CHECK();
+            exclusive_scan_i32(bsmall, bsmall, (size_t)kblocks, tmpk, s, d_small);
+            sort_order_i32(keys, p.nitems, p.use_rank ? end_bit + 1 : end_bit, order, ws, s);
             int4 *items_s = ws.alloc_n<int4>(ni + 1);
             int64_t *off_s = ws.alloc_n<int64_t>(ni + 1);
             hipLaunchKernelGGL(k_gather_items, dim3(grid_for(ni, 256)), dim3(256), 0, s,
@@ -1512,6 +1539,13 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
             TSG_LAUNCH_CHECK();
             p.items = items_s;
             p.item_off = off_s;
+            if (p.use_rank) {
+                int64_t h_small = 0;
+                TSG_HIP_TRY(hipMemcpyAsync(&h_small, d_small, sizeof(h_small),
+                                           hipMemcpyDeviceToHost, s));
+                TSG_HIP_TRY(hipStreamSynchronize(s));
+                p.nitems_small = h_small;
+            }
         }
         tm.end();
     } else {
@@ -1575,21 +1609,29 @@ static void row_numeric_t(RowPlan &p, const CsrIn &A, const CsrIn &B, const int6
                                B.nnz, B.idx, B.val, p.bpk);
             TSG_LAUNCH_CHECK();
         }
-        auto launch_rank = [&](auto kern, size_t sm) {
+        auto launch_rank = [&](auto kern, size_t sm, int64_t i0, int64_t i1) {
+            if (i1 <= i0) return;
             set_smem(kern, sm);
-            const unsigned grid = item_grid(p.nitems);
-            hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * ITEM_WAVES), sm, s, p.nitems, p.items,
-                               p.item_off, p.rowbits, p.span_sb0, p.sb_off, A.ptr, A.idx, A.val,
-                               B.idx, B.val, p.bpk, block_ptrs(p, B.ptr), d_jc, d_c,
-                               (int)p.b_strict);
+            const unsigned grid = item_grid(i1 - i0);
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * ITEM_WAVES), sm, s, i1 - i0,
+                               p.items + i0, p.item_off + i0, p.rowbits, p.span_sb0, p.sb_off,
+                               A.ptr, A.idx, A.val, B.idx, B.val, p.bpk, block_ptrs(p, B.ptr),
+                               d_jc, d_c, (int)p.b_strict);
             TSG_LAUNCH_CHECK();
         };
-        if (p.narrow_items)
+        // items of <= SMALL_ITEM outputs (sorted first) in the small-LDS kernel
+        const int64_t ns = p.nitems_small;
+        if (p.narrow_items) {
+            launch_rank(k_big_numeric_rank<FUSED, ITEM_WORDS / 2, SMALL_ITEM, SMALL_RANK_U>,
+                        sizeof(RankSmem<ITEM_WORDS / 2, SMALL_ITEM>) * ITEM_WAVES, 0, ns);
             launch_rank(k_big_numeric_rank<FUSED, ITEM_WORDS / 2>,
-                        sizeof(RankSmem<ITEM_WORDS / 2>) * ITEM_WAVES);
-        else
+                        sizeof(RankSmem<ITEM_WORDS / 2>) * ITEM_WAVES, ns, p.nitems);
+        } else {
+            launch_rank(k_big_numeric_rank<FUSED, ITEM_WORDS, SMALL_ITEM, SMALL_RANK_U>,
+                        sizeof(RankSmem<ITEM_WORDS, SMALL_ITEM>) * ITEM_WAVES, 0, ns);
             launch_rank(k_big_numeric_rank<FUSED, ITEM_WORDS>,
-                        sizeof(RankSmem<ITEM_WORDS>) * ITEM_WAVES);
+                        sizeof(RankSmem<ITEM_WORDS>) * ITEM_WAVES, ns, p.nitems);
+        }
         tm.end();
     } else if (p.nitems > 0) {
         tm.begin("row.big_numeric");
