@@ -156,7 +156,8 @@ struct RowPlan {
     bool column_order = false;       // big-row bands and items run in column order
     // items
     int64_t nitems = 0;
-    int64_t nitems_small = 0;        // items of few outputs, first in items (column order)
+    int64_t nitems_small = 0;        // items of <= 256 outputs, first in items (column order)
+    int64_t nitems_mid = 0;          // end of the items of <= 512 outputs, next
     int64_t *item_cnt = nullptr;     // [nbig+1] items per big row -> offsets
     int4 *items = nullptr;           // {row, c0, c1, big-row index}, by c0
     bool narrow_items = false;       // items of <= 8192 columns (else <= 16384)

@@ -901,26 +901,51 @@ __global__ void __launch_bounds__(256) k_items(int nbig, const int *__restrict__
 // items in this order, so the items in flight at any time gather B entries
 // of one narrow column band, which the L2 caches keep, instead of bands
 // spread over all of B.
-// Items of at most SMALL_ITEM outputs sort first (the numeric phase runs
-// them in a kernel with a quarter of the LDS, twice the items resident),
-// each class in first-column order: key = class << class_bit | first
-// sub-block (class_bit < 0: no classes).  block_small[b] counts block b's
-// items of the first class.
+// Items sort by size class -- at most SMALL_ITEM outputs, at most MID_ITEM,
+// more -- then by first column: key = class << class_bit | first sub-block
+// (class_bit < 0: no classes).  The numeric phase runs each class in a
+// kernel whose value array holds just that class's items, so smaller items
+// take less LDS and more of them are resident.  block_small[b] and
+// block_mid[b] count block b's items of the first two classes.
 constexpr int SMALL_ITEM = 256;
-constexpr int SMALL_RANK_U = RANK_U;
+constexpr int MID_ITEM = 512;
+// The small-item kernel is bound by how many items are resident: with its
+// 5.3 KB of LDS a wave, registers set the limit, so it runs 4 product
+// batches in flight per lane at 6 waves per SIMD (<= 80 VGPRs) instead of 8
+// at 4.  Big numeric on R-MAT 2^20 A*A^T: 149.7 ms at (8, 4); 142.5 at
+// (6, 5); 136.4 at (4, 6); 139.7 at (3, 7) and 138.9 at (2, 8), both with
+// a few registers spilled.
+constexpr int SMALL_RANK_U = 4;
+constexpr int SMALL_MINW = 6;
+// the middle class: 7.3 KB of LDS a wave allows 21 waves per CU, so 5 per
+// SIMD, with 6 batches in flight (<= 96 VGPRs).  R-MAT 2^20 A*A^T: items
+// of > 256 outputs 96.8 ms in one class, 77.3 + 16.9 ms in two.  (Cutting
+// items at 512 outputs where whole sub-blocks allow, so that fewer land in
+// the last class: 148 ms in all instead of 134, more (A entry, item) pairs.)
+constexpr int MID_RANK_U = 6;
+constexpr int MID_MINW = 5;
 
 __global__ void __launch_bounds__(256) k_item_keys(int64_t n, const int4 *__restrict__ items,
                                                    const int *__restrict__ item_n, int class_bit,
                                                    int *__restrict__ keys,
-                                                   int *__restrict__ block_small) {
-    __shared__ int s_cnt[4];
+                                                   int *__restrict__ block_small,
+                                                   int *__restrict__ block_mid) {
+    __shared__ int s_cnt[2][4];
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool sm = i < n && class_bit >= 0 && item_n[i] <= SMALL_ITEM;
-    if (i < n) keys[i] = (sm || class_bit < 0 ? 0 : (1 << class_bit)) | (items[i].y >> SUB_SHIFT);
-    const unsigned long long b = __ballot(sm);
-    if (__lane_id() == 0) s_cnt[threadIdx.x >> 6] = __popcll(b);
+    int cls = 0;
+    if (i < n && class_bit >= 0) cls = item_n[i] <= SMALL_ITEM ? 0 : item_n[i] <= MID_ITEM ? 1 : 2;
+    if (i < n) keys[i] = (cls << max(class_bit, 0)) | (items[i].y >> SUB_SHIFT);
+    const unsigned long long b0 = __ballot(i < n && class_bit >= 0 && cls == 0);
+    const unsigned long long b1 = __ballot(i < n && class_bit >= 0 && cls == 1);
+    if (__lane_id() == 0) {
+        s_cnt[0][threadIdx.x >> 6] = __popcll(b0);
+        s_cnt[1][threadIdx.x >> 6] = __popcll(b1);
+    }
     __syncthreads();
-    if (threadIdx.x == 0) block_small[blockIdx.x] = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+    if (threadIdx.x < 2)
+        (threadIdx.x ? block_mid : block_small)[blockIdx.x] =
+            s_cnt[threadIdx.x][0] + s_cnt[threadIdx.x][1] + s_cnt[threadIdx.x][2] +
+            s_cnt[threadIdx.x][3];
 }
 
 __global__ void __launch_bounds__(256) k_gather_items(int64_t n, const int *__restrict__ order,
@@ -1122,8 +1147,9 @@ __device__ __forceinline__ void item_accumulate(RankSmem<WORDS, CAP> &S, int c0,
                                          (unsigned)__lane_id(), false);
 }
 
-template <bool FUSED, int WORDS, int CAP = ITEM_CAP, int U = RANK_U>
-__global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric_rank(
+template <bool FUSED, int WORDS, int CAP = ITEM_CAP, int U = RANK_U, int MINW = 1>
+__global__ void __launch_bounds__(64 * ITEM_WAVES) __attribute__((amdgpu_waves_per_eu(MINW)))
+k_big_numeric_rank(
     int64_t nitems, const int4 *__restrict__ items, const int64_t *__restrict__ item_off,
     const unsigned *__restrict__ rowbits, const int *__restrict__ span_sb0,
     const int64_t *__restrict__ sb_off, const int *__restrict__ ia, const int *__restrict__ ja,
@@ -1516,22 +1542,25 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
         else
             items_kernel(k_items<true, 16>, p.item_cnt, p.items, p.item_off, item_n);
         p.nitems_small = 0;
+        p.nitems_mid = 0;
         if (p.column_order && p.nitems > 1 && p.nitems < (int64_t(1) << 31)) {
             const size_t ni = (size_t)p.nitems;
             int *keys = ws.alloc_n<int>(ni);
             int *order = ws.alloc_n<int>(ni);
             const unsigned kblocks = grid_for(ni, 256);
             int *bsmall = ws.alloc_n<int>((size_t)kblocks + 1);
+            int *bmid = ws.alloc_n<int>((size_t)kblocks + 1);
             int64_t *tmpk = ws.alloc_n<int64_t>(scan_tmp_elems((size_t)kblocks + 1));
-            int64_t *d_small = ws.alloc_n<int64_t>(1);
+            int64_t *d_small = ws.alloc_n<int64_t>(2);
             int end_bit = 1;
-            while (end_bit < 30 && (int64_t(1) << end_bit) <= ((int64_t)k >> SUB_SHIFT)) end_bit++;
+            while (end_bit < 29 && (int64_t(1) << end_bit) <= ((int64_t)k >> SUB_SHIFT)) end_bit++;
             // the size class splits the rank path's items only
             hipLaunchKernelGGL(k_item_keys, dim3(kblocks), dim3(256), 0, s, p.nitems,
-                               p.items, item_n, p.use_rank ? end_bit : -1, keys, bsmall);
+                               p.items, item_n, p.use_rank ? end_bit : -1, keys, bsmall, bmid);
             TSG_LAUNCH_CHECK();
             exclusive_scan_i32(bsmall, bsmall, (size_t)kblocks, tmpk, s, d_small);
-            sort_order_i32(keys, p.nitems, p.use_rank ? end_bit + 1 : end_bit, order, ws, s);
+            exclusive_scan_i32(bmid, bmid, (size_t)kblocks, tmpk, s, d_small + 1);
+            sort_order_i32(keys, p.nitems, p.use_rank ? end_bit + 2 : end_bit, order, ws, s);
             int4 *items_s = ws.alloc_n<int4>(ni + 1);
             int64_t *off_s = ws.alloc_n<int64_t>(ni + 1);
             hipLaunchKernelGGL(k_gather_items, dim3(grid_for(ni, 256)), dim3(256), 0, s,
@@ -1540,11 +1569,12 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
             p.items = items_s;
             p.item_off = off_s;
             if (p.use_rank) {
-                int64_t h_small = 0;
-                TSG_HIP_TRY(hipMemcpyAsync(&h_small, d_small, sizeof(h_small),
-                                           hipMemcpyDeviceToHost, s));
+                int64_t h_cls[2] = {0, 0};
+                TSG_HIP_TRY(hipMemcpyAsync(h_cls, d_small, sizeof(h_cls), hipMemcpyDeviceToHost,
+                                           s));
                 TSG_HIP_TRY(hipStreamSynchronize(s));
-                p.nitems_small = h_small;
+                p.nitems_small = h_cls[0];
+                p.nitems_mid = h_cls[0] + h_cls[1];
             }
         }
         tm.end();
@@ -1619,18 +1649,25 @@ static void row_numeric_t(RowPlan &p, const CsrIn &A, const CsrIn &B, const int6
                                d_jc, d_c, (int)p.b_strict);
             TSG_LAUNCH_CHECK();
         };
-        // items of <= SMALL_ITEM outputs (sorted first) in the small-LDS kernel
-        const int64_t ns = p.nitems_small;
+        // the size classes (sorted in this order) in kernels with value
+        // arrays of their size
+        const int64_t ns = p.nitems_small, nm = std::max(p.nitems_mid, ns);
         if (p.narrow_items) {
-            launch_rank(k_big_numeric_rank<FUSED, ITEM_WORDS / 2, SMALL_ITEM, SMALL_RANK_U>,
-                        sizeof(RankSmem<ITEM_WORDS / 2, SMALL_ITEM>) * ITEM_WAVES, 0, ns);
-            launch_rank(k_big_numeric_rank<FUSED, ITEM_WORDS / 2>,
-                        sizeof(RankSmem<ITEM_WORDS / 2>) * ITEM_WAVES, ns, p.nitems);
+            constexpr int W = ITEM_WORDS / 2;
+            launch_rank(k_big_numeric_rank<FUSED, W, SMALL_ITEM, SMALL_RANK_U, SMALL_MINW>,
+                        sizeof(RankSmem<W, SMALL_ITEM>) * ITEM_WAVES, 0, ns);
+            launch_rank(k_big_numeric_rank<FUSED, W, MID_ITEM, MID_RANK_U, MID_MINW>,
+                        sizeof(RankSmem<W, MID_ITEM>) * ITEM_WAVES, ns, nm);
+            launch_rank(k_big_numeric_rank<FUSED, W>, sizeof(RankSmem<W>) * ITEM_WAVES, nm,
+                        p.nitems);
         } else {
-            launch_rank(k_big_numeric_rank<FUSED, ITEM_WORDS, SMALL_ITEM, SMALL_RANK_U>,
-                        sizeof(RankSmem<ITEM_WORDS, SMALL_ITEM>) * ITEM_WAVES, 0, ns);
-            launch_rank(k_big_numeric_rank<FUSED, ITEM_WORDS>,
-                        sizeof(RankSmem<ITEM_WORDS>) * ITEM_WAVES, ns, p.nitems);
+            constexpr int W = ITEM_WORDS;
+            launch_rank(k_big_numeric_rank<FUSED, W, SMALL_ITEM, SMALL_RANK_U, SMALL_MINW>,
+                        sizeof(RankSmem<W, SMALL_ITEM>) * ITEM_WAVES, 0, ns);
+            launch_rank(k_big_numeric_rank<FUSED, W, MID_ITEM, MID_RANK_U, MID_MINW>,
+                        sizeof(RankSmem<W, MID_ITEM>) * ITEM_WAVES, ns, nm);
+            launch_rank(k_big_numeric_rank<FUSED, W>, sizeof(RankSmem<W>) * ITEM_WAVES, nm,
+                        p.nitems);
         }
         tm.end();
     } else if (p.nitems > 0) {
