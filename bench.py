@@ -13,8 +13,10 @@ rewritten every step.  GFLOP/s = 2 * intermediate products / step time
 
 Prints one JSON line (rank 0 only; this build runs on one GPU):
   value         whole-job GFLOP/s
-  roofline      dominant kernel's algorithmic bytes / its average duration
-                (HIP events on the library's stream) against 8 TB/s
+  roofline      the dominant stage's algorithmic bytes / its duration per step
+                (HIP events on the library's stream; on R-MAT the big-row
+                numeric stage, three launches by item size class) against
+                8 TB/s
   cpu_baseline  the reference's own CPU SpGEMM (oracle/_ref, OpenMP) on a
                 bounded, evenly strided sample of A's rows, same B
 """
@@ -762,8 +764,9 @@ def main():
         traffic = tj.get(dom)
         if traffic is not None:
             traffic_src = (f"profiles/traffic_{args.workload}.md: rocprofv3 --pmc FETCH_SIZE, "
-                           "then WRITE_SIZE, per dispatch of this kernel, scaled by the same "
-                           "counters on a plain 4 GiB copy; profiled, not measured in this run")
+                           "then WRITE_SIZE, per dispatch, summed over the stage's dispatches in "
+                           "one step, scaled by the same counters on a plain 4 GiB copy; "
+                           "profiled, not measured in this run")
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "traffic_source": traffic_src,
