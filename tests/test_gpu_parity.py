@@ -134,6 +134,42 @@ def test_big_row_items_in_column_order(handle):
         handle.set_item_order(-1)
 
 
+def _size_class_case():
+    """Big rows whose items (8192-column windows, <= 1024 outputs) fall in
+    all three size classes of the rank kernels: rows of A with 10, 60 or 200
+    entries against B rows of 128 entries spread over 131072 columns give
+    ~80, ~470 and (cut by count) ~900 outputs per item."""
+    rng = np.random.default_rng(23)
+    n, k = 4096, 1 << 17
+    degs = np.repeat([10, 60, 200], 300)
+    rng.shuffle(degs)
+    m = degs.size
+    arows = np.repeat(np.arange(m), degs)
+    acols = np.concatenate([rng.choice(n, d, replace=False) for d in degs])
+    A = gen.from_coo(m, arows, acols, rng.uniform(-1, 1, arows.size))
+    brows = np.repeat(np.arange(n), 128)
+    bcols = np.concatenate([rng.choice(k, 128, replace=False) for _ in range(n)])
+    B = gen.from_coo(n, brows, bcols, rng.uniform(-1, 1, brows.size))
+    return m, n, k, A, B
+
+
+@pytest.mark.parametrize("fp", [tsg.FP_FUSED, tsg.FP_SEPARATE])
+def test_big_row_item_size_classes(handle, fp):
+    """Items in column order split into the three size classes, each run by
+    its own rank kernel (both FP modes): the oracle's bits."""
+    fused = fp == tsg.FP_FUSED
+    m, n, k, A, B = _size_class_case()
+    handle.set_item_order(0)
+    try:
+        got = run(handle, m, n, k, A, B, tsg.METHOD_ROW, fp=fp)
+    finally:
+        handle.set_item_order(1 << 20)
+        handle.set_fp_mode(tsg.FP_FUSED)
+    want = O.csr_spgemm(m, k, A, B, fused=fused)
+    print(f"[size classes fp{fp}] nnzC={want[0][-1]}", flush=True)
+    assert_csr_equal(got, want)
+
+
 @pytest.mark.parametrize("fp", [tsg.FP_FUSED, tsg.FP_SEPARATE])
 def test_big_rows_hash_fallback(handle, fp):
     """No memory for span bitmaps: the big-row numeric phase keeps each
