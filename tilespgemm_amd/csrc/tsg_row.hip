@@ -918,7 +918,9 @@ constexpr int MID_ITEM = 512;
 constexpr int SMALL_RANK_U = 4;
 constexpr int SMALL_MINW = 6;
 // the middle class: 7.3 KB of LDS a wave allows 21 waves per CU, so 5 per
-// SIMD, with 6 batches in flight (<= 96 VGPRs).  R-MAT 2^20 A*A^T: items
+// SIMD, with 6 batches in flight (<= 96 VGPRs).  (The last class stays at 8
+// batches: its LDS caps it at 14 waves per CU, and 10 or 12 batches in the
+// registers that leaves gave 133.3 and 135.1 ms against 134.2.)  R-MAT 2^20 A*A^T: items
 // of > 256 outputs 96.8 ms in one class, 77.3 + 16.9 ms in two.  (Cutting
 // items at 512 outputs where whole sub-blocks allow, so that fewer land in
 // the last class: 148 ms in all instead of 134, more (A entry, item) pairs.)
