@@ -246,6 +246,11 @@ int tsg_batch_numeric(tsg_handle_t handle, int count, const tsg_product_t *prods
  * per-call processing in arrival order.
  * Outputs are device buffers owned by the server until released; calls on
  * one server are thread-safe.
+ * The estimator and the workers use workers + 1 streams; give the process at
+ * least that many HIP hardware queues beyond the caller's own busy streams
+ * (GPU_MAX_HW_QUEUES, read when the runtime starts; HIP's default is 4).
+ * Streams that share a hardware queue run in turn, so a worker's host round
+ * trips would wait behind another worker's kernels.
  * ==================================================================== */
 typedef struct tsg_server_s *tsg_server_t;
 

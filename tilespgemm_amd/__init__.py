@@ -234,7 +234,10 @@ class Server:
     submit(A, B) enqueues C = A*B on device-resident operands (keep them alive
     until the result is released) and returns a ticket; wait(ticket) blocks
     and returns the Result; fetch(ticket) copies it to host numpy arrays and
-    releases it."""
+    releases it.  The estimator and each worker use a stream of their own:
+    set GPU_MAX_HW_QUEUES to at least workers + 1 plus the caller's busy
+    streams before HIP starts (its default is 4), or streams that share a
+    hardware queue run in turn."""
 
     def __init__(self, device: int = 0, workers: int = 2, max_batch: int = 64,
                  mem_budget: int = 32 << 30, method: int = METHOD_AUTO, tuning=None):
