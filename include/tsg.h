@@ -136,8 +136,9 @@ int tsg_get_tuning(tsg_handle_t handle, double *tile_density, int *bin1, int *bi
  * units in flight together read B entries of one narrow column band and
  * find them in L2; the B block pointers are then laid out block-major and
  * the items cut to 8192 columns.  Below that the sort costs more than it
- * saves.  Default 2^20 (R-MAT n=2^20 A*A^T: 23M windows).  0
- * always sorts.  Changes speed only. */
+ * saves.  Default 2^18 (R-MAT n=2^20 A*A^T: 24M windows; n=2^18 with 4
+ * draws per row: 523K, 6.48 -> 6.10 ms in column order; n=2^16: 108K,
+ * even; n=2^14: 1.68 -> 1.89 ms).  0 always sorts.  Changes speed only. */
 int tsg_set_item_order(tsg_handle_t handle, int64_t min_items);
 /* Memory for big-row span bitmaps (no counterpart in the reference): the
  * symbolic phase keeps every big row's column bitmap for the rank-indexed

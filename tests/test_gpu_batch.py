@@ -160,7 +160,7 @@ def test_graph_captured_numeric_replays(handle, method, big):
     finally:
         handle.set_stream(None)
         handle.set_tuning({"row_bins": (64, 256, 1024)})
-        handle.set_item_order(1 << 20)
+        handle.set_item_order(1 << 18)
 
 
 def test_auto_batch_tile_dense_with_unsorted_rows(handle):

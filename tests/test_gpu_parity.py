@@ -118,7 +118,7 @@ def test_big_row_items_in_column_order(handle):
     """Items run in first-column order (forced for any item count) give the
     oracle's bits: wide rows with many items, and R-MAT A*A^T with
     duplicate-free B (single-entry batches) and ordered accumulation."""
-    assert handle.item_order() == 1 << 20
+    assert handle.item_order() == 1 << 18
     handle.set_item_order(0)
     try:
         m, n, k, A, B = _wide_case()
@@ -129,7 +129,7 @@ def test_big_row_items_in_column_order(handle):
         got = run(handle, 4096, 4096, 4096, R, RT, tsg.METHOD_ROW)
         assert_csr_equal(got, O.csr_spgemm(4096, 4096, R, RT))
     finally:
-        handle.set_item_order(1 << 20)
+        handle.set_item_order(1 << 18)
     with pytest.raises(tsg.TsgError):
         handle.set_item_order(-1)
 
@@ -163,7 +163,7 @@ def test_big_row_item_size_classes(handle, fp):
     try:
         got = run(handle, m, n, k, A, B, tsg.METHOD_ROW, fp=fp)
     finally:
-        handle.set_item_order(1 << 20)
+        handle.set_item_order(1 << 18)
         handle.set_fp_mode(tsg.FP_FUSED)
     want = O.csr_spgemm(m, k, A, B, fused=fused)
     print(f"[size classes fp{fp}] nnzC={want[0][-1]}", flush=True)
@@ -218,7 +218,7 @@ def test_coarse_block_pointers(handle, cap):
     finally:
         handle.set_block_pointer_budget(-1)
         handle.set_bitmap_budget(-1)
-        handle.set_item_order(1 << 20)
+        handle.set_item_order(1 << 18)
 
 
 @pytest.mark.parametrize("n,deg,bins", [(40000, 12, (64, 256, 1024)),
