@@ -236,7 +236,7 @@ __global__ void __launch_bounds__(256) k_small_symbolic(int nrows, const int *__
                                                [&](bool act, int col, double, double, bool) {
                                                    if (act) {
                                                        const int cc = col - lo;
-                                                       atomicOr(&bmp[cc >> 5], 1u << (cc & 31));
+                                                       lds_set_bit(&bmp[cc >> 5], 1u << (cc & 31));
                                                    }
                                                });
         wave_lds_fence();
@@ -345,7 +345,7 @@ __global__ void __launch_bounds__(RPB * G) k_small_numeric(int nrows, const int 
                 s_mark + grp * G, [&](bool act, int col, double, double, bool) {
                     if (act) {
                         const int cc = col - lo;
-                        atomicOr(&bmp[cc >> 5], 1u << (cc & 31));
+                        lds_set_bit(&bmp[cc >> 5], 1u << (cc & 31));
                     }
                 });
             wave_lds_fence();
@@ -559,7 +559,7 @@ __global__ void __launch_bounds__(64) k_small_numeric_rank(
                                            [&](bool act, int col, double, double, bool) {
                                                const int cc = col - lo;
                                                if (act && (unsigned)cc < span_cols)
-                                                   atomicOr(&bmp[cc >> 5], 1u << (cc & 31));
+                                                   lds_set_bit(&bmp[cc >> 5], 1u << (cc & 31));
                                            });
     wave_lds_fence();
     int carry = 0;
@@ -715,7 +715,7 @@ __global__ void __launch_bounds__(BAND_T) k_big_symbolic(
             [&](bool act, int col, double, double, bool) {
                 if (act) {
                     const int cc = col - c0;
-                    atomicOr(&bits[cc >> 5], 1u << (cc & 31));
+                    lds_set_bit(&bits[cc >> 5], 1u << (cc & 31));
                 }
             },
             wave, NW);

@@ -289,7 +289,7 @@ __global__ void __launch_bounds__(64 * S1_WAVES) k_step1(
                                           [&](bool act, int J, double, double, bool) {
                                            if (act && J >= w0 && J < w1) {
                                                const int c = J - w0;
-                                               atomicOr(&bits[c >> 5], 1u << (c & 31));
+                                               lds_set_bit(&bits[c >> 5], 1u << (c & 31));
                                            }
                                        });
         wave_lds_fence();
