@@ -140,12 +140,17 @@ int tsg_get_tuning(tsg_handle_t handle, double *tile_density, int *bin1, int *bi
  * draws per row: 523K, 6.48 -> 6.10 ms in column order; n=2^16: 108K,
  * even; n=2^14: 1.68 -> 1.89 ms).  0 always sorts.  Changes speed only. */
 int tsg_set_item_order(tsg_handle_t handle, int64_t min_items);
-/* Memory for big-row span bitmaps (no counterpart in the reference): the
- * symbolic phase keeps every big row's column bitmap for the rank-indexed
- * numeric kernel when they take at most this many bytes; beyond that the
- * numeric phase builds each item's columns in an LDS hash table instead
- * (same results, slower: R-MAT n=2^20 A*A^T needs 47 GB of bitmaps).
- * Negative (the default): min(48 GiB, 40% of free device memory). */
+/* Memory for the big rows' column bitmaps (no counterpart in the
+ * reference): the symbolic phase keeps every big row's bitmap in sparse form
+ * -- per 1024-column sub-block a mask of its non-empty 32-bit words, and
+ * those words -- for the rank-indexed numeric kernel when the space reserved
+ * for it (4 bytes per sub-block of the rows' spans plus, per row, 4 bytes
+ * times min(products, span words)) is at most this many bytes; beyond that
+ * the numeric phase builds each item's columns in an LDS hash table instead
+ * (same results, slower).  R-MAT n=2^20 A*A^T: a fifth of its spans' words
+ * are non-empty, so the sparse form takes about a quarter of the 47 GB the
+ * dense bitmaps would.  Negative (the default): min(48 GiB, 40% of free
+ * device memory). */
 int tsg_set_bitmap_budget(tsg_handle_t handle, int64_t bytes);
 /* Memory for the B block-pointer table of the big-row path (no counterpart
  * in the reference): one position per B row and 1024-column block while the

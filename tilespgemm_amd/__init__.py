@@ -405,8 +405,9 @@ class Handle:
         _check(load().tsg_set_item_order(self._h, int(min_items)), self._h)
 
     def set_bitmap_budget(self, nbytes: int):
-        """Bytes of big-row span bitmaps kept for the rank-indexed numeric
-        kernel (tsg_set_bitmap_budget; negative: automatic)."""
+        """Bytes reserved for the big rows' column bitmaps (sparse form)
+        kept for the rank-indexed numeric kernel (tsg_set_bitmap_budget;
+        negative: automatic)."""
         _check(load().tsg_set_bitmap_budget(self._h, int(nbytes)), self._h)
 
     def set_block_pointer_budget(self, nbytes: int):

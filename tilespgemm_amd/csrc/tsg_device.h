@@ -118,8 +118,6 @@ __device__ __forceinline__ double acc_step(double acc, double a, double b) {
     return acc + p;
 }
 
-// Ordering point inside one wave: later LDS operations observe earlier ones
-// of any lane of the same wave.
 // Sets a bit of an LDS bitmap word unless it is already set.  While a
 // bitmap is being marked its bits are only ever set, so a stale read costs
 // at most a redundant atomic; repeated outputs (half the products of
@@ -129,6 +127,8 @@ __device__ __forceinline__ void lds_set_bit(unsigned *word, unsigned bit) {
     if (!(*word & bit)) atomicOr(word, bit);
 }
 
+// Ordering point inside one wave: later LDS operations observe earlier ones
+// of any lane of the same wave.
 __device__ __forceinline__ void wave_lds_fence() {
     // LDS operations of one wave complete in issue order, so ordering only has
     // to be enforced against the compiler.

@@ -118,7 +118,7 @@ struct StageTimer {
 struct RowTuning {
     int bin1 = 64, bin2 = 256, bin3 = 1024;
     int64_t item_sort_min = int64_t(1) << 18;  // big-row items column-ordered from this many
-    int64_t bitmap_budget = -1;  // bytes of span bitmaps kept; < 0: min(48 GiB, 40% of free)
+    int64_t bitmap_budget = -1;  // bytes of big-row column lists kept; < 0: min(48 GiB, 40% of free)
     int64_t bptr_budget = -1;    // bytes of the B block-pointer table; < 0: 6 GiB
 };
 
@@ -161,9 +161,15 @@ struct RowPlan {
     int64_t *item_cnt = nullptr;     // [nbig+1] items per big row -> offsets
     int4 *items = nullptr;           // {row, c0, c1, big-row index}, by c0
     bool narrow_items = false;       // items of <= 8192 columns (else <= 16384)
-    // per big row span bitmaps kept from the symbolic phase (rank mode)
+    // big rows' column bitmaps kept from the symbolic phase (rank mode), in
+    // sparse form: per sub-block a mask of its non-empty words, and per
+    // 65536-column band those words in column order (a band word list),
+    // bands placed in their row's region as they finished
     bool use_rank = false;
-    unsigned *rowbits = nullptr;     // [nsb_total * 32] words
+    int64_t words_cap = 0;           // words of bandwords (a bound)
+    unsigned *bandwords = nullptr;   // [words_cap]
+    unsigned *subocc = nullptr;      // [nsb_total] non-empty-word masks
+    int64_t *item_words = nullptr;   // per item: position of its first word
     int4 *bpk = nullptr;             // [nnz(B)] {column, 0, value} records (rank mode)
     int64_t *item_off = nullptr;     // output offset (within C) per item
     // B with rows sorted, when the input's were not (set after row_symbolic):

@@ -17,10 +17,11 @@
 //   * Big rows: the symbolic phase cuts each row's column span into bands of
 //     65536 columns, marks the row's columns of a band in an LDS bitmap and
 //     records the count of every 1024-column sub-block (and, when memory
-//     allows, the span's bitmap); the numeric phase splits the row into items
-//     of <= ITEM_CAP outputs and <= 16384 columns, each accumulated by one
-//     wave into a rank-ordered LDS array (rank from the stored bitmap) and
-//     written in column order.  A B block pointer table gives each B row's
+//     allows, the band's bitmap in sparse form: a mask of non-empty words per
+//     sub-block, and those words); the numeric phase splits the row into
+//     items of <= ITEM_CAP outputs and <= 16384 columns, each accumulated by
+//     one wave into a rank-ordered LDS array (rank from the stored bitmap)
+//     and written in column order.  A B block pointer table gives each B row's
 //     sub-range of a band or item without a search; bands and items run in
 //     order of their first column, so those in flight share a band of B.
 #include "tsg_internal.h"
@@ -44,6 +45,7 @@ constexpr int SUB_SHIFT = 10;          // sub-block = 1024 columns
 constexpr int BAND_SHIFT = 6;          // symbolic band = 64 sub-blocks (65536 columns)
 constexpr int BAND_T = 128;            // threads per band unit
 constexpr int BAND_WORDS = (1 << (BAND_SHIFT + SUB_SHIFT)) / 32;
+constexpr int BAND_PAD = 4;            // band word lists: words rounded up to this
 constexpr int ITEM_CAP = 1024;         // outputs per item
 constexpr int ITEM_SLOTS = 2048;       // hash slots per item (load <= 1/2)
 constexpr int ITEM_WBITS_SHIFT = 14;   // item width <= 16384 columns
@@ -614,7 +616,9 @@ __global__ void __launch_bounds__(64) k_small_numeric_rank(
 // -------------------------------------------------------- big rows: span
 // Column span of each big row, in 1024-column sub-blocks: from the first
 // sub-block of the first B column to the last sub-block of the last B column
-// over the row's A entries.  One wave per big row.
+// over the row's A entries, and a bound on the words of the row's band word
+// lists: min(products, span words) plus each band's padding.  One wave per
+// big row.
 __global__ void __launch_bounds__(256) k_big_span(int nbig, const int *__restrict__ rows,
                                                   const int *__restrict__ ia,
                                                   const int *__restrict__ ja,
@@ -624,7 +628,8 @@ __global__ void __launch_bounds__(256) k_big_span(int nbig, const int *__restric
                                                   int *__restrict__ sb0,
                                                   int64_t *__restrict__ nsb,
                                                   int64_t *__restrict__ nent,
-                                                  int64_t *__restrict__ nprod) {
+                                                  int64_t *__restrict__ nprod,
+                                                  int64_t *__restrict__ lbound) {
     const int r = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int l = threadIdx.x & 63;
     if (r >= nbig) return;
@@ -644,10 +649,14 @@ __global__ void __launch_bounds__(256) k_big_span(int nbig, const int *__restric
         hi = max(hi, __shfl_xor(hi, d, 64));
     }
     if (l == 0) {
+        const int64_t ns = hi >= lo ? (int64_t)(hi - lo + 1) : 0;
         sb0[r] = hi >= lo ? lo : 0;
-        nsb[r] = hi >= lo ? (int64_t)(hi - lo + 1) : 0;
+        nsb[r] = ns;
         nent[r] = ia[row + 1] - ia[row];
         nprod[r] = rowprod[row];
+        // a multiple of BAND_PAD, so every band's list starts 16-byte aligned
+        lbound[r] = ((min(rowprod[row], ns << (SUB_SHIFT - 5)) + BAND_PAD - 1) & -BAND_PAD) +
+                    BAND_PAD * ((ns + (1 << BAND_SHIFT) - 1) >> BAND_SHIFT);
     }
 }
 
@@ -684,18 +693,39 @@ __global__ void __launch_bounds__(256) k_gather_units(int64_t n, const int *__re
 
 // One block per band unit: LDS bitmap over the band, the row's products in
 // the band (B rows restricted through the block pointers) marked by the 2
-// waves taking every other batch of 64 products; then the band's bitmap words
-// (when rowbits is given), its sub-block counts and its share of the row's
-// output count (rownnz of big rows is zero on entry).
-__global__ void __launch_bounds__(BAND_T) k_big_symbolic(
+// waves taking every other batch of 64 products; then the band's sub-block
+// output counts, its share of the row's output count (rownnz of big rows is
+// zero on entry) and, when words is given, the band's bitmap in sparse form:
+// per sub-block a mask of its non-empty words (subocc), and the non-empty
+// words themselves in column order (the band word list), in the row's region
+// [row_off[r], row_off[r + 1]) at a position taken from the row's cursor
+// (row_cur[r], starting at row_off[r]) and recorded in
+// band_base[band_off[r] + u].  On R-MAT's big rows a fifth of the span's
+// words are non-empty, so the lists and masks take a quarter of the bitmap's
+// bytes, and no step expands bits.  A band takes its word count rounded up to
+// BAND_PAD, so its list starts 16-byte aligned and goes out in whole 16-byte
+// stores, staged in the bitmap's LDS once the words are in registers (one
+// cursor for all bands had them queue on a single address; the cursor's
+// result is first needed after the staging).  A band whose list would pass
+// its row's region writes none and sets *overflow, which the host checks.
+// 8 waves per SIMD, as with no word lists (the kernel waits on its gathers;
+// its arguments alone would take the scalar registers past that).
+__global__ void __launch_bounds__(BAND_T) __attribute__((amdgpu_waves_per_eu(8))) k_big_symbolic(
     int64_t nunits, const int2 *__restrict__ units, const int *__restrict__ rows, int k,
     const int *__restrict__ ia, const int *__restrict__ ja, const int *__restrict__ ib,
     const int *__restrict__ jb, BlockPtrs bp, const int *__restrict__ span_sb0,
-    const int64_t *__restrict__ sb_off, uint16_t *__restrict__ subcnt, int64_t *__restrict__ rownnz, unsigned *__restrict__ rowbits) {
-    __shared__ unsigned bits[BAND_WORDS];
-    __shared__ int s_mark[BAND_T / 64][64];
-    __shared__ int s_total[BAND_T / 64];
+    const int64_t *__restrict__ sb_off, uint16_t *__restrict__ subcnt,
+    int64_t *__restrict__ rownnz, const int64_t *__restrict__ band_off,
+    int64_t *__restrict__ band_base, unsigned long long *__restrict__ row_cur,
+    const int64_t *__restrict__ row_off, unsigned *__restrict__ overflow,
+    unsigned *__restrict__ subocc, unsigned *__restrict__ words_out) {
     constexpr int NW = BAND_T / 64;
+    constexpr int TW = BAND_WORDS / BAND_T;        // words per thread, half a sub-block
+    static_assert(TW == 16 && 2 * TW == (1 << (SUB_SHIFT - 5)), "a sub-block is two threads' words");
+    __shared__ __align__(16) unsigned bits[BAND_WORDS];
+    __shared__ int s_mark[NW][64];
+    __shared__ int s_wsum[NW];
+    __shared__ long long s_base;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     s_mark[wave][lane] = -1;
     for (int64_t ui = blockIdx.x; ui < nunits; ui += gridDim.x) {
@@ -720,35 +750,88 @@ __global__ void __launch_bounds__(BAND_T) k_big_symbolic(
             },
             wave, NW);
         __syncthreads();
-        if (rowbits) {
-            // written once, read once by the numeric phase much later:
-            // streaming stores keep the L2 for B's rows
-            unsigned *dst = rowbits + so * (1 << (SUB_SHIFT - 5));
-            for (int i = threadIdx.x; i < words; i += BAND_T)
-                __builtin_nontemporal_store(bits[i], dst + i);
-        }
-        // sub-block counts: a sub-block's 32 words are one half-wave, read
-        // consecutively (no bank conflicts) and summed across the half
-        static_assert(SUB_SHIFT - 5 == 5, "a sub-block is 32 bitmap words");
-        int part = 0;
-        for (int w0 = wave * 64; w0 < (nsb << 5); w0 += BAND_T) {
-            const int w = w0 + lane;
-            int pc = w < words ? __popc(bits[w]) : 0;
+        // thread t takes words [TW t, TW t + TW) (words past the band's end
+        // are stale); threads 2s and 2s + 1 hold sub-block s
+        unsigned wv[TW];
+        int tot = 0;
+        unsigned nzm = 0;
+        {
+            const uint4 *src = reinterpret_cast<const uint4 *>(bits) + threadIdx.x * (TW / 4);
 #pragma unroll
-            for (int d = 16; d > 0; d >>= 1) pc += __shfl_xor(pc, d, 64);
-            if ((lane & 31) == 0 && (w >> 5) < nsb) {
-                subcnt[so + (w >> 5)] = (uint16_t)pc;
-                part += pc;
+            for (int q = 0; q < TW / 4; q++) {
+                const uint4 v = src[q];
+                wv[4 * q] = v.x;
+                wv[4 * q + 1] = v.y;
+                wv[4 * q + 2] = v.z;
+                wv[4 * q + 3] = v.w;
+            }
+#pragma unroll
+            for (int j = 0; j < TW; j++) {
+                if ((int)threadIdx.x * TW + j >= words) wv[j] = 0u;
+                tot += __popc(wv[j]);
+                nzm |= (wv[j] != 0u ? 1u : 0u) << j;
             }
         }
-        part = wave_sum(part);
-        if (lane == 0) s_total[wave] = part;
+        const int sbc = tot + __shfl_xor(tot, 1, 64);
+        const unsigned occ = nzm | (__shfl_xor(nzm, 1, 64) << TW);
+        if (!(threadIdx.x & 1) && (int)(threadIdx.x >> 1) < nsb) {
+            subcnt[so + (threadIdx.x >> 1)] = (uint16_t)sbc;
+            if (words_out) subocc[so + (threadIdx.x >> 1)] = occ;
+        }
+        // block-wide scans: outputs (for the row's count) and non-empty words
+        const int nz = __popc(nzm);
+        const int packed_incl = wave_incl_scan(tot | (nz << 16));
+        if (lane == 63) s_wsum[wave] = packed_incl;
         __syncthreads();
+        int before = (packed_incl >> 16) - nz, total = 0, total_nz = 0;
+#pragma unroll
+        for (int i = 0; i < NW; i++) {
+            const int wsum = s_wsum[i];
+            if (i < wave) before += wsum >> 16;
+            total += wsum & 0xffff;
+            total_nz += wsum >> 16;
+        }
+        const int padded = (total_nz + BAND_PAD - 1) & -BAND_PAD;
+        // thread 0 takes the band's place in its row's region; the result is
+        // first needed after the list is staged, so the atomic's round trip
+        // overlaps the staging
+        unsigned long long ticket = 0;
         if (threadIdx.x == 0) {
-            int t = 0;
-            for (int i = 0; i < NW; i++) t += s_total[i];
-            if (t) atomicAdd(reinterpret_cast<unsigned long long *>(rownnz + row),
-                             (unsigned long long)t);
+            if (total)
+                atomicAdd(reinterpret_cast<unsigned long long *>(rownnz + row),
+                          (unsigned long long)total);
+            if (words_out && padded)
+                ticket = atomicAdd(row_cur + r, (unsigned long long)padded);
+        }
+        if (words_out && total_nz > 0) {
+            // the band's <= 2048 non-empty words fit the bitmap's LDS
+            int pos = before;
+#pragma unroll
+            for (int j = 0; j < TW; j++) {
+                if (wv[j]) bits[pos] = wv[j];
+                pos += wv[j] ? 1 : 0;
+            }
+            if (threadIdx.x == 0) {
+                long long base = (long long)ticket;
+                if (base + padded > row_off[r + 1]) {
+                    atomicOr(overflow, 1u);
+                    base = -1;
+                }
+                band_base[band_off[r] + u] = base;
+                s_base = base;
+            }
+            __syncthreads();
+            const long long base = s_base;
+            if (base >= 0) {
+                // whole 16-byte stores, the last into the band's padding.
+                // Written once, read once by the numeric phase much later:
+                // streaming stores keep the L2 for B's rows
+                typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+                u32x4 *dst = reinterpret_cast<u32x4 *>(words_out + base);
+                const u32x4 *sv = reinterpret_cast<const u32x4 *>(bits);
+                for (int i = threadIdx.x; i < padded / 4; i += BAND_T)
+                    __builtin_nontemporal_store(sv[i], dst + i);
+            }
         }
         __syncthreads();
     }
@@ -805,6 +888,7 @@ __global__ void __launch_bounds__(256) k_bptr(int n, const int *__restrict__ ib,
 }
 
 // ------------------------------------------------------------- item plan
+
 // Split of each big row into items: the row's span is cut into windows of
 // WSB sub-blocks (at most ITEM_WBITS columns); inside a window, runs of
 // sub-blocks with at most ITEM_CAP outputs, leading empty sub-blocks skipped.
@@ -812,7 +896,14 @@ __global__ void __launch_bounds__(256) k_bptr(int n, const int *__restrict__ ib,
 // loads its window's counts at once and splits them in registers, and wave
 // scans place the items and their outputs.  FILL=false counts items per
 // row; FILL=true (item_cnt = exclusive item offsets) writes {row, c0, c1,
-// big-row index} and each item's first output position.
+// big-row index}, each item's first output position and output count, and,
+// when item_words is given, where its first non-empty word is in the band
+// word lists of the symbolic phase (from the sub-blocks' masks of non-empty
+// words); the item's fourth field then holds its first sub-block's index in
+// subcnt and subocc (< 2^31 in that mode) instead of the big-row index, so
+// the numeric phase finds the item's masks with no other lookup.  Windows lie inside bands, and a band's windows inside one pass of
+// 64 windows, so the item's place in its band's list is its count of
+// non-empty words before it less that of the band's first window.
 template <bool FILL, int WSB>
 __global__ void __launch_bounds__(256) k_items(int nbig, const int *__restrict__ rows, int k,
                                                const int *__restrict__ span_sb0,
@@ -822,8 +913,14 @@ __global__ void __launch_bounds__(256) k_items(int nbig, const int *__restrict__
                                                int64_t *__restrict__ item_cnt,
                                                int4 *__restrict__ items,
                                                int64_t *__restrict__ item_off,
-                                               int *__restrict__ item_n) {
+                                               int *__restrict__ item_n,
+                                               const unsigned *__restrict__ subocc,
+                                               const int64_t *__restrict__ band_off,
+                                               const int64_t *__restrict__ band_base,
+                                               int64_t *__restrict__ item_words) {
     static_assert(WSB <= (1 << (ITEM_WBITS_SHIFT - SUB_SHIFT)), "item wider than its LDS");
+    constexpr int WPB = (1 << BAND_SHIFT) / WSB;   // windows per band
+    static_assert(WPB * WSB == (1 << BAND_SHIFT) && 64 % WPB == 0, "windows tile the bands");
     const int r = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int l = threadIdx.x & 63;
     if (r >= nbig) return;
@@ -838,9 +935,14 @@ __global__ void __launch_bounds__(256) k_items(int nbig, const int *__restrict__
         const int win = w0 + l;
         const int j0 = win * WSB;
         const int len = win < nwin ? min(WSB, nsb - j0) : 0;
-        int cs[WSB];
+        int cs[WSB], os[WSB];
+        int nzt = 0;
 #pragma unroll
-        for (int q = 0; q < WSB; q++) cs[q] = q < len ? (int)sc[j0 + q] : 0;
+        for (int q = 0; q < WSB; q++) {
+            cs[q] = q < len ? (int)sc[j0 + q] : 0;
+            os[q] = FILL && item_words && q < len ? __popc(subocc[sb_off[r] + j0 + q]) : 0;
+            nzt += os[q];
+        }
         int nit = 0, tot = 0;
         {
             int start = -1, cnt = 0;
@@ -861,33 +963,48 @@ __global__ void __launch_bounds__(256) k_items(int nbig, const int *__restrict__
         }
         const int it_incl = wave_incl_scan(nit);
         const int out_incl = wave_incl_scan(tot);
+        // non-empty words of this pass before this window, and before the
+        // first window of this lane's band
+        const int nz_excl = FILL ? wave_incl_scan(nzt) - nzt : 0;
+        const int band_nz = FILL ? __shfl(nz_excl, (l / WPB) * WPB, 64) : 0;
         if (FILL && nit > 0) {
             int64_t w = wbase + it_incl - nit;
             int64_t out = obase + out_incl - tot;
             const int cbase = first + j0;
-            int start = -1, cnt = 0;
+            // this window's first non-empty word in the band word lists
+            const int64_t wb =
+                item_words ? band_base[band_off[r] + win / WPB] + (nz_excl - band_nz) : 0;
+            int start = -1, cnt = 0, nzrun = 0, nzs = 0;
 #pragma unroll
             for (int q = 0; q < WSB; q++) {
                 if (start >= 0 && cnt + cs[q] > ITEM_CAP) {
                     items[w] = make_int4(row, (cbase + start) << SUB_SHIFT,
-                                         min(k, (cbase + q) << SUB_SHIFT), r);
+                                         min(k, (cbase + q) << SUB_SHIFT),
+                                         item_words ? (int)(sb_off[r] + j0 + start) : r);
                     item_off[w] = out;
                     item_n[w] = cnt;
+                    if (item_words) item_words[w] = wb + nzs;
                     w++;
                     out += cnt;
                     start = -1;
                     cnt = 0;
                 }
                 if (cs[q] != 0 || start >= 0) {
-                    if (start < 0) start = q;
+                    if (start < 0) {
+                        start = q;
+                        nzs = nzrun;
+                    }
                     cnt += cs[q];
                 }
+                nzrun += os[q];
             }
             if (start >= 0 && cnt > 0) {
                 items[w] = make_int4(row, (cbase + start) << SUB_SHIFT,
-                                     min(k, (cbase + len) << SUB_SHIFT), r);
+                                     min(k, (cbase + len) << SUB_SHIFT),
+                                     item_words ? (int)(sb_off[r] + j0 + start) : r);
                 item_off[w] = out;
                 item_n[w] = cnt;
+                if (item_words) item_words[w] = wb + nzs;
             }
         }
         wbase += group_last<64>(it_incl);
@@ -953,13 +1070,16 @@ __global__ void __launch_bounds__(256) k_item_keys(int64_t n, const int4 *__rest
 __global__ void __launch_bounds__(256) k_gather_items(int64_t n, const int *__restrict__ order,
                                                       const int4 *__restrict__ items,
                                                       const int64_t *__restrict__ off,
+                                                      const int64_t *__restrict__ cols,
                                                       int4 *__restrict__ items_out,
-                                                      int64_t *__restrict__ off_out) {
+                                                      int64_t *__restrict__ off_out,
+                                                      int64_t *__restrict__ cols_out) {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j < n) {
         const int i = order[j];
         items_out[j] = items[i];
         off_out[j] = off[i];
+        if (cols) cols_out[j] = cols[i];
     }
 }
 
@@ -1045,10 +1165,11 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric(
 }
 
 // Rank-indexed variant: the symbolic phase stored every big row's column
-// bitmap, so an item's outputs are known before its products are visited.
-// The rank of column j inside the item is prefix[w] + popc(bits[w] below j)
-// -- one LDS read of a {bits, prefix} pair -- and values accumulate in a
-// dense rank-ordered array that is written out coalesced.
+// bitmap (in sparse form: per sub-block a mask of its non-empty words, and
+// those words), so an item's outputs are known before its products are
+// visited.  The rank of column j inside the item is prefix[w] + popc(bits[w]
+// below j) -- one LDS read of a {bits, prefix} pair -- and values accumulate
+// in a dense rank-ordered array that is written out coalesced.
 // WORDS: bitmap words of the widest item (256 or 512: items of 8192 or
 // 16384 columns).
 template <int WORDS, int CAP = ITEM_CAP>
@@ -1066,16 +1187,33 @@ struct RankSmem {
 // the item's columns to jc at offset o (expanded into S.vals, then stored
 // coalesced; the stores drain while the products are processed), zeroes the
 // item's first cnt values and returns cnt, the item's output count.
+// item_words rebuilds the words from the band word list: lane l takes words
+// l, l + 64, ..., and word w of sub-block s (lanes 0..15 of occ hold the
+// item's sub-block masks, of pre their exclusive popcount prefix) is there
+// when its bit of occ[s] is set, at list + pre[s] + popc(occ[s] below it).
 template <int WCH>
-__device__ __forceinline__ void item_words(const unsigned *__restrict__ src, int c0, int c1,
-                                           unsigned (&wv)[WCH]) {
+__device__ __forceinline__ void item_words(const unsigned *__restrict__ list, unsigned occ,
+                                           int pre, int c0, int c1, unsigned (&wv)[WCH]) {
+    static_assert(2 * WCH <= 16, "an item's sub-block masks sit in lanes 0..15");
     const int l = __lane_id();
     const int nwords = (c1 - c0 + 31) >> 5;
+    const unsigned below = (1u << (l & 31)) - 1u;
 #pragma unroll
     for (int j = 0; j < WCH; j++) {
         const int w = j * 64 + l;
-        // read once: a streaming load keeps the L2 for B's rows
-        wv[j] = w < nwords ? __builtin_nontemporal_load(src + w) : 0u;
+        // word w is in sub-block 2j (lanes 0..31) or 2j + 1 (lanes 32..63)
+        const unsigned o0 = __builtin_amdgcn_readlane(occ, 2 * j);
+        const unsigned o1 = __builtin_amdgcn_readlane(occ, 2 * j + 1);
+        const int p0 = __builtin_amdgcn_readlane(pre, 2 * j);
+        const int p1 = __builtin_amdgcn_readlane(pre, 2 * j + 1);
+        const unsigned om = l < 32 ? o0 : o1;
+        const int pm = l < 32 ? p0 : p1;
+        const bool present = w < nwords && ((om >> (l & 31)) & 1u);
+        // read once: a streaming load keeps the L2 for B's rows.  A 32-bit
+        // offset from the wave's scalar base keeps one address register
+        // per load
+        const unsigned off = (unsigned)pm + (unsigned)__popc(om & below);
+        wv[j] = present ? __builtin_nontemporal_load(list + off) : 0u;
     }
 }
 
@@ -1153,8 +1291,8 @@ template <bool FUSED, int WORDS, int CAP = ITEM_CAP, int U = RANK_U, int MINW = 
 __global__ void __launch_bounds__(64 * ITEM_WAVES) __attribute__((amdgpu_waves_per_eu(MINW)))
 k_big_numeric_rank(
     int64_t nitems, const int4 *__restrict__ items, const int64_t *__restrict__ item_off,
-    const unsigned *__restrict__ rowbits, const int *__restrict__ span_sb0,
-    const int64_t *__restrict__ sb_off, const int *__restrict__ ia, const int *__restrict__ ja,
+    const int64_t *__restrict__ item_wpos, const unsigned *__restrict__ wlist,
+    const unsigned *__restrict__ subocc, const int *__restrict__ ia, const int *__restrict__ ja,
     const double *__restrict__ a, const int *__restrict__ jb, const double *__restrict__ b,
     const int4 *__restrict__ bpk, BlockPtrs bp, int *__restrict__ jc, double *__restrict__ c,
     int b_strict) {
@@ -1165,33 +1303,54 @@ k_big_numeric_rank(
     S.mark[l] = -1;
     const int64_t wave_id = (int64_t)blockIdx.x * ITEM_WAVES + (threadIdx.x >> 6);
     const int64_t nwaves = (int64_t)gridDim.x * ITEM_WAVES;
-    // the next item's descriptor is loaded when an item starts and its row
-    // bounds and bitmap address while the item's outputs are stored, so an
-    // item's first product loads need no chain of dependent loads
+    // the next item's descriptor is loaded when an item starts, and its
+    // sub-block masks, word-list position and A row bounds while the item's
+    // outputs are stored, so an item's first product loads need no chain of
+    // dependent loads.  Those and the item's output offset go into one
+    // register, a lane each (lanes 0..15 the masks, 16 and 17 the position,
+    // 18 and 19 the row bounds, 20 and 21 the offset: one load instruction,
+    // and they reach scalar registers at the item's start -- the small-item
+    // kernel has no vector registers to spare).
     int4 item = make_int4(0, 0, 0, 0);
-    int pa = 0, pe = 0;
-    const unsigned *src = rowbits;
-    auto locate = [&]() {
-        pa = ia[item.x];
-        pe = ia[item.x + 1];
-        src = rowbits + sb_off[item.w] * (1 << (SUB_SHIFT - 5)) +
-              ((item.y >> 5) - (span_sb0[item.w] << (SUB_SHIFT - 5)));
+    unsigned nx = 0u;
+    auto locate = [&](int64_t at) {
+        // item.w: the item's first sub-block in subocc
+        const int nsb = (item.z - item.y + (1 << SUB_SHIFT) - 1) >> SUB_SHIFT;
+        const unsigned *src = nullptr;
+        if (l < min(nsb, 2 * (WORDS / 64)))
+            src = subocc + ((int64_t)item.w + l);
+        else if (l == 16 || l == 17)
+            src = reinterpret_cast<const unsigned *>(item_wpos + at) + (l - 16);
+        else if (l == 18 || l == 19)
+            src = reinterpret_cast<const unsigned *>(ia + item.x) + (l - 18);
+        else if (l == 20 || l == 21)
+            src = reinterpret_cast<const unsigned *>(item_off + at) + (l - 20);
+        nx = src ? *src : 0u;
     };
     // (streaming loads of the item descriptors and A entries, read once per
     // item, were measured slower: 158 -> 166 ms on R-MAT 2^20 A*A^T)
     if (wave_id < nitems) {
         item = items[wave_id];
-        locate();
+        locate(wave_id);
     }
     for (int64_t it = wave_id; it < nitems; it += nwaves) {
-        const int c0 = item.y, c1 = item.z, ipa = pa, ipe = pe;
-        const unsigned *isrc = src;
-        const int64_t o = item_off[it];
+        const int c0 = __builtin_amdgcn_readfirstlane(item.y);
+        const int c1 = __builtin_amdgcn_readfirstlane(item.z);
+        const unsigned iocc = l < 16 ? nx : 0u;
+        const int ipa = __builtin_amdgcn_readlane((int)nx, 18);
+        const int ipe = __builtin_amdgcn_readlane((int)nx, 19);
+        const int64_t iwpos =
+            (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)nx, 17) << 32) |
+                      (uint32_t)__builtin_amdgcn_readlane((int)nx, 16));
+        const unsigned *ilist = wlist + iwpos;
+        const int64_t o =
+            (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)nx, 21) << 32) |
+                      (uint32_t)__builtin_amdgcn_readlane((int)nx, 20));
         const int64_t next = it + nwaves;
         if (next < nitems) item = items[next];
         // the product pipeline of for_products, with the rank table built
         // after the first groups' A entries and B ranges are issued, so the
-        // bitmap load and that chain of dependent loads overlap
+        // word loads and that chain of dependent loads overlap
         auto load_t = [&](int g, int &t, double &av) {
             const int p = g + l;
             t = -1;
@@ -1213,8 +1372,10 @@ k_big_numeric_rank(
         double a0, a1;
         load_t(ipa, t0, a0);
         load_t(ipa + 64, t1, a1);
+        const int pc = __popc(iocc);
+        const int pre = dpp_incl_scan<16>(pc) - pc;
         unsigned wv[WORDS / 64];
-        item_words(isrc, c0, c1, wv);
+        item_words(ilist, iocc, pre, c0, c1, wv);
         int lo, hi;
         load_range(t0, lo, hi);
         const int cnt = item_rank_build<WORDS, CAP>(S, wv, c0, c1, o, jc);
@@ -1234,7 +1395,7 @@ k_big_numeric_rank(
             a0 = a1;
             a1 = a2;
         }
-        if (next < nitems) locate();
+        if (next < nitems) locate(next);
         item_store_values(S, cnt, o, c);
     }
 }
@@ -1376,6 +1537,9 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
     // big rows
     p.nbig = st.s[5] - st.s[4];
     p.sub_shift = SUB_SHIFT;
+    int64_t *band_off = nullptr, *band_base = nullptr, *row_loff = nullptr;
+    unsigned long long *row_cur = nullptr;
+    unsigned *d_overflow = nullptr;
     if (p.nbig > 0) {
         const int *rowsb = p.bin_rows + st.s[4];
         tm.begin("row.big_symbolic");
@@ -1383,13 +1547,14 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
         // band units over the spans
         p.span_sb0 = ws.alloc_n<int>((size_t)p.nbig + 1);
         p.sb_off = ws.alloc_n<int64_t>((size_t)p.nbig + 1);
-        int64_t *band_off = ws.alloc_n<int64_t>((size_t)p.nbig + 1);
+        band_off = ws.alloc_n<int64_t>((size_t)p.nbig + 1);
         int64_t *tmp_sb = ws.alloc_n<int64_t>(scan_tmp_elems((size_t)p.nbig + 1));
         int64_t *a_ent = ws.alloc_n<int64_t>((size_t)p.nbig + 1);
         int64_t *a_prod = ws.alloc_n<int64_t>((size_t)p.nbig + 1);
+        row_loff = ws.alloc_n<int64_t>((size_t)p.nbig + 1);
         hipLaunchKernelGGL(k_big_span, dim3(grid_for((size_t)p.nbig * 64, 256)), dim3(256), 0, s,
                            p.nbig, rowsb, A.ptr, A.idx, B.ptr, B.idx, p.rowprod, p.span_sb0,
-                           p.sb_off, a_ent, a_prod);
+                           p.sb_off, a_ent, a_prod, row_loff);
         TSG_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_band_count, dim3(grid_for((size_t)p.nbig, 256)), dim3(256), 0, s,
                            p.nbig, p.sb_off, band_off);
@@ -1398,7 +1563,8 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
         exclusive_scan_i64(band_off, band_off, (size_t)p.nbig, tmp_sb, s);
         exclusive_scan_i64(a_ent, a_ent, (size_t)p.nbig, tmp_sb, s);
         exclusive_scan_i64(a_prod, a_prod, (size_t)p.nbig, tmp_sb, s);
-        int64_t tot2[4] = {0, 0, 0, 0};
+        exclusive_scan_i64(row_loff, row_loff, (size_t)p.nbig, tmp_sb, s);
+        int64_t tot2[5] = {0, 0, 0, 0, 0};
         TSG_HIP_TRY(hipMemcpyAsync(&tot2[0], p.sb_off + p.nbig, sizeof(int64_t),
                                    hipMemcpyDeviceToHost, s));
         TSG_HIP_TRY(hipMemcpyAsync(&tot2[1], band_off + p.nbig, sizeof(int64_t),
@@ -1407,21 +1573,36 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
                                    hipMemcpyDeviceToHost, s));
         TSG_HIP_TRY(hipMemcpyAsync(&tot2[3], a_prod + p.nbig, sizeof(int64_t),
                                    hipMemcpyDeviceToHost, s));
+        TSG_HIP_TRY(hipMemcpyAsync(&tot2[4], row_loff + p.nbig, sizeof(int64_t),
+                                   hipMemcpyDeviceToHost, s));
         TSG_HIP_TRY(hipStreamSynchronize(s));
         const int64_t nsb_total = tot2[0], nunits = tot2[1];
         p.nsb_total = nsb_total;
         p.subcnt = ws.alloc_n<uint16_t>((size_t)nsb_total + 1);
-        // keep every big row's span bitmap for the rank-indexed numeric phase
-        // when it fits a budget of device memory; else the numeric phase uses
-        // per-item hash tables
-        const size_t bits_bytes = (size_t)nsb_total * ((1u << SUB_SHIFT) / 8);
+        // keep every big row's bitmap in sparse form (masks of non-empty
+        // words per sub-block, and band word lists: each row's region sized
+        // by the bound min(products, span words) plus its bands' padding) for
+        // the rank-indexed numeric phase when they fit a budget of device
+        // memory; else the numeric phase uses per-item hash tables
+        p.words_cap = tot2[4];
+        const size_t cols_bytes =
+            ((size_t)p.words_cap + (size_t)nsb_total) * sizeof(unsigned);
         size_t free_b = 0, total_b = 0;
         if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) { (void)hipGetLastError(); free_b = 0; }
         const size_t budget = tune.bitmap_budget >= 0
                                   ? (size_t)tune.bitmap_budget
                                   : std::min<size_t>(size_t(48) << 30, free_b / 10 * 4);
-        p.use_rank = bits_bytes <= budget;
-        p.rowbits = p.use_rank ? ws.alloc_n<unsigned>(bits_bytes / 4 + 1) : nullptr;
+        p.use_rank = cols_bytes <= budget && nsb_total < (int64_t(1) << 31);
+        p.bandwords = p.use_rank ? ws.alloc_n<unsigned>((size_t)p.words_cap + 1) : nullptr;
+        p.subocc = p.use_rank ? ws.alloc_n<unsigned>((size_t)nsb_total + 1) : nullptr;
+        if (p.use_rank) {
+            band_base = ws.alloc_n<int64_t>((size_t)nunits + 1);
+            row_cur = ws.alloc_n<unsigned long long>((size_t)p.nbig + 1);
+            d_overflow = ws.alloc_n<unsigned>(1);
+            TSG_HIP_TRY(hipMemcpyAsync(row_cur, row_loff, sizeof(int64_t) * (size_t)p.nbig,
+                                       hipMemcpyDeviceToDevice, s));
+            TSG_HIP_TRY(hipMemsetAsync(d_overflow, 0, sizeof(unsigned), s));
+        }
         // B as {column, value} records for the rank kernel's gathers
         p.bpk = p.use_rank ? ws.alloc_n<int4>((size_t)B.nnz + 1) : nullptr;
         // Spans this wide (at least item_sort_min windows of 16 sub-blocks)
@@ -1499,7 +1680,8 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
                            dim3((unsigned)std::min<int64_t>(nunits, int64_t(1) << 30)),
                            dim3(BAND_T), 0, s, nunits, units, rowsb, k, A.ptr, A.idx, B.ptr,
                            B.idx, block_ptrs(p, B.ptr), p.span_sb0, p.sb_off, p.subcnt, p.rownnz,
-                           p.rowbits);
+                           band_off, band_base, row_cur, row_loff, d_overflow, p.subocc,
+                           p.bandwords);
         TSG_LAUNCH_CHECK();
         tm.end();
     }
@@ -1520,29 +1702,38 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
         // windows otherwise (fewer items, fewer (A entry, item) pairs).
         const bool narrow = p.column_order;
         p.narrow_items = narrow;
-        auto items_kernel = [&](auto kern, int64_t *cnt, int4 *it, int64_t *off, int *itn) {
+        auto items_kernel = [&](auto kern, int64_t *cnt, int4 *it, int64_t *off, int *itn,
+                                int64_t *iwords) {
             hipLaunchKernelGGL(kern, dim3(grid_for((size_t)p.nbig * 64, 256)), dim3(256), 0, s,
                                p.nbig, rowsb, k, p.span_sb0, p.sb_off, p.subcnt, d_ic, cnt, it,
-                               off, itn);
+                               off, itn, p.subocc, band_off, band_base, iwords);
             TSG_LAUNCH_CHECK();
         };
         if (narrow)
-            items_kernel(k_items<false, 8>, p.item_cnt, nullptr, nullptr, nullptr);
+            items_kernel(k_items<false, 8>, p.item_cnt, nullptr, nullptr, nullptr, nullptr);
         else
-            items_kernel(k_items<false, 16>, p.item_cnt, nullptr, nullptr, nullptr);
+            items_kernel(k_items<false, 16>, p.item_cnt, nullptr, nullptr, nullptr, nullptr);
         exclusive_scan_i64(p.item_cnt, p.item_cnt, (size_t)p.nbig, tmp2, s);
+        unsigned h_overflow = 0;
         TSG_HIP_TRY(hipMemcpyAsync(&h_tot[1], p.item_cnt + p.nbig, sizeof(int64_t),
                                    hipMemcpyDeviceToHost, s));
         TSG_HIP_TRY(hipMemcpyAsync(&h_tot[0], d_ic + m, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        if (d_overflow)
+            TSG_HIP_TRY(hipMemcpyAsync(&h_overflow, d_overflow, sizeof(h_overflow),
+                                       hipMemcpyDeviceToHost, s));
         TSG_HIP_TRY(hipStreamSynchronize(s));
+        // the lists are sized by a bound on the outputs, so this holds unless
+        // the symbolic phase itself went wrong
+        TSG_CHECK(h_overflow == 0, TSG_ERR_OVERFLOW, "big-row word lists exceed their bound");
         p.nitems = h_tot[1];
         p.items = ws.alloc_n<int4>((size_t)p.nitems + 1);
         p.item_off = ws.alloc_n<int64_t>((size_t)p.nitems + 1);
+        p.item_words = p.use_rank ? ws.alloc_n<int64_t>((size_t)p.nitems + 1) : nullptr;
         int *item_n = ws.alloc_n<int>((size_t)p.nitems + 1);
         if (narrow)
-            items_kernel(k_items<true, 8>, p.item_cnt, p.items, p.item_off, item_n);
+            items_kernel(k_items<true, 8>, p.item_cnt, p.items, p.item_off, item_n, p.item_words);
         else
-            items_kernel(k_items<true, 16>, p.item_cnt, p.items, p.item_off, item_n);
+            items_kernel(k_items<true, 16>, p.item_cnt, p.items, p.item_off, item_n, p.item_words);
         p.nitems_small = 0;
         p.nitems_mid = 0;
         if (p.column_order && p.nitems > 1 && p.nitems < (int64_t(1) << 31)) {
@@ -1565,11 +1756,14 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
             sort_order_i32(keys, p.nitems, p.use_rank ? end_bit + 2 : end_bit, order, ws, s);
             int4 *items_s = ws.alloc_n<int4>(ni + 1);
             int64_t *off_s = ws.alloc_n<int64_t>(ni + 1);
+            int64_t *words_s = p.item_words ? ws.alloc_n<int64_t>(ni + 1) : nullptr;
             hipLaunchKernelGGL(k_gather_items, dim3(grid_for(ni, 256)), dim3(256), 0, s,
-                               p.nitems, order, p.items, p.item_off, items_s, off_s);
+                               p.nitems, order, p.items, p.item_off, p.item_words, items_s, off_s,
+                               words_s);
             TSG_LAUNCH_CHECK();
             p.items = items_s;
             p.item_off = off_s;
+            p.item_words = words_s;
             if (p.use_rank) {
                 int64_t h_cls[2] = {0, 0};
                 TSG_HIP_TRY(hipMemcpyAsync(h_cls, d_small, sizeof(h_cls), hipMemcpyDeviceToHost,
@@ -1646,8 +1840,8 @@ static void row_numeric_t(RowPlan &p, const CsrIn &A, const CsrIn &B, const int6
             set_smem(kern, sm);
             const unsigned grid = item_grid(i1 - i0);
             hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * ITEM_WAVES), sm, s, i1 - i0,
-                               p.items + i0, p.item_off + i0, p.rowbits, p.span_sb0, p.sb_off,
-                               A.ptr, A.idx, A.val, B.idx, B.val, p.bpk, block_ptrs(p, B.ptr),
+                               p.items + i0, p.item_off + i0, p.item_words + i0, p.bandwords,
+                               p.subocc, A.ptr, A.idx, A.val, B.idx, B.val, p.bpk, block_ptrs(p, B.ptr),
                                d_jc, d_c, (int)p.b_strict);
             TSG_LAUNCH_CHECK();
         };
