@@ -236,7 +236,8 @@ struct TilePlan {
     int64_t *rownnz = nullptr;   // [m+1] CSR row counts
     int64_t nnzC = 0;
     // matched (A tile, B tile) pairs of every C tile in ascending K
-    int64_t *tpair = nullptr;    // [numblkC+1] offsets into pairs
+    int64_t *tpair = nullptr;    // [numblkC+1] offsets into pairs, or counts with tslot
+    int64_t *tslot = nullptr;    // [numblkC] first pair of each C tile (pairs in slots)
     int2 *pairs = nullptr;       // [npairs] {A tile position, B tile position}
     int64_t npairs = 0;
     // numeric work units (tile * 4 + part) by C tile output count: bins 0..3

@@ -418,6 +418,10 @@ __device__ __forceinline__ void for_pairs(bool valid, int pa0, int pa1, int cb0,
 // ------------------------------------------------------------------ step2
 constexpr int S2_WAVES = 4;
 
+// With pslot (slots of C tile row I from pslot[I], each of its C tiles
+// taking as many as the row has A tiles -- a bound on its matched pairs),
+// the matched pairs are stored as they are found, at tslot[t], and npair[t]
+// counts them; without, npair[t] counts them for k_fill_pairs.
 __global__ void __launch_bounds__(64 * S2_WAVES) k_step2(
     int m, int blkm, const int *__restrict__ tptrA, const int *__restrict__ tcolA,
     const uint16_t *__restrict__ maskA, const int *__restrict__ csc_ptr,
@@ -425,7 +429,8 @@ __global__ void __launch_bounds__(64 * S2_WAVES) k_step2(
     const uint16_t *__restrict__ maskB, const int *__restrict__ tptrC,
     const int *__restrict__ tcolC, uint16_t *__restrict__ maskC, uint8_t *__restrict__ rowptrC,
     int64_t *__restrict__ tnnz, int *__restrict__ rowoffC, int64_t *__restrict__ rownnz,
-    int64_t *__restrict__ npair) {
+    int64_t *__restrict__ npair, const int64_t *__restrict__ pslot,
+    int64_t *__restrict__ tslot, int2 *__restrict__ pairs) {
     __shared__ unsigned s_words[S2_WAVES][RT_WORDS];
     __shared__ uint16_t s_pre[S2_WAVES][RT_WORDS];
     const int I = blockIdx.x * S2_WAVES + (threadIdx.x >> 6);
@@ -436,6 +441,7 @@ __global__ void __launch_bounds__(64 * S2_WAVES) k_step2(
     const int t0 = tptrC[I], t1 = tptrC[I + 1];
     RowTiles rt{s_words[threadIdx.x >> 6], s_pre[threadIdx.x >> 6]};
     rt.build(tcolA, pa0, pa1);
+    const int64_t slot0 = pslot ? pslot[I] : 0;
     int running = 0;
     for (int base = t0; base < t1; base += 4) {
         const int t = base + g;
@@ -444,7 +450,9 @@ __global__ void __launch_bounds__(64 * S2_WAVES) k_step2(
         const int cb0 = valid ? csc_ptr[J] : 0, cb1 = valid ? csc_ptr[J + 1] : 0;
         unsigned mc = 0;
         int np = 0;
+        const int64_t slot = slot0 + (int64_t)(t - t0) * (pa1 - pa0);
         for_pairs(valid, pa0, pa1, cb0, cb1, tcolA, csc_row, csc_tile, [&](int pa, int pb) {
+            if (pslot && r == 0) pairs[slot + np] = make_int2(pa, pb);
             np++;
             // B_KJ's 16 row masks in registers (the group's lanes read the
             // same 32 bytes), then the rows selected by A_IK's row r
@@ -463,6 +471,7 @@ __global__ void __launch_bounds__(64 * S2_WAVES) k_step2(
             rowptrC[(size_t)t * 16 + r] = (uint8_t)(incl - pc);
             if (r == 15) tnnz[t] = incl;
             if (r == 0) npair[t] = np;
+            if (pslot && r == 1) tslot[t] = slot;
         }
         int pre = 0, tot = 0;
 #pragma unroll
@@ -476,6 +485,14 @@ __global__ void __launch_bounds__(64 * S2_WAVES) k_step2(
     }
     const int row = I * 16 + r;
     if (g == 0 && row < m) rownnz[row] = running;
+}
+
+// Pair slots of C tile row I: its C tiles times its A tiles.
+__global__ void __launch_bounds__(256) k_pair_slots(int blkm, const int *__restrict__ tptrA,
+                                                    const int *__restrict__ cntC,
+                                                    int64_t *__restrict__ slots) {
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    if (I < blkm) slots[I] = (int64_t)cntC[I] * (tptrA[I + 1] - tptrA[I]);
 }
 
 // ------------------------------------------------------------- pairs, bins
@@ -561,8 +578,11 @@ __global__ void __launch_bounds__(256) k_tbin_count(int n, const int64_t *__rest
 // atomic per tile on the counters serialised the scatter).  Bin 2 takes a
 // tile's parts 0, 1, ... (the remainder last, when it is there); bins 0 and
 // 1 take only a remainder, part n / 64.
+// The tile's pairs: from tslot[t] with npair[t] of them when step2 stored
+// them in slots, else from the scanned counts tpair.
 __global__ void __launch_bounds__(256) k_tbin_scatter(int n, const int64_t *__restrict__ off,
                                                       const int64_t *__restrict__ tpair,
+                                                      const int64_t *__restrict__ tslot,
                                                       const int *__restrict__ cnt,
                                                       int *__restrict__ fill,
                                                       int4 *__restrict__ units) {
@@ -595,8 +615,8 @@ __global__ void __launch_bounds__(256) k_tbin_scatter(int n, const int64_t *__re
     const int rb = rem_bin(nz);
     const int u2 = tile_units_in(nz, 2);
     if (u2 == 0 && rb < 0) return;
-    const int64_t p0 = tpair[t];
-    const int np = (int)(tpair[t + 1] - p0);
+    const int64_t p0 = tslot ? tslot[t] : tpair[t];
+    const int np = (int)(tslot ? tpair[t] : tpair[t + 1] - p0);
     auto rec = [&](int part) {
         return make_int4(t, part | (np << 2), (int)(uint32_t)(uint64_t)p0,
                          (int)(uint32_t)((uint64_t)p0 >> 32));
@@ -993,10 +1013,24 @@ void tile_symbolic(TilePlan &p, const CsrIn &A, const CsrIn &B, bool b_is_a, int
     // sized from the count
     int64_t *d_nC = ws.alloc_n<int64_t>(1);
     exclusive_scan_i32(cnt, p.tptrC, (size_t)blkm, tmp, s, d_nC);
-    int64_t nC64 = 0;
+    // pair slots per C tile row (its C tiles times its A tiles), so step2
+    // can store the matched pairs as it finds them
+    int64_t *pslot = ws.alloc_n<int64_t>((size_t)blkm + 1);
+    if (blkm > 0) {
+        hipLaunchKernelGGL(k_pair_slots, dim3(grid_for((size_t)blkm, 256)), dim3(256), 0, s, blkm,
+                           p.A.tile_ptr, cnt, pslot);
+        TSG_LAUNCH_CHECK();
+    }
+    exclusive_scan_i64(pslot, pslot, (size_t)blkm, tmp, s);
+    int64_t nC64 = 0, nslots = 0;
     TSG_HIP_TRY(hipMemcpyAsync(&nC64, d_nC, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    TSG_HIP_TRY(hipMemcpyAsync(&nslots, pslot + blkm, sizeof(int64_t), hipMemcpyDeviceToHost, s));
     TSG_HIP_TRY(hipStreamSynchronize(s));
     check_tile_count(nC64);
+    // slots when they stay within a few times the C tiles (a tile row of
+    // many A tiles whose C tiles match few of them -- hypersparse inputs --
+    // would make them too many); else a second pass stores the pairs
+    const bool direct_pairs = nslots <= 32 * nC64 + (int64_t(1) << 20);
     const int nC = (int)nC64;
     p.numblkC = nC;
     p.tcolC = ws.alloc_n<int>((size_t)nC + 1);
@@ -1015,18 +1049,24 @@ void tile_symbolic(TilePlan &p, const CsrIn &A, const CsrIn &B, bool b_is_a, int
     p.offC = ws.alloc_n<int64_t>((size_t)nC + 1);
     p.rowoffC = ws.alloc_n<int>((size_t)nC * 16 + 16);
     p.tpair = ws.alloc_n<int64_t>((size_t)nC + 1);
+    p.tslot = direct_pairs ? ws.alloc_n<int64_t>((size_t)nC + 1) : nullptr;
+    if (direct_pairs) {
+        p.npairs = nslots;
+        p.pairs = ws.alloc_n<int2>((size_t)nslots + 1);
+    }
     p.rownnz = ws.alloc_n<int64_t>((size_t)m + 1);
     TSG_HIP_TRY(hipMemsetAsync(p.rownnz, 0, sizeof(int64_t) * ((size_t)m + 1), s));
     if (blkm > 0) {
         hipLaunchKernelGGL(k_step2, dim3(grid_for(blkm, S2_WAVES)), dim3(64 * S2_WAVES), 0, s, m,
                            blkm, p.A.tile_ptr, p.A.tile_col, p.A.mask, TB.csc_ptr, TB.csc_row,
                            TB.csc_tile, TB.mask, p.tptrC, p.tcolC, p.maskC, p.rowptrC, p.offC,
-                           p.rowoffC, p.rownnz, p.tpair);
+                           p.rowoffC, p.rownnz, p.tpair, direct_pairs ? pslot : nullptr, p.tslot,
+                           p.pairs);
         TSG_LAUNCH_CHECK();
     }
     int64_t *tmpC = ws.alloc_n<int64_t>(scan_tmp_elems((size_t)nC + 1));
     exclusive_scan_i64(p.offC, p.offC, (size_t)nC, tmpC, s);
-    exclusive_scan_i64(p.tpair, p.tpair, (size_t)nC, tmpC, s);
+    if (!direct_pairs) exclusive_scan_i64(p.tpair, p.tpair, (size_t)nC, tmpC, s);
     int64_t *tmpm = ws.alloc_n<int64_t>(scan_tmp_elems((size_t)m + 1));
     exclusive_scan_i64(p.rownnz, d_ic, (size_t)m, tmpm, s);
     // C tiles by output count for the numeric kernels
@@ -1042,7 +1082,9 @@ void tile_symbolic(TilePlan &p, const CsrIn &A, const CsrIn &B, bool b_is_a, int
     int64_t h64[2] = {0, 0};
     int hb[TBINS] = {0, 0, 0};
     TSG_HIP_TRY(hipMemcpyAsync(&h64[0], d_ic + m, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-    TSG_HIP_TRY(hipMemcpyAsync(&h64[1], p.tpair + nC, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    if (!direct_pairs)
+        TSG_HIP_TRY(hipMemcpyAsync(&h64[1], p.tpair + nC, sizeof(int64_t), hipMemcpyDeviceToHost,
+                                   s));
     TSG_HIP_TRY(hipMemcpyAsync(hb, bcnt, sizeof(hb), hipMemcpyDeviceToHost, s));
     TSG_HIP_TRY(hipStreamSynchronize(s));
     // the numeric phase's work units, sized by the counts just read
@@ -1052,14 +1094,16 @@ void tile_symbolic(TilePlan &p, const CsrIn &A, const CsrIn &B, bool b_is_a, int
     tm.begin("tile.pairs");
     if (nunits > 0) {
         hipLaunchKernelGGL(k_tbin_scatter, dim3(grid_for((size_t)nC, 256)), dim3(256), 0, s, nC,
-                           p.offC, p.tpair, bcnt, bcnt + TBINS, p.tbin_units);
+                           p.offC, p.tpair, p.tslot, bcnt, bcnt + TBINS, p.tbin_units);
         TSG_LAUNCH_CHECK();
     }
-    // the matched pairs themselves (the numeric phase reads them instead of
-    // intersecting tile lists again)
-    p.npairs = h64[1];
-    p.pairs = ws.alloc_n<int2>((size_t)p.npairs + 1);
-    if (blkm > 0 && p.npairs > 0) {
+    // the matched pairs themselves, unless step2 stored them (the numeric
+    // phase reads them instead of intersecting tile lists again)
+    if (!direct_pairs) {
+        p.npairs = h64[1];
+        p.pairs = ws.alloc_n<int2>((size_t)p.npairs + 1);
+    }
+    if (!direct_pairs && blkm > 0 && p.npairs > 0) {
         hipLaunchKernelGGL(k_fill_pairs, dim3(grid_for(blkm, FP_WAVES)), dim3(64 * FP_WAVES), 0,
                            s, blkm, p.A.tile_ptr, p.A.tile_col, TB.csc_ptr, TB.csc_row,
                            TB.csc_tile, p.tptrC, p.tcolC, p.tpair, p.pairs);
