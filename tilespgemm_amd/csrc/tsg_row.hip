@@ -857,31 +857,57 @@ __global__ void __launch_bounds__(256) k_bptr_rows(int n, const int *__restrict_
 // >= b << bs.  A block of 4 waves takes 64 B rows; for each range of 64 blocks
 // the waves fill an LDS tile [block][row], a lane per block searching inside
 // its row, then write it out as 64 consecutive rows per block (256-byte runs).
-// The 16.6 KB tile keeps 8 blocks per CU resident; the searches are latency
-// bound, so residency decides the time (a 128-block tile took 6.2 ms instead
-// of 4.5 ms for R-MAT 2^20's 16,385 x 2^20 table, a 32-block one 7.1 ms).
-constexpr int BPT_ROWS = 64, BPT_BLK = 64;
+// The 64 rows' columns are one contiguous range of jb: when it holds at most
+// BPT_STAGE entries (R-MAT 2^20's B averages ~940 per 64 rows) it is staged
+// in LDS once and every search runs there; otherwise the searches read jb.
+// The searches wait on their loads, so residency mattered with global
+// searches (a 128-block tile took 6.2 ms instead of 4.5 ms for R-MAT 2^20's
+// 16,385 x 2^20 table, a 32-block one 7.1 ms).
+constexpr int BPT_ROWS = 64, BPT_BLK = 64, BPT_STAGE = 2048;
+
+__device__ __forceinline__ int lds_lower_bound(const int *a, int lo, int hi, int key) {
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
 __global__ void __launch_bounds__(256) k_bptr(int n, const int *__restrict__ ib,
                                               const int *__restrict__ jb, int nblk, int bs,
                                               int *__restrict__ bptr) {
     __shared__ int tile[BPT_BLK][BPT_ROWS + 1];
+    __shared__ int s_col[BPT_STAGE];
+    __shared__ int s_rp[BPT_ROWS + 1];
     const int t0 = blockIdx.x * BPT_ROWS;
+    const int nr = min(BPT_ROWS, n - t0);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if ((int)threadIdx.x <= nr) s_rp[threadIdx.x] = ib[t0 + threadIdx.x];
+    __syncthreads();
+    const int e0 = s_rp[0];
+    const bool staged = s_rp[nr] - e0 <= BPT_STAGE;
+    if (staged)
+        for (int i = threadIdx.x; i < s_rp[nr] - e0; i += blockDim.x) s_col[i] = jb[e0 + i];
+    __syncthreads();
     for (int b0 = 0; b0 <= nblk; b0 += BPT_BLK) {
-        for (int r = wave; r < BPT_ROWS; r += 4) {
-            const int t = t0 + r;
-            if (t >= n) break;
-            const int lo = ib[t], hi = ib[t + 1];
+        for (int r = wave; r < nr; r += 4) {
+            const int lo = s_rp[r], hi = s_rp[r + 1];
             for (int bb = lane; bb < BPT_BLK; bb += 64) {
                 const int b = b0 + bb;
-                if (b <= nblk) tile[bb][r] = b == nblk ? hi : lower_bound_i32(jb, lo, hi, b << bs);
+                if (b > nblk) continue;
+                int pos = hi;
+                if (b < nblk)
+                    pos = staged ? e0 + lds_lower_bound(s_col, lo - e0, hi - e0, b << bs)
+                                 : lower_bound_i32(jb, lo, hi, b << bs);
+                tile[bb][r] = pos;
             }
         }
         __syncthreads();
         for (int bb = wave; bb < BPT_BLK; bb += 4) {
             const int b = b0 + bb;
             const int t = t0 + lane;
-            if (b <= nblk && t < n) bptr[(size_t)b * n + t] = tile[bb][lane];
+            if (b <= nblk && lane < nr) bptr[(size_t)b * n + t] = tile[bb][lane];
         }
         __syncthreads();
     }
