@@ -170,6 +170,59 @@ def test_big_row_item_size_classes(handle, fp):
     assert_csr_equal(got, want)
 
 
+def _tight_bound_case():
+    """Big rows whose stored bitmaps fill their reserved space exactly: row 0
+    is dense over 201,600 columns (three full 65536-column bands, every
+    bitmap word non-empty, and a partial fourth band; k ends 16 columns past
+    it), row 1 has 1,100 products on 1,100 columns 181 apart (one per word:
+    as many non-empty words as products), row 2 both at once, then small
+    rows."""
+    rng = np.random.default_rng(31)
+    k = 3 * 65536 + 5000 + 8
+    n = 1300
+    seg = 1008
+    brows, bcols = [], []
+    for i in range(200):                       # B rows 0..199: dense segments
+        brows.append(np.full(seg, i))
+        bcols.append(np.arange(i * seg, (i + 1) * seg))
+    sparse_cols = (np.arange(1100) * 181) % k
+    for j in range(1100):                      # B rows 200..1299: one entry each
+        brows.append(np.array([200 + j]))
+        bcols.append(np.array([sparse_cols[j]]))
+    brows = np.concatenate(brows)
+    bcols = np.concatenate(bcols)
+    B = gen.from_coo(n, brows, bcols, rng.uniform(-1, 1, brows.size))
+    arows = [np.zeros(200, int), np.ones(1100, int), np.full(1300, 2)]
+    acols = [np.arange(200), np.arange(200, 1300), np.arange(1300)]
+    for r in range(3, 51):
+        d = int(rng.integers(1, 6))
+        arows.append(np.full(d, r))
+        acols.append(rng.choice(n, d, replace=False))
+    arows = np.concatenate(arows)
+    acols = np.concatenate(acols)
+    m = 51
+    A = gen.from_coo(m, arows, acols, rng.uniform(-1, 1, arows.size))
+    return m, n, k, A, B
+
+
+@pytest.mark.parametrize("order", [0, 1 << 18])
+def test_big_rows_bitmaps_fill_their_bound(handle, order):
+    """Stored big-row bitmaps (non-empty words per band, masks per
+    sub-block) that fill the space reserved for them exactly, in rows' order
+    (16384-column items) and column order (8192-column items in three size
+    classes): the oracle's bits."""
+    m, n, k, A, B = _tight_bound_case()
+    handle.set_item_order(order)
+    try:
+        got = run(handle, m, n, k, A, B, tsg.METHOD_ROW)
+    finally:
+        handle.set_item_order(1 << 18)
+    want = O.csr_spgemm(m, k, A, B, fused=True)
+    print(f"[tight bound order={order}] nnzC={want[0][-1]}", flush=True)
+    assert want[0][1] == 201600 and want[0][2] - want[0][1] == 1100
+    assert_csr_equal(got, want)
+
+
 @pytest.mark.parametrize("fp", [tsg.FP_FUSED, tsg.FP_SEPARATE])
 def test_big_rows_hash_fallback(handle, fp):
     """No memory for span bitmaps: the big-row numeric phase keeps each
