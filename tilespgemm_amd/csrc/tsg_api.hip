@@ -122,19 +122,29 @@ void symbolic_into(tsg_handle_t h, RowPlan &row, TilePlan &tile, int &method_use
                    const int *d_jb, int64_t nnz_b, int64_t *d_ic, int64_t *nnz_c) {
     CsrIn A{m, n, d_ia, d_ja, nullptr, nnz_a};
     CsrIn B{n, k, d_ib, d_jb, nullptr, nnz_b};
-    // with AUTO, A's tile-density sample shares the validation's round trip
+    // with AUTO, A's tile-density sample shares the validation's round trip;
+    // with the tile path forced, the tile counts of A and B do
     unsigned long long sample[2] = {0, 0};
     const bool want_sample = h->method == TSG_METHOD_AUTO;
-    const int vf = validate_csr_pair(A, B, ws, h->stream, want_sample ? sample : nullptr);
+    const bool b_is_a = d_ia == d_ib && d_ja == d_jb && m == n && n == k;
+    EarlyTileCount earlyA, earlyB;
+    std::function<void()> early;
+    if (h->method == TSG_METHOD_TILE)
+        early = [&] {
+            tile_count_early(A, earlyA, ws, h->stream);
+            if (!b_is_a) tile_count_early(B, earlyB, ws, h->stream);
+        };
+    const int vf =
+        validate_csr_pair(A, B, ws, h->stream, want_sample ? sample : nullptr, early);
     TSG_CHECK((vf & 0x303) == 0, TSG_ERR_INVALID,
               std::string("invalid CSR input: ") + ((vf & 0x3) ? "A" : "B") +
                   ((vf & 0x101) ? " has bad row pointers" : " has a column out of range"));
     int method = choose_method(h, A, (vf & 0x404) == 0, want_sample ? sample : nullptr, ws);
     method_used = method;
     if (method == TSG_METHOD_TILE) {
-        const bool b_is_a = d_ia == d_ib && d_ja == d_jb && m == n && n == k;
         try {
-            tile_symbolic(tile, A, B, b_is_a, d_ic, nnz_c, ws, h->stream, tm, h->cus);
+            tile_symbolic(tile, A, B, b_is_a, d_ic, nnz_c, ws, h->stream, tm, h->cus, &earlyA,
+                          &earlyB);
         } catch (const TsgError &e) {
             // AUTO's choice falls back to the row path when the tile path's
             // int32 tile indices would overflow or its structures do not fit

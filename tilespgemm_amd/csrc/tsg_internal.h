@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stddef.h>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -249,9 +250,19 @@ struct TilePlan {
 
 // Throws TSG_ERR_OVERFLOW unless 0 <= n_c_tiles <= INT32_MAX.
 void check_tile_count(int64_t n_c_tiles);
+// A matrix's tile count per tile row, scanned, enqueued ahead of the tile
+// path (beside the input's validation, whose host round trip then brings
+// the total back into nt): the count clamps every index, so it is safe on
+// unchecked input.  tile_ptr and nt are used only once the input is valid.
+struct EarlyTileCount {
+    int *tile_ptr = nullptr;     // [blkm+1] device
+    int nt = -1;                 // host, after the round trip
+};
+void tile_count_early(const CsrIn &X, EarlyTileCount &e, Arena &ws, hipStream_t s);
 void tile_symbolic(TilePlan &p, const CsrIn &A, const CsrIn &B, bool b_is_a,
                    int64_t *d_ic, int64_t *nnz_c, Arena &ws, hipStream_t s,
-                   StageTimer &tm, int cus);
+                   StageTimer &tm, int cus, const EarlyTileCount *earlyA = nullptr,
+                   const EarlyTileCount *earlyB = nullptr);
 void tile_numeric(TilePlan &p, const CsrIn &A, const CsrIn &B, const int64_t *d_ic,
                   int *d_jc, double *d_c, int fused, Arena &ws, hipStream_t s,
                   StageTimer &tm, int cus);
@@ -302,15 +313,15 @@ void batch_numeric(tsg_handle_t h, int count, const tsg_product_t *prods, const 
 void batch_rebase(tsg_handle_t h, const int64_t *d_ic, int64_t *d_out);
 
 // CSR utilities
-// Validates A and B on the device (one sync).  Returns flags of A in bits
-// 0-7 and of B in bits 8-15: 1 bad row pointers, 2 column out of range,
-// 4 a row not strictly ascending.
-// Validation of A and B (flags: A in bits 0-7, B in bits 8-15; 1 bad row
-// pointers, 2 column out of range, 4 not strictly ascending).  With
-// density_sample, A's tile-density sample (tile_sample_launch) runs beside
-// the checks and its {tiles, entries} come back in the same host round trip.
+// Validation of A and B on the device, one host round trip (flags: A in
+// bits 0-7, B in bits 8-15; 1 bad row pointers, 2 column out of range, 4 not
+// strictly ascending).  With density_sample, A's tile-density sample
+// (tile_sample_launch) runs beside the checks and its {tiles, entries} come
+// back in the same round trip; before_sync enqueues more work (and read-
+// backs) ahead of that round trip.
 int validate_csr_pair(const CsrIn &A, const CsrIn &B, Arena &ws, hipStream_t s,
-                      unsigned long long *density_sample = nullptr);
+                      unsigned long long *density_sample = nullptr,
+                      const std::function<void()> &before_sync = {});
 // entries per non-empty 16x16 tile of X (tsg_tile.hip; a sample decides when
 // clear of the threshold)
 void tile_sample_launch(const CsrIn &X, unsigned long long *sums, hipStream_t s);

@@ -52,57 +52,66 @@ __device__ __forceinline__ int group_min(int x) {
 __device__ __forceinline__ int wave_min(int x) { return group_min<64>(x); }
 
 // ---------------------------------------------------------------- convert
-// One 16-lane group per tile row; lane r handles CSR row 16*I + r.
-template <bool FILL>
+// One 16-lane group per tile row; lane r handles CSR row 16*I + r.  A
+// block's 16 tile rows hold one contiguous range of ja: when it has at most
+// CV_STAGE entries it is staged in LDS first (coalesced loads), and the rows
+// are walked there instead of with a dependent global load per entry (the
+// 27-point stencil's 6,912 entries per block fit).
+constexpr int CV_STAGE = 8192;
+
 __global__ void __launch_bounds__(256) k_tile_convert(
     int m, int blkm, const int *__restrict__ ia, const int *__restrict__ ja,
-    int *__restrict__ tcount, const int *__restrict__ tile_ptr, int *__restrict__ tile_col,
-    int *__restrict__ tile_off, uint16_t *__restrict__ mask, uint8_t *__restrict__ rowptr,
-    uint8_t *__restrict__ idx, int *__restrict__ perm) {
+    const int *__restrict__ tile_ptr, int *__restrict__ tile_col, int *__restrict__ tile_off,
+    uint16_t *__restrict__ mask, uint8_t *__restrict__ rowptr, uint8_t *__restrict__ idx,
+    int *__restrict__ perm) {
+    __shared__ int s_col[CV_STAGE];
+    const int I0 = blockIdx.x * (blockDim.x >> 4);
+    const int e0 = ia[I0 * 16];
+    const int e1 = ia[min(m, (I0 + (int)(blockDim.x >> 4)) * 16)];
+    const bool staged = e1 - e0 <= CV_STAGE;
+    if (staged)
+        for (int i = threadIdx.x; i < e1 - e0; i += blockDim.x) s_col[i] = ja[e0 + i];
+    __syncthreads();
+    auto col_at = [&](int p) { return staged ? s_col[p - e0] : ja[p]; };
     const int I = (blockIdx.x * blockDim.x + threadIdx.x) >> 4;
     const int r = threadIdx.x & 15;
     if (I >= blkm) return;
     const int row = I * 16 + r;
     int cur = 0, end = 0;
     if (row < m) { cur = ia[row]; end = ia[row + 1]; }
-    int t = FILL ? tile_ptr[I] : 0;
-    int off = FILL ? ia[I * 16] : 0;
-    int ntiles = 0;
+    int t = tile_ptr[I];
+    int off = ia[I * 16];
     while (true) {
-        const int Jr = cur < end ? (ja[cur] >> 4) : INT_MAX;
+        const int Jr = cur < end ? (col_at(cur) >> 4) : INT_MAX;
         const int J = group_min<16>(Jr);
         if (J == INT_MAX) break;
         const int c0 = cur;
         unsigned mk = 0;
         while (cur < end) {
-            const int col = ja[cur];
+            const int col = col_at(cur);
             if ((col >> 4) != J) break;
             mk |= 1u << (15 - (col & 15));
             cur++;
         }
-        if (FILL) {
-            const int cnt = cur - c0;
-            const int incl = group_incl_scan<16>(cnt);
-            const int tot = __shfl(incl, 15, 16);
-            const int rp = incl - cnt;
-            rowptr[(size_t)t * 16 + r] = (uint8_t)rp;
-            mask[(size_t)t * 16 + r] = (uint16_t)mk;
-            if (r == 0) {
-                tile_col[t] = J;
-                tile_off[t] = off;
-            }
-            for (int i = 0; i < cnt; i++) {
-                const int q = off + rp + i;
-                idx[q] = (uint8_t)((r << 4) | (ja[c0 + i] & 15));
-                perm[c0 + i] = q;
-            }
-            off += tot;
-            t++;
+        const int cnt = cur - c0;
+        const int incl = group_incl_scan<16>(cnt);
+        const int tot = __shfl(incl, 15, 16);
+        const int rp = incl - cnt;
+        rowptr[(size_t)t * 16 + r] = (uint8_t)rp;
+        mask[(size_t)t * 16 + r] = (uint16_t)mk;
+        if (r == 0) {
+            tile_col[t] = J;
+            tile_off[t] = off;
         }
-        ntiles++;
+        for (int i = 0; i < cnt; i++) {
+            const int q = off + rp + i;
+            idx[q] = (uint8_t)((r << 4) | (col_at(c0 + i) & 15));
+            perm[c0 + i] = q;
+        }
+        off += tot;
+        t++;
     }
-    if (!FILL && r == 0) tcount[I] = ntiles;
-    if (FILL && r == 0 && I == blkm - 1) tile_off[t] = off;
+    if (r == 0 && I == blkm - 1) tile_off[t] = off;
 }
 
 // Tile count of each tile row through an LDS bitmap over its tile columns
@@ -625,6 +634,31 @@ __global__ void __launch_bounds__(256) k_tbin_scatter(int n, const int64_t *__re
     if (rb == 0 || rb == 1) units[s_base[rb] + pos[rb]] = rec(full);
 }
 
+// Tile-CSC index of a square tile pattern that is symmetric (tile (K, J)
+// present exactly when (J, K) is, as in stencil and mesh matrices): column
+// J's tiles are then row J's tile columns K, in the same ascending order, and
+// each one's index is that of tile (K, J), found by a search in row K.  16
+// lanes per tile row.  A tile whose partner is missing sets *asym (the
+// general transpose then replaces the index).  csc_ptr is tile_ptr.
+__global__ void __launch_bounds__(256) k_sym_csc(int blk, const int *__restrict__ tile_ptr,
+                                                 const int *__restrict__ tile_col,
+                                                 int *__restrict__ csc_row,
+                                                 int *__restrict__ csc_tile,
+                                                 unsigned *__restrict__ asym) {
+    const int J = (blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+    const int l = threadIdx.x & 15;
+    if (J >= blk) return;
+    for (int q = tile_ptr[J] + l; q < tile_ptr[J + 1]; q += 16) {
+        const int K = tile_col[q];
+        const int lo = tile_ptr[K], hi = tile_ptr[K + 1];
+        const int pos = lower_bound_i32(tile_col, lo, hi, J);
+        const bool found = pos < hi && tile_col[pos] == J;
+        csc_row[q] = K;
+        csc_tile[q] = found ? pos : 0;
+        if (!found) atomicOr(asym, 1u);
+    }
+}
+
 // Column layout of 16x16 tiles (B only), the transposed sub-tiles of
 // main_spgemm-col.cu:5255-5290, as tile records: mask[c] has bit 15 - k set
 // when the tile holds entry (k, c) -- the transpose of the row masks, in the
@@ -899,24 +933,38 @@ __global__ void __launch_bounds__(256) k_step3_out(int nunits, const int4 *__res
     }
 }
 
-void convert(TileMat &T, const CsrIn &X, bool fill_values, Arena &ws, hipStream_t s) {
+// tile counts per tile row, scanned into tile_ptr
+void tile_count_scan(const CsrIn &X, int blkm, int *tile_ptr, Arena &ws, hipStream_t s) {
+    int *tcount = ws.alloc_n<int>((size_t)blkm + 1);
+    int64_t *tmp = ws.alloc_n<int64_t>(scan_tmp_elems((size_t)blkm + 1));
+    if (blkm > 0) {
+        hipLaunchKernelGGL(k_tile_count, dim3(grid_for(blkm, TC_WAVES)), dim3(64 * TC_WAVES), 0,
+                           s, X.rows, X.cols, X.nnz, blkm, X.ptr, X.idx, tcount);
+        TSG_LAUNCH_CHECK();
+    }
+    exclusive_scan_i32(tcount, tile_ptr, (size_t)blkm, tmp, s);
+}
+
+// early: the tile count already taken (tile_count_early), else counted here
+// with a host round trip of its own
+void convert(TileMat &T, const CsrIn &X, bool fill_values, Arena &ws, hipStream_t s,
+             const EarlyTileCount *early) {
     T.rows = X.rows;
     T.cols = X.cols;
     T.blkm = (X.rows + 15) / 16;
     T.blkn = (X.cols + 15) / 16;
     T.nnz = X.nnz;
-    int *tcount = ws.alloc_n<int>((size_t)T.blkm + 1);
-    int64_t *tmp = ws.alloc_n<int64_t>(scan_tmp_elems((size_t)T.blkm + 1));
-    T.tile_ptr = ws.alloc_n<int>((size_t)T.blkm + 1);
-    if (T.blkm > 0) {
-        hipLaunchKernelGGL(k_tile_count, dim3(grid_for(T.blkm, TC_WAVES)), dim3(64 * TC_WAVES), 0,
-                           s, X.rows, X.cols, X.nnz, T.blkm, X.ptr, X.idx, tcount);
-        TSG_LAUNCH_CHECK();
-    }
-    exclusive_scan_i32(tcount, T.tile_ptr, (size_t)T.blkm, tmp, s);
     int nt = 0;
-    TSG_HIP_TRY(hipMemcpyAsync(&nt, T.tile_ptr + T.blkm, sizeof(int), hipMemcpyDeviceToHost, s));
-    TSG_HIP_TRY(hipStreamSynchronize(s));
+    if (early && early->nt >= 0) {
+        T.tile_ptr = early->tile_ptr;
+        nt = early->nt;
+    } else {
+        T.tile_ptr = ws.alloc_n<int>((size_t)T.blkm + 1);
+        tile_count_scan(X, T.blkm, T.tile_ptr, ws, s);
+        TSG_HIP_TRY(
+            hipMemcpyAsync(&nt, T.tile_ptr + T.blkm, sizeof(int), hipMemcpyDeviceToHost, s));
+        TSG_HIP_TRY(hipStreamSynchronize(s));
+    }
     T.numblk = nt;
     T.tile_col = ws.alloc_n<int>((size_t)nt + 1);
     T.tile_off = ws.alloc_n<int>((size_t)nt + 1);
@@ -928,9 +976,9 @@ void convert(TileMat &T, const CsrIn &X, bool fill_values, Arena &ws, hipStream_
     if (nt == 0) {
         TSG_HIP_TRY(hipMemsetAsync(T.tile_off, 0, sizeof(int), s));
     } else {
-        hipLaunchKernelGGL(k_tile_convert<true>, dim3(grid_for((size_t)T.blkm * 16, 256)),
-                           dim3(256), 0, s, X.rows, T.blkm, X.ptr, X.idx, nullptr, T.tile_ptr,
-                           T.tile_col, T.tile_off, T.mask, T.rowptr, T.idx, T.perm);
+        hipLaunchKernelGGL(k_tile_convert, dim3(grid_for((size_t)T.blkm * 16, 256)), dim3(256),
+                           0, s, X.rows, T.blkm, X.ptr, X.idx, T.tile_ptr, T.tile_col,
+                           T.tile_off, T.mask, T.rowptr, T.idx, T.perm);
         TSG_LAUNCH_CHECK();
     }
     if (fill_values && X.val && X.nnz > 0) {
@@ -942,6 +990,14 @@ void convert(TileMat &T, const CsrIn &X, bool fill_values, Arena &ws, hipStream_
 
 }  // namespace
 
+void tile_count_early(const CsrIn &X, EarlyTileCount &e, Arena &ws, hipStream_t s) {
+    const int blkm = (X.rows + 15) / 16;
+    e.tile_ptr = ws.alloc_n<int>((size_t)blkm + 1);
+    tile_count_scan(X, blkm, e.tile_ptr, ws, s);
+    TSG_HIP_TRY(
+        hipMemcpyAsync(&e.nt, e.tile_ptr + blkm, sizeof(int), hipMemcpyDeviceToHost, s));
+}
+
 void check_tile_count(int64_t n_c_tiles) {
     TSG_CHECK(n_c_tiles >= 0 && n_c_tiles <= INT_MAX, TSG_ERR_OVERFLOW,
               "the tile path indexes C tiles with int32: " + std::to_string(n_c_tiles) +
@@ -949,26 +1005,44 @@ void check_tile_count(int64_t n_c_tiles) {
 }
 
 void tile_symbolic(TilePlan &p, const CsrIn &A, const CsrIn &B, bool b_is_a, int64_t *d_ic,
-                   int64_t *nnz_c, Arena &ws, hipStream_t s, StageTimer &tm, int cus) {
+                   int64_t *nnz_c, Arena &ws, hipStream_t s, StageTimer &tm, int cus,
+                   const EarlyTileCount *earlyA, const EarlyTileCount *earlyB) {
     (void)cus;
     p = TilePlan();
     p.b_is_a = b_is_a;
     const int m = A.rows;
     tm.begin("tile.convert");
-    convert(p.A, A, false, ws, s);
+    convert(p.A, A, false, ws, s, earlyA);
     if (b_is_a) {
         // B shares A's structure (its column-major values are its own, below)
         p.B = p.A;
     } else {
-        convert(p.B, B, false, ws, s);
+        convert(p.B, B, false, ws, s, earlyB);
     }
-    // tile-CSC index of B: transpose of its tile pattern
+    // tile-CSC index of B: transpose of its tile pattern.  For C = A*A the
+    // pattern is first taken as symmetric (its transpose is itself, with the
+    // partner tiles' indices), which the C tile count's round trip confirms;
+    // otherwise, or when B is not A, a sort of the tiles by column.
     TileMat &TB = p.B;
-    TB.csc_ptr = ws.alloc_n<int>((size_t)TB.blkn + 1);
     TB.csc_row = ws.alloc_n<int>((size_t)TB.numblk + 1);
     TB.csc_tile = ws.alloc_n<int>((size_t)TB.numblk + 1);
-    csr_transpose(TB.blkm, TB.blkn, TB.tile_ptr, TB.tile_col, nullptr, TB.numblk, TB.csc_ptr,
-                  TB.csc_row, nullptr, ws, s, TB.csc_tile);
+    const bool try_sym = b_is_a && TB.blkm == TB.blkn && TB.numblk > 0;
+    unsigned *d_asym = nullptr;
+    auto transpose = [&] {
+        TB.csc_ptr = ws.alloc_n<int>((size_t)TB.blkn + 1);
+        csr_transpose(TB.blkm, TB.blkn, TB.tile_ptr, TB.tile_col, nullptr, TB.numblk,
+                      TB.csc_ptr, TB.csc_row, nullptr, ws, s, TB.csc_tile);
+    };
+    if (try_sym) {
+        d_asym = ws.alloc_n<unsigned>(1);
+        TSG_HIP_TRY(hipMemsetAsync(d_asym, 0, sizeof(unsigned), s));
+        TB.csc_ptr = TB.tile_ptr;
+        hipLaunchKernelGGL(k_sym_csc, dim3(grid_for((size_t)TB.blkm * 16, 256)), dim3(256), 0, s,
+                           TB.blkm, TB.tile_ptr, TB.tile_col, TB.csc_row, TB.csc_tile, d_asym);
+        TSG_LAUNCH_CHECK();
+    } else {
+        transpose();
+    }
     // B's column layout: column masks and starts, and every entry's position
     // in the column-major payload the numeric phase reads
     TB.rec = ws.alloc_n<TileRec>((size_t)TB.numblk + 1);
@@ -989,11 +1063,6 @@ void tile_symbolic(TilePlan &p, const CsrIn &A, const CsrIn &B, bool b_is_a, int
         hipLaunchKernelGGL(k_row_records, dim3(grid_for((size_t)p.A.numblk * 4, 256)), dim3(256),
                            0, s, p.A.numblk, p.A.tile_off, p.A.mask, p.A.rowptr, p.A.rec);
         TSG_LAUNCH_CHECK();
-    }
-    if (b_is_a) {
-        p.A.csc_ptr = TB.csc_ptr;
-        p.A.csc_row = TB.csc_row;
-        p.A.csc_tile = TB.csc_tile;
     }
     tm.end();
 
@@ -1023,9 +1092,18 @@ void tile_symbolic(TilePlan &p, const CsrIn &A, const CsrIn &B, bool b_is_a, int
     }
     exclusive_scan_i64(pslot, pslot, (size_t)blkm, tmp, s);
     int64_t nC64 = 0, nslots = 0;
+    unsigned h_asym = 0;
     TSG_HIP_TRY(hipMemcpyAsync(&nC64, d_nC, sizeof(int64_t), hipMemcpyDeviceToHost, s));
     TSG_HIP_TRY(hipMemcpyAsync(&nslots, pslot + blkm, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    if (d_asym)
+        TSG_HIP_TRY(hipMemcpyAsync(&h_asym, d_asym, sizeof(unsigned), hipMemcpyDeviceToHost, s));
     TSG_HIP_TRY(hipStreamSynchronize(s));
+    if (h_asym) transpose();
+    if (b_is_a) {
+        p.A.csc_ptr = TB.csc_ptr;
+        p.A.csc_row = TB.csc_row;
+        p.A.csc_tile = TB.csc_tile;
+    }
     check_tile_count(nC64);
     // slots when they stay within a few times the C tiles (a tile row of
     // many A tiles whose C tiles match few of them -- hypersparse inputs --

@@ -277,7 +277,8 @@ __global__ void __launch_bounds__(256) k_validate(int m, int n, const int *__res
 }
 
 int validate_csr_pair(const CsrIn &A, const CsrIn &B, Arena &ws, hipStream_t s,
-                      unsigned long long *density_sample) {
+                      unsigned long long *density_sample,
+                      const std::function<void()> &before_sync) {
     // six flag/count ints, then the two sums of the density sample
     unsigned long long *buf = ws.alloc_n<unsigned long long>(5);
     int *d = reinterpret_cast<int *>(buf);
@@ -297,6 +298,7 @@ int validate_csr_pair(const CsrIn &A, const CsrIn &B, Arena &ws, hipStream_t s,
                            B.cols, B.ptr, B.idx, B.nnz, d + 3);
         TSG_LAUNCH_CHECK();
     }
+    if (before_sync) before_sync();
     unsigned long long hb[5] = {0, 0, 0, 0, 0};
     TSG_HIP_TRY(hipMemcpyAsync(hb, buf, sizeof(hb), hipMemcpyDeviceToHost, s));
     TSG_HIP_TRY(hipStreamSynchronize(s));
