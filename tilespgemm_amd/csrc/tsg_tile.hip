@@ -258,6 +258,19 @@ __global__ void k_perm_scatter(int64_t nnz, const int *__restrict__ perm,
     if (p < nnz) out[perm[p]] = v[p];
 }
 
+// The same values into two layouts (C = A*A: A's row-major payload and B's
+// column-major one), read once.
+__global__ void k_perm_scatter2(int64_t nnz, const int *__restrict__ perm,
+                                const int *__restrict__ perm2, const double *__restrict__ v,
+                                double *__restrict__ out, double *__restrict__ out2) {
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < nnz) {
+        const double x = v[p];
+        out[perm[p]] = x;
+        out2[perm2[p]] = x;
+    }
+}
+
 // ------------------------------------------------------------------ step1
 constexpr int S1_WAVES = 4;
 constexpr int S1_WORDS = 2048;           // 65536 tile columns per pass
@@ -564,6 +577,8 @@ __device__ __forceinline__ int tile_units_in(int64_t nnz, int q) {
     return (q == 2 ? (int)(nnz >> 6) : 0) + (rem_bin(nnz) == q ? 1 : 0);
 }
 
+// Units per bin: summed over each wave first (one LDS atomic per wave and
+// bin; a thread's own atomics on the three counters had serialised).
 __global__ void __launch_bounds__(256) k_tbin_count(int n, const int64_t *__restrict__ off,
                                                     int *__restrict__ cnt) {
     __shared__ int s_cnt[TBINS];
@@ -573,8 +588,8 @@ __global__ void __launch_bounds__(256) k_tbin_count(int n, const int64_t *__rest
     const int64_t nz = t < n ? off[t + 1] - off[t] : 0;
 #pragma unroll
     for (int q = 0; q < TBINS; q++) {
-        const int u = tile_units_in(nz, q);
-        if (u) atomicAdd(&s_cnt[q], u);
+        const int u = wave_sum(tile_units_in(nz, q));
+        if (__lane_id() == 0 && u) atomicAdd(&s_cnt[q], u);
     }
     __syncthreads();
     if (threadIdx.x < TBINS && s_cnt[threadIdx.x]) atomicAdd(&cnt[threadIdx.x], s_cnt[threadIdx.x]);
@@ -1229,14 +1244,20 @@ void tile_numeric(TilePlan &p, const CsrIn &A, const CsrIn &B, const int64_t *d_
     TSG_CHECK(p.valid && p.A.rows == A.rows && p.A.cols == A.cols && p.B.cols == B.cols,
               TSG_ERR_STATE, "tile numeric phase without a matching symbolic phase");
     tm.begin("tile.values");
-    if (A.nnz > 0) {
+    // B's values go into its column-major payload (also when B is A: A's
+    // payload is row-major); when B is A with the same values, one pass
+    // writes both
+    const bool same_vals = p.b_is_a && A.val == B.val && A.nnz == B.nnz;
+    if (same_vals && A.nnz > 0) {
+        hipLaunchKernelGGL(k_perm_scatter2, dim3(grid_for((size_t)A.nnz, 256)), dim3(256), 0, s,
+                           A.nnz, p.A.perm, p.B.permc, A.val, p.A.val, p.B.valc);
+        TSG_LAUNCH_CHECK();
+    } else if (A.nnz > 0) {
         hipLaunchKernelGGL(k_perm_scatter, dim3(grid_for((size_t)A.nnz, 256)), dim3(256), 0, s,
                            A.nnz, p.A.perm, A.val, p.A.val);
         TSG_LAUNCH_CHECK();
     }
-    // B's values into its column-major payload (also when B is A: A's
-    // payload is row-major)
-    if (B.nnz > 0) {
+    if (!same_vals && B.nnz > 0) {
         hipLaunchKernelGGL(k_perm_scatter, dim3(grid_for((size_t)B.nnz, 256)), dim3(256), 0, s,
                            B.nnz, p.B.permc, B.val, p.B.valc);
         TSG_LAUNCH_CHECK();
