@@ -267,7 +267,10 @@ typedef struct {
                                     requests; 0 = 64 GiB                    */
     int method;                  /* TSG_METHOD_*                            */
     double tile_density;         /* tsg_set_tile_density, 0 = default       */
-    int bin1, bin2, bin3;        /* tsg_set_row_bins, bin3 = 0: defaults    */
+    int bin1, bin2, bin3;        /* tsg_set_row_bins; bin3 = 0: 64/256/512,
+                                    not the library's 64/256/1024 (mid-size
+                                    requests: rows of 513-1024 products run
+                                    faster on the big-row path)            */
 } tsg_server_config_t;
 
 typedef struct {

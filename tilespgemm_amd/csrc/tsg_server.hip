@@ -469,6 +469,17 @@ int tsg_server_create(tsg_server_t *server, int device, const tsg_server_config_
     s->device = device;
     s->cfg = *config;
     if (s->cfg.mem_budget == 0) s->cfg.mem_budget = size_t(64) << 30;
+    // a server's requests are many mid-size products, whose rows of 513 to
+    // 1024 products run faster on the big-row path than in the hash-capable
+    // small-row kernel (R-MAT 2^18 A*A^T with 4 draws per row 6.04 -> 5.43
+    // ms; the 512-product ER batch 67.3 -> 62.7 ms); the library default of
+    // 1024 suits single products of a million rows and more (R-MAT 2^20:
+    // 183 -> 187 ms with 512)
+    if (s->cfg.bin3 <= 0) {
+        s->cfg.bin1 = 64;
+        s->cfg.bin2 = 256;
+        s->cfg.bin3 = 512;
+    }
     int rc = tsg_create(&s->est, device);
     if (rc != TSG_OK) return rc;
     // the estimator's handle checks the tuning the workers will apply
