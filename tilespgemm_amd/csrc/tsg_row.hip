@@ -862,7 +862,9 @@ __global__ void __launch_bounds__(256) k_bptr_rows(int n, const int *__restrict_
 // in LDS once and every search runs there; otherwise the searches read jb.
 // The searches wait on their loads, so residency mattered with global
 // searches (a 128-block tile took 6.2 ms instead of 4.5 ms for R-MAT 2^20's
-// 16,385 x 2^20 table, a 32-block one 7.1 ms).
+// 16,385 x 2^20 table, a 32-block one 7.1 ms), and still does with staged
+// ones: blocks of 128 or 256 rows (512- or 1024-byte runs) took 6.6 and
+// 8.4 ms instead of 4.5.
 constexpr int BPT_ROWS = 64, BPT_BLK = 64, BPT_STAGE = 2048;
 
 __device__ __forceinline__ int lds_lower_bound(const int *a, int lo, int hi, int key) {
@@ -882,8 +884,7 @@ __global__ void __launch_bounds__(256) k_bptr(int n, const int *__restrict__ ib,
     __shared__ int s_rp[BPT_ROWS + 1];
     const int t0 = blockIdx.x * BPT_ROWS;
     const int nr = min(BPT_ROWS, n - t0);
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if ((int)threadIdx.x <= nr) s_rp[threadIdx.x] = ib[t0 + threadIdx.x];
+    for (int i = threadIdx.x; i <= nr; i += blockDim.x) s_rp[i] = ib[t0 + i];
     __syncthreads();
     const int e0 = s_rp[0];
     const bool staged = s_rp[nr] - e0 <= BPT_STAGE;
@@ -891,23 +892,24 @@ __global__ void __launch_bounds__(256) k_bptr(int n, const int *__restrict__ ib,
         for (int i = threadIdx.x; i < s_rp[nr] - e0; i += blockDim.x) s_col[i] = jb[e0 + i];
     __syncthreads();
     for (int b0 = 0; b0 <= nblk; b0 += BPT_BLK) {
-        for (int r = wave; r < nr; r += 4) {
+        // (row, block) pairs, a thread each: a row's blocks in consecutive lanes
+        for (int i = threadIdx.x; i < nr * BPT_BLK; i += blockDim.x) {
+            const int r = i / BPT_BLK, bb = i % BPT_BLK;
+            const int b = b0 + bb;
+            if (b > nblk) continue;
             const int lo = s_rp[r], hi = s_rp[r + 1];
-            for (int bb = lane; bb < BPT_BLK; bb += 64) {
-                const int b = b0 + bb;
-                if (b > nblk) continue;
-                int pos = hi;
-                if (b < nblk)
-                    pos = staged ? e0 + lds_lower_bound(s_col, lo - e0, hi - e0, b << bs)
-                                 : lower_bound_i32(jb, lo, hi, b << bs);
-                tile[bb][r] = pos;
-            }
+            int pos = hi;
+            if (b < nblk)
+                pos = staged ? e0 + lds_lower_bound(s_col, lo - e0, hi - e0, b << bs)
+                             : lower_bound_i32(jb, lo, hi, b << bs);
+            tile[bb][r] = pos;
         }
         __syncthreads();
-        for (int bb = wave; bb < BPT_BLK; bb += 4) {
+        // a block's rows in consecutive lanes: runs of 4 * BPT_ROWS bytes
+        for (int i = threadIdx.x; i < BPT_BLK * BPT_ROWS; i += blockDim.x) {
+            const int bb = i / BPT_ROWS, r = i % BPT_ROWS;
             const int b = b0 + bb;
-            const int t = t0 + lane;
-            if (b <= nblk && lane < nr) bptr[(size_t)b * n + t] = tile[bb][lane];
+            if (b <= nblk && r < nr) bptr[(size_t)b * n + t0 + r] = tile[bb][r];
         }
         __syncthreads();
     }
