@@ -123,6 +123,17 @@ struct RowTuning {
     int64_t bptr_budget = -1;    // bytes of the B block-pointer table; < 0: 6 GiB
 };
 
+// One big-row item: d = {row, first column, end column, first sub-block in
+// subocc (rank mode) or big-row index}, the position of its first output in
+// C, and of its first word in the band word lists (rank mode).  One record,
+// so that gathering the items into column order reads one line per item.
+struct ItemRec {
+    int4 d;
+    int64_t off;
+    int64_t wpos;
+};
+static_assert(sizeof(ItemRec) == 32, "item records are 32 bytes");
+
 struct RowPlan {
     bool valid = false;
     bool b_strict = false;           // B rows strictly ascending (no duplicates)
@@ -160,7 +171,7 @@ struct RowPlan {
     int64_t nitems_small = 0;        // items of <= 256 outputs, first in items (column order)
     int64_t nitems_mid = 0;          // end of the items of <= 512 outputs, next
     int64_t *item_cnt = nullptr;     // [nbig+1] items per big row -> offsets
-    int4 *items = nullptr;           // {row, c0, c1, big-row index}, by c0
+    ItemRec *items = nullptr;        // by c0 (column order) or by row
     bool narrow_items = false;       // items of <= 8192 columns (else <= 16384)
     // big rows' column bitmaps kept from the symbolic phase (rank mode), in
     // sparse form: per sub-block a mask of its non-empty words, and per
@@ -170,9 +181,9 @@ struct RowPlan {
     int64_t words_cap = 0;           // words of bandwords (a bound)
     unsigned *bandwords = nullptr;   // [words_cap]
     unsigned *subocc = nullptr;      // [nsb_total] non-empty-word masks
-    int64_t *item_words = nullptr;   // per item: position of its first word
+
     int4 *bpk = nullptr;             // [nnz(B)] {column, 0, value} records (rank mode)
-    int64_t *item_off = nullptr;     // output offset (within C) per item
+
     // B with rows sorted, when the input's were not (set after row_symbolic):
     // the plan ran on (B.ptr, b_idx); numeric phases gather values by b_perm
     bool b_canon = false;

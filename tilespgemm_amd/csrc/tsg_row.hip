@@ -923,15 +923,17 @@ __global__ void __launch_bounds__(256) k_bptr(int n, const int *__restrict__ ib,
 // One wave per big row, one lane per window (64 windows per pass): a lane
 // loads its window's counts at once and splits them in registers, and wave
 // scans place the items and their outputs.  FILL=false counts items per
-// row; FILL=true (item_cnt = exclusive item offsets) writes {row, c0, c1,
-// big-row index}, each item's first output position and output count, and,
-// when item_words is given, where its first non-empty word is in the band
-// word lists of the symbolic phase (from the sub-blocks' masks of non-empty
-// words); the item's fourth field then holds its first sub-block's index in
-// subcnt and subocc (< 2^31 in that mode) instead of the big-row index, so
-// the numeric phase finds the item's masks with no other lookup.  Windows lie inside bands, and a band's windows inside one pass of
-// 64 windows, so the item's place in its band's list is its count of
-// non-empty words before it less that of the band's first window.
+// row; FILL=true (item_cnt = exclusive item offsets) writes each item's
+// record -- {row, c0, c1, big-row index}, its first output position -- and
+// output count, and, in rank mode (band_base given), where its first
+// non-empty word is in the band word lists of the symbolic phase (from the
+// sub-blocks' masks of non-empty words); the record's fourth field then
+// holds the item's first sub-block's index in subcnt and subocc (< 2^31 in
+// that mode) instead of the big-row index, so the numeric phase finds the
+// item's masks with no other lookup.  Windows lie inside bands, and a
+// band's windows inside one pass of 64 windows, so the item's place in its
+// band's list is its count of non-empty words before it less that of the
+// band's first window.
 template <bool FILL, int WSB>
 __global__ void __launch_bounds__(256) k_items(int nbig, const int *__restrict__ rows, int k,
                                                const int *__restrict__ span_sb0,
@@ -939,16 +941,15 @@ __global__ void __launch_bounds__(256) k_items(int nbig, const int *__restrict__
                                                const uint16_t *__restrict__ subcnt,
                                                const int64_t *__restrict__ ic,
                                                int64_t *__restrict__ item_cnt,
-                                               int4 *__restrict__ items,
-                                               int64_t *__restrict__ item_off,
+                                               ItemRec *__restrict__ items,
                                                int *__restrict__ item_n,
                                                const unsigned *__restrict__ subocc,
                                                const int64_t *__restrict__ band_off,
-                                               const int64_t *__restrict__ band_base,
-                                               int64_t *__restrict__ item_words) {
+                                               const int64_t *__restrict__ band_base) {
     static_assert(WSB <= (1 << (ITEM_WBITS_SHIFT - SUB_SHIFT)), "item wider than its LDS");
     constexpr int WPB = (1 << BAND_SHIFT) / WSB;   // windows per band
     static_assert(WPB * WSB == (1 << BAND_SHIFT) && 64 % WPB == 0, "windows tile the bands");
+    const bool words = band_base != nullptr;   // rank mode: word-list positions
     const int r = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int l = threadIdx.x & 63;
     if (r >= nbig) return;
@@ -968,7 +969,7 @@ __global__ void __launch_bounds__(256) k_items(int nbig, const int *__restrict__
 #pragma unroll
         for (int q = 0; q < WSB; q++) {
             cs[q] = q < len ? (int)sc[j0 + q] : 0;
-            os[q] = FILL && item_words && q < len ? __popc(subocc[sb_off[r] + j0 + q]) : 0;
+            os[q] = FILL && words && q < len ? __popc(subocc[sb_off[r] + j0 + q]) : 0;
             nzt += os[q];
         }
         int nit = 0, tot = 0;
@@ -1001,17 +1002,16 @@ __global__ void __launch_bounds__(256) k_items(int nbig, const int *__restrict__
             const int cbase = first + j0;
             // this window's first non-empty word in the band word lists
             const int64_t wb =
-                item_words ? band_base[band_off[r] + win / WPB] + (nz_excl - band_nz) : 0;
+                words ? band_base[band_off[r] + win / WPB] + (nz_excl - band_nz) : 0;
             int start = -1, cnt = 0, nzrun = 0, nzs = 0;
 #pragma unroll
             for (int q = 0; q < WSB; q++) {
                 if (start >= 0 && cnt + cs[q] > ITEM_CAP) {
-                    items[w] = make_int4(row, (cbase + start) << SUB_SHIFT,
-                                         min(k, (cbase + q) << SUB_SHIFT),
-                                         item_words ? (int)(sb_off[r] + j0 + start) : r);
-                    item_off[w] = out;
+                    items[w] = ItemRec{make_int4(row, (cbase + start) << SUB_SHIFT,
+                                                 min(k, (cbase + q) << SUB_SHIFT),
+                                                 words ? (int)(sb_off[r] + j0 + start) : r),
+                                       out, wb + nzs};
                     item_n[w] = cnt;
-                    if (item_words) item_words[w] = wb + nzs;
                     w++;
                     out += cnt;
                     start = -1;
@@ -1027,12 +1027,11 @@ __global__ void __launch_bounds__(256) k_items(int nbig, const int *__restrict__
                 nzrun += os[q];
             }
             if (start >= 0 && cnt > 0) {
-                items[w] = make_int4(row, (cbase + start) << SUB_SHIFT,
-                                     min(k, (cbase + len) << SUB_SHIFT),
-                                     item_words ? (int)(sb_off[r] + j0 + start) : r);
-                item_off[w] = out;
+                items[w] = ItemRec{make_int4(row, (cbase + start) << SUB_SHIFT,
+                                             min(k, (cbase + len) << SUB_SHIFT),
+                                             words ? (int)(sb_off[r] + j0 + start) : r),
+                                   out, wb + nzs};
                 item_n[w] = cnt;
-                if (item_words) item_words[w] = wb + nzs;
             }
         }
         wbase += group_last<64>(it_incl);
@@ -1072,7 +1071,7 @@ constexpr int SMALL_MINW = 6;
 constexpr int MID_RANK_U = 6;
 constexpr int MID_MINW = 5;
 
-__global__ void __launch_bounds__(256) k_item_keys(int64_t n, const int4 *__restrict__ items,
+__global__ void __launch_bounds__(256) k_item_keys(int64_t n, const ItemRec *__restrict__ items,
                                                    const int *__restrict__ item_n, int class_bit,
                                                    int *__restrict__ keys,
                                                    int *__restrict__ block_small,
@@ -1081,7 +1080,7 @@ __global__ void __launch_bounds__(256) k_item_keys(int64_t n, const int4 *__rest
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int cls = 0;
     if (i < n && class_bit >= 0) cls = item_n[i] <= SMALL_ITEM ? 0 : item_n[i] <= MID_ITEM ? 1 : 2;
-    if (i < n) keys[i] = (cls << max(class_bit, 0)) | (items[i].y >> SUB_SHIFT);
+    if (i < n) keys[i] = (cls << max(class_bit, 0)) | (items[i].d.y >> SUB_SHIFT);
     const unsigned long long b0 = __ballot(i < n && class_bit >= 0 && cls == 0);
     const unsigned long long b1 = __ballot(i < n && class_bit >= 0 && cls == 1);
     if (__lane_id() == 0) {
@@ -1096,19 +1095,10 @@ __global__ void __launch_bounds__(256) k_item_keys(int64_t n, const int4 *__rest
 }
 
 __global__ void __launch_bounds__(256) k_gather_items(int64_t n, const int *__restrict__ order,
-                                                      const int4 *__restrict__ items,
-                                                      const int64_t *__restrict__ off,
-                                                      const int64_t *__restrict__ cols,
-                                                      int4 *__restrict__ items_out,
-                                                      int64_t *__restrict__ off_out,
-                                                      int64_t *__restrict__ cols_out) {
+                                                      const ItemRec *__restrict__ items,
+                                                      ItemRec *__restrict__ items_out) {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < n) {
-        const int i = order[j];
-        items_out[j] = items[i];
-        off_out[j] = off[i];
-        if (cols) cols_out[j] = cols[i];
-    }
+    if (j < n) items_out[j] = items[order[j]];
 }
 
 // --------------------------------------------------------- big rows: numeric
@@ -1128,7 +1118,7 @@ struct ItemSmem {
 
 template <bool FUSED>
 __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric(
-    int64_t nitems, const int4 *__restrict__ items, const int64_t *__restrict__ item_off,
+    int64_t nitems, const ItemRec *__restrict__ items,
     const int *__restrict__ ia, const int *__restrict__ ja, const double *__restrict__ a,
     const int *__restrict__ ib, const int *__restrict__ jb, const double *__restrict__ b,
     BlockPtrs bp, int *__restrict__ jc,
@@ -1141,7 +1131,7 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric(
     const int64_t wave_id = (int64_t)blockIdx.x * ITEM_WAVES + (threadIdx.x >> 6);
     const int64_t nwaves = (int64_t)gridDim.x * ITEM_WAVES;
     for (int64_t it = wave_id; it < nitems; it += nwaves) {
-        const int4 item = items[it];
+        const int4 item = items[it].d;
         const int row = item.x, c0 = item.y, c1 = item.z;
         const int nwords = (c1 - c0 + 31) >> 5;
         for (int i = l; i < ITEM_SLOTS; i += 64) S.keys[i] = -1;
@@ -1170,7 +1160,7 @@ __global__ void __launch_bounds__(64 * ITEM_WAVES) k_big_numeric(
             });
         wave_lds_fence();
         // emit in column order: bitmap words -> ranks -> hash lookups
-        int64_t o = item_off[it];
+        int64_t o = items[it].off;
         for (int w0 = 0; w0 < nwords; w0 += 64) {
             const int w = w0 + l;
             unsigned word = w < nwords ? S.bits[w] : 0u;
@@ -1318,8 +1308,7 @@ __device__ __forceinline__ void item_accumulate(RankSmem<WORDS, CAP> &S, int c0,
 template <bool FUSED, int WORDS, int CAP = ITEM_CAP, int U = RANK_U, int MINW = 1>
 __global__ void __launch_bounds__(64 * ITEM_WAVES) __attribute__((amdgpu_waves_per_eu(MINW)))
 k_big_numeric_rank(
-    int64_t nitems, const int4 *__restrict__ items, const int64_t *__restrict__ item_off,
-    const int64_t *__restrict__ item_wpos, const unsigned *__restrict__ wlist,
+    int64_t nitems, const ItemRec *__restrict__ items, const unsigned *__restrict__ wlist,
     const unsigned *__restrict__ subocc, const int *__restrict__ ia, const int *__restrict__ ja,
     const double *__restrict__ a, const int *__restrict__ jb, const double *__restrict__ b,
     const int4 *__restrict__ bpk, BlockPtrs bp, int *__restrict__ jc, double *__restrict__ c,
@@ -1348,17 +1337,17 @@ k_big_numeric_rank(
         if (l < min(nsb, 2 * (WORDS / 64)))
             src = subocc + ((int64_t)item.w + l);
         else if (l == 16 || l == 17)
-            src = reinterpret_cast<const unsigned *>(item_wpos + at) + (l - 16);
+            src = reinterpret_cast<const unsigned *>(&items[at].wpos) + (l - 16);
         else if (l == 18 || l == 19)
             src = reinterpret_cast<const unsigned *>(ia + item.x) + (l - 18);
         else if (l == 20 || l == 21)
-            src = reinterpret_cast<const unsigned *>(item_off + at) + (l - 20);
+            src = reinterpret_cast<const unsigned *>(&items[at].off) + (l - 20);
         nx = src ? *src : 0u;
     };
     // (streaming loads of the item descriptors and A entries, read once per
     // item, were measured slower: 158 -> 166 ms on R-MAT 2^20 A*A^T)
     if (wave_id < nitems) {
-        item = items[wave_id];
+        item = items[wave_id].d;
         locate(wave_id);
     }
     for (int64_t it = wave_id; it < nitems; it += nwaves) {
@@ -1375,7 +1364,7 @@ k_big_numeric_rank(
             (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)nx, 21) << 32) |
                       (uint32_t)__builtin_amdgcn_readlane((int)nx, 20));
         const int64_t next = it + nwaves;
-        if (next < nitems) item = items[next];
+        if (next < nitems) item = items[next].d;
         // the product pipeline of for_products, with the rank table built
         // after the first groups' A entries and B ranges are issued, so the
         // word loads and that chain of dependent loads overlap
@@ -1730,17 +1719,16 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
         // windows otherwise (fewer items, fewer (A entry, item) pairs).
         const bool narrow = p.column_order;
         p.narrow_items = narrow;
-        auto items_kernel = [&](auto kern, int64_t *cnt, int4 *it, int64_t *off, int *itn,
-                                int64_t *iwords) {
+        auto items_kernel = [&](auto kern, int64_t *cnt, ItemRec *it, int *itn) {
             hipLaunchKernelGGL(kern, dim3(grid_for((size_t)p.nbig * 64, 256)), dim3(256), 0, s,
                                p.nbig, rowsb, k, p.span_sb0, p.sb_off, p.subcnt, d_ic, cnt, it,
-                               off, itn, p.subocc, band_off, band_base, iwords);
+                               itn, p.subocc, band_off, band_base);
             TSG_LAUNCH_CHECK();
         };
         if (narrow)
-            items_kernel(k_items<false, 8>, p.item_cnt, nullptr, nullptr, nullptr, nullptr);
+            items_kernel(k_items<false, 8>, p.item_cnt, nullptr, nullptr);
         else
-            items_kernel(k_items<false, 16>, p.item_cnt, nullptr, nullptr, nullptr, nullptr);
+            items_kernel(k_items<false, 16>, p.item_cnt, nullptr, nullptr);
         exclusive_scan_i64(p.item_cnt, p.item_cnt, (size_t)p.nbig, tmp2, s);
         unsigned h_overflow = 0;
         TSG_HIP_TRY(hipMemcpyAsync(&h_tot[1], p.item_cnt + p.nbig, sizeof(int64_t),
@@ -1754,14 +1742,12 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
         // the symbolic phase itself went wrong
         TSG_CHECK(h_overflow == 0, TSG_ERR_OVERFLOW, "big-row word lists exceed their bound");
         p.nitems = h_tot[1];
-        p.items = ws.alloc_n<int4>((size_t)p.nitems + 1);
-        p.item_off = ws.alloc_n<int64_t>((size_t)p.nitems + 1);
-        p.item_words = p.use_rank ? ws.alloc_n<int64_t>((size_t)p.nitems + 1) : nullptr;
+        p.items = ws.alloc_n<ItemRec>((size_t)p.nitems + 1);
         int *item_n = ws.alloc_n<int>((size_t)p.nitems + 1);
         if (narrow)
-            items_kernel(k_items<true, 8>, p.item_cnt, p.items, p.item_off, item_n, p.item_words);
+            items_kernel(k_items<true, 8>, p.item_cnt, p.items, item_n);
         else
-            items_kernel(k_items<true, 16>, p.item_cnt, p.items, p.item_off, item_n, p.item_words);
+            items_kernel(k_items<true, 16>, p.item_cnt, p.items, item_n);
         p.nitems_small = 0;
         p.nitems_mid = 0;
         if (p.column_order && p.nitems > 1 && p.nitems < (int64_t(1) << 31)) {
@@ -1782,16 +1768,11 @@ void row_symbolic(RowPlan &p, const CsrIn &A, const CsrIn &B, int64_t *d_ic, int
             exclusive_scan_i32(bsmall, bsmall, (size_t)kblocks, tmpk, s, d_small);
             exclusive_scan_i32(bmid, bmid, (size_t)kblocks, tmpk, s, d_small + 1);
             sort_order_i32(keys, p.nitems, p.use_rank ? end_bit + 2 : end_bit, order, ws, s);
-            int4 *items_s = ws.alloc_n<int4>(ni + 1);
-            int64_t *off_s = ws.alloc_n<int64_t>(ni + 1);
-            int64_t *words_s = p.item_words ? ws.alloc_n<int64_t>(ni + 1) : nullptr;
+            ItemRec *items_s = ws.alloc_n<ItemRec>(ni + 1);
             hipLaunchKernelGGL(k_gather_items, dim3(grid_for(ni, 256)), dim3(256), 0, s,
-                               p.nitems, order, p.items, p.item_off, p.item_words, items_s, off_s,
-                               words_s);
+                               p.nitems, order, p.items, items_s);
             TSG_LAUNCH_CHECK();
             p.items = items_s;
-            p.item_off = off_s;
-            p.item_words = words_s;
             if (p.use_rank) {
                 int64_t h_cls[2] = {0, 0};
                 TSG_HIP_TRY(hipMemcpyAsync(h_cls, d_small, sizeof(h_cls), hipMemcpyDeviceToHost,
@@ -1868,8 +1849,7 @@ static void row_numeric_t(RowPlan &p, const CsrIn &A, const CsrIn &B, const int6
             set_smem(kern, sm);
             const unsigned grid = item_grid(i1 - i0);
             hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * ITEM_WAVES), sm, s, i1 - i0,
-                               p.items + i0, p.item_off + i0, p.item_words + i0, p.bandwords,
-                               p.subocc, A.ptr, A.idx, A.val, B.idx, B.val, p.bpk, block_ptrs(p, B.ptr),
+                               p.items + i0, p.bandwords, p.subocc, A.ptr, A.idx, A.val, B.idx, B.val, p.bpk, block_ptrs(p, B.ptr),
                                d_jc, d_c, (int)p.b_strict);
             TSG_LAUNCH_CHECK();
         };
@@ -1901,7 +1881,7 @@ static void row_numeric_t(RowPlan &p, const CsrIn &A, const CsrIn &B, const int6
         set_smem(kern, sm);
         const unsigned grid = item_grid(p.nitems);
         hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * ITEM_WAVES), sm, s, p.nitems, p.items,
-                           p.item_off, A.ptr, A.idx, A.val, B.ptr, B.idx, B.val, block_ptrs(p, B.ptr),
+                           A.ptr, A.idx, A.val, B.ptr, B.idx, B.val, block_ptrs(p, B.ptr),
                            d_jc, d_c, (int)p.b_strict);
         TSG_LAUNCH_CHECK();
         tm.end();
