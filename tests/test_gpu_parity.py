@@ -393,27 +393,34 @@ def test_unsorted_input_goes_to_row_path(handle):
     _ = rng
 
 
-def test_invalid_input_is_rejected(handle):
+@pytest.mark.parametrize("mname,method", [("auto", tsg.METHOD_AUTO), ("tile", tsg.METHOD_TILE)])
+def test_invalid_input_is_rejected(handle, mname, method):
+    """Malformed A is refused.  AUTO's tile-density sample, and with the tile
+    path forced the tile counts of A and B, run beside the checks and must
+    stay in bounds whatever the arrays hold."""
     A = gen.erdos_renyi(50, 4.0, seed=1)
     bad = (A[0], A[1].copy(), A[2])
     bad[1][3] = 50                            # column out of range
     with pytest.raises(tsg.TsgError):
-        run(handle, 50, 50, 50, bad, A, tsg.METHOD_AUTO)
+        run(handle, 50, 50, 50, bad, A, method)
     badp = (A[0].copy(), A[1], A[2])
     badp[0][10] = badp[0][11] + 5             # non-monotone row pointers
     with pytest.raises(tsg.TsgError):
-        run(handle, 50, 50, 50, badp, A, tsg.METHOD_AUTO)
-    # far out of range: AUTO's tile-density sample runs beside the checks and
-    # must stay in bounds whatever the arrays hold
+        run(handle, 50, 50, 50, badp, A, method)
+    # far out of range
     for row, v in [(5, 10**9), (7, -5)]:
         badq = (A[0].copy(), A[1], A[2])
         badq[0][row] = v
         with pytest.raises(tsg.TsgError):
-            run(handle, 50, 50, 50, badq, A, tsg.METHOD_AUTO)
+            run(handle, 50, 50, 50, badq, A, method)
     badc = (A[0], A[1].copy(), A[2])
     badc[1][0], badc[1][1] = -1, 2**31 - 1
     with pytest.raises(tsg.TsgError):
-        run(handle, 50, 50, 50, badc, A, tsg.METHOD_AUTO)
+        run(handle, 50, 50, 50, badc, A, method)
+    # the handle still works afterwards
+    got = run(handle, 50, 50, 50, A, A, method)
+    handle.set_method(tsg.METHOD_AUTO)
+    assert_csr_equal(got, O.csr_spgemm(50, 50, A, A, fused=True))
 
 
 def test_transpose_matches_host(handle):
