@@ -436,7 +436,7 @@ int tsg_spgemm_noalloc(tsg_handle_t h, int m, int n, int k, const int *d_ia, con
         StageTimer tm;   // disabled: records no events
         tm.stream = h->stream;
         RowTuning tune = h->row_tune;
-        // span bitmaps and block pointers must fit beside the rest
+        // the big rows' bitmaps and the block pointers must fit beside the rest
         if (tune.bitmap_budget < 0 || (size_t)tune.bitmap_budget > cap / 4)
             tune.bitmap_budget = (int64_t)(cap / 4);
         if (tune.bptr_budget < 0 || (size_t)tune.bptr_budget > cap / 8)

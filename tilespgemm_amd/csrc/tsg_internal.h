@@ -119,7 +119,7 @@ struct StageTimer {
 struct RowTuning {
     int bin1 = 64, bin2 = 256, bin3 = 1024;
     int64_t item_sort_min = int64_t(1) << 18;  // big-row items column-ordered from this many
-    int64_t bitmap_budget = -1;  // bytes of big-row column lists kept; < 0: min(48 GiB, 40% of free)
+    int64_t bitmap_budget = -1;  // bytes of big-row bitmaps (sparse form) kept; < 0: min(48 GiB, 40% of free)
     int64_t bptr_budget = -1;    // bytes of the B block-pointer table; < 0: 6 GiB
 };
 
