@@ -662,16 +662,22 @@ __global__ void __launch_bounds__(256) k_sym_csc(int blk, const int *__restrict_
                                                  unsigned *__restrict__ asym) {
     const int J = (blockIdx.x * blockDim.x + threadIdx.x) >> 4;
     const int l = threadIdx.x & 15;
-    if (J >= blk) return;
-    for (int q = tile_ptr[J] + l; q < tile_ptr[J + 1]; q += 16) {
-        const int K = tile_col[q];
-        const int lo = tile_ptr[K], hi = tile_ptr[K + 1];
-        const int pos = lower_bound_i32(tile_col, lo, hi, J);
-        const bool found = pos < hi && tile_col[pos] == J;
-        csc_row[q] = K;
-        csc_tile[q] = found ? pos : 0;
-        if (!found) atomicOr(asym, 1u);
+    bool missing = false;
+    if (J < blk) {
+        for (int q = tile_ptr[J] + l; q < tile_ptr[J + 1]; q += 16) {
+            const int K = tile_col[q];
+            const int lo = tile_ptr[K], hi = tile_ptr[K + 1];
+            const int pos = lower_bound_i32(tile_col, lo, hi, J);
+            const bool found = pos < hi && tile_col[pos] == J;
+            csc_row[q] = K;
+            csc_tile[q] = found ? pos : 0;
+            missing |= !found;
+        }
     }
+    // one plain store per wave that found a tile without its partner (an
+    // asymmetric pattern has millions: atomics on one word would queue)
+    const unsigned long long bad = __ballot(missing);
+    if (bad && __lane_id() == __ffsll(bad) - 1) *asym = 1u;
 }
 
 // Column layout of 16x16 tiles (B only), the transposed sub-tiles of
